@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Throughput of the explicit DG advection step on one MI355X.
+
+Workload (the drop-in GPU step): domainSquare hmax = 2^-9 (K = 524 288
+triangles), p = 2 (N = 6), FP64, rotation velocity plus a seeded 16-mode
+divergence-free perturbation (amplitude 0.05), 8 seeded Gaussian humps,
+zero inflow, SSP-RK3, Taylor-linear vertex-based limiter on (dc/dt and
+solution limiting), time step tau = 0.1 h / (max|u| (2p+1)).
+
+A "step" is one full time step (3 RK stages, all kernels), run from a
+captured hipGraph with every input resident in HBM.  value = K N S steps /
+wall time (DoF-stage updates per second).
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes per
+launch / average launch time from HIP events on the problem's stream), a
+per-kernel breakdown, and the oracle (the numpy/scipy restatement of the
+reference's sparse-assembly path) timed on a bounded sample on the host.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 10]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = 'DoF-stage updates/s at p=2 + % HBM roofline; batched & adjoint vs plain'
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(kind, p, K, numV, lim, ordRK):
+    """Algorithmic HBM bytes of all launches of one kernel kind in one time
+    step (each datum read or written once; neighbour and vertex gathers count
+    once per element / vertex)."""
+    N = (p + 1) * (p + 2) // 2
+    Re = p + 1
+    omega = {1: [0], 2: [0, 0.5], 3: [0, 0.75, 1 / 3], 4: [1, 1, 1, 0]}[ordRK]
+    S = len(omega)
+    tot = 0.0
+    for s in range(S):
+        w = omega[s] != 0
+        if kind == 'stage':
+            if lim and p >= 2:       # cdot -> Taylor: c, u1, u2, vn, B, |E|, nbr, flags -> cdotT, centroid
+                tot += K * (8 * (N + 2 * N + 3 * Re + 4 + 3 + N + 1) + 4 * 4)
+            elif lim:                # RK -> Taylor
+                tot += K * (8 * (N + w * N + 2 * N + 3 * Re + 7 + N + 1) + 16)
+            else:                    # RK -> DG
+                tot += K * (8 * (N + w * N + 2 * N + 3 * Re + 7 + N) + 16)
+        elif kind == 'vertex' and lim:
+            nv = 2 if p >= 2 else 1
+            tot += nv * (4 * (numV + 1) + 4 * 3 * K + 8 * K + 16 * numV)
+        elif kind == 'limit_dot' and lim and p >= 2:
+            tot += K * (8 * (N + 4 + N + w * N + N + 1) + 12) + 16 * numV
+        elif kind == 'limit' and lim:
+            tot += K * (8 * (N + 4 + N) + 12) + 16 * numV
+    return tot
+
+
+def cpu_baseline(hmax, p, ordRK, lim, tc_factory):
+    """The oracle (numpy/scipy restatement of the reference's per-stage
+    sparse assembly, SpMV, block solves and limiter) on one thread, one time
+    step on a mesh of the same family; returns DoF-stage updates/s."""
+    from threadpoolctl import threadpool_limits
+    from oracle import advection as OA
+    tc = tc_factory()
+    cfg = OA.Struct(testcase='custom', p=p, hmax=hmax, ordRK=ordRK, isSlopeLim=lim, numSteps=100)
+    for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):
+        cfg[k] = tc[k]
+    with threadpool_limits(1):
+        pd = OA.main('advection', cfg, maxSteps=1)
+    N = (p + 1) * (p + 2) // 2
+    S = {1: 1, 2: 2, 3: 3, 4: 4}[ordRK]
+    return pd.K * N * S / pd.timeLoop, pd.K, pd.timeLoop
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=100)
+    ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--hmax', type=float, default=2.0 ** -9)
+    ap.add_argument('--p', type=int, default=2)
+    ap.add_argument('--ordRK', type=int, default=3)
+    ap.add_argument('--no-limiter', action='store_true')
+    ap.add_argument('--kernel-steps', type=int, default=5, help='eager steps timed per kernel')
+    ap.add_argument('--cpu-hmax', type=float, default=2.0 ** -8, help='mesh of the CPU sample')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print('bench.py measures one GPU only: the path runs on a single MI355X; '
+              'multi-GPU sharding is described in DESIGN.md but not run.', file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    if not torch.cuda.is_available():
+        print('no GPU visible', file=sys.stderr)
+        sys.exit(1)
+    from festung_amd.advection import steps as S
+    from festung_amd.advection import testcases as TC
+
+    lim = not a.no_limiter
+    p, ordRK = a.p, a.ordRK
+    t_setup = time.perf_counter()
+    pd = S.Struct(testcase='rotating_humps', p=p, hmax=a.hmax, ordRK=ordRK, isSlopeLim=lim,
+                  numSteps=a.steps, useGraph=True)
+    pd = S.configureProblem(pd)
+    # tau = CFL 0.1 h / (max|u| (2p+1)), max|u| over the vertices
+    g_dim = int(np.ceil(1.0 / a.hmax))
+    pd.g = pd.generateGridData(pd.hmax)
+    xy = pd.g.coordV
+    umax = float(np.max(np.hypot(pd.u1Cont(0, xy[:, 0], xy[:, 1]), pd.u2Cont(0, xy[:, 0], xy[:, 1]))))
+    tau = 0.1 * (1.0 / g_dim) / (umax * (2 * p + 1))
+    pd.tEnd = tau * a.steps
+    pd.generateGridData = lambda h: pd.g
+    pd = S.preprocessProblem(pd)
+    pd = S.initializeProblem(pd)
+    pd = S.solveStep(pd, 1)          # first step sets the device velocity
+    prob = pd.problem
+    K, N, numV = prob.K, prob.N, pd.g.numV
+    nstage = {1: 1, 2: 2, 3: 3, 4: 4}[ordRK]
+    prob.step(a.warmup, tau)
+    prob.sync()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+
+    # timed region: K steps from the captured graph, events on the problem's stream
+    stream = torch.cuda.ExternalStream(prob.stream)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    prob.step(a.steps, tau)
+    e1.record(stream)
+    prob.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1)
+    state = prob.get_state()
+    if not np.all(np.isfinite(state)):
+        print('non-finite state after the timed steps', file=sys.stderr)
+        sys.exit(1)
+
+    # per-kernel timing pass (eager launches bracketed by HIP events)
+    prob.profile(True)
+    prob.step(a.kernel_steps, tau)
+    kt = prob.profile_read()
+    prob.profile(False)
+    kernels = {}
+    for kind, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        per_step_bytes = kernel_bytes(kind, p, K, numV, lim, ordRK)
+        launches_per_step = cnt / a.kernel_steps
+        avg_ms = ms / cnt
+        bytes_per_launch = per_step_bytes / launches_per_step if launches_per_step else 0.0
+        kernels[kind] = dict(launches_per_step=launches_per_step, avg_us=1e3 * avg_ms,
+                             bytes_per_launch=bytes_per_launch,
+                             GBs=bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0,
+                             share=0.0)
+    tot_ms = sum(v['avg_us'] * v['launches_per_step'] for v in kernels.values())
+    for v in kernels.values():
+        v['share'] = v['avg_us'] * v['launches_per_step'] / tot_ms
+    dom = max(kernels, key=lambda k: kernels[k]['share'])
+    step_bytes = sum(kernel_bytes(k, p, K, numV, lim, ordRK) for k in ('stage', 'vertex', 'limit_dot', 'limit'))
+    secs = wall
+    value = K * N * nstage * a.steps / secs
+    roof = dict(bound='hbm', kernel=dom, achieved=kernels[dom]['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
+                frac=kernels[dom]['GBs'] / HBM_PEAK_GBS, traffic=None,
+                step_GBs=step_bytes / (secs / a.steps) / 1e9,
+                step_frac=step_bytes / (secs / a.steps) / 1e9 / HBM_PEAK_GBS)
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        v, Kc, tl = cpu_baseline(a.cpu_hmax, p, ordRK, lim, TC.rotating_humps)
+        cpu = dict(value=v, unit='DoF-stage updates/s', cores=1, kind='port',
+                   sample='oracle (numpy/scipy reference-structured restatement: per-stage sparse '
+                          'kron/kronVec assembly, SpMV, block solves, limiter), 1 thread, 1 time step '
+                          '(%d stages) on domainSquare hmax=%g (K=%d), same p/limiter/closures; loop '
+                          'time %.1f s, preprocessing excluded' % (nstage, a.cpu_hmax, Kc, tl))
+
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'DoF-stage updates/s', 'n_gpus': 1,
+        'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': 1e3 * secs / a.steps,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': 'synthetic',
+        'config': {'workload': 'advection drop-in step: domainSquare hmax=2^%g K=%d p=%d N=%d SSP-RK%d '
+                               'limiter=%s, rotation+16-mode stream function, 8 Gaussian humps'
+                               % (np.log2(a.hmax), K, p, N, ordRK, 'on' if lim else 'off'),
+                   'K': K, 'p': p, 'N': N, 'ordRK': ordRK, 'limiter': lim, 'tau': tau,
+                   'stages_per_step': nstage, 'event_ms_per_step': ev_ms / a.steps,
+                   'setup_s': t_setup},
+        'roofline': roof,
+        'kernels': kernels,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,255 @@
+"""Python handles over the C-ABI: ``Grid`` (host-side mesh, the struct ``g``
+of core/generateGridData.m) and ``Problem`` (device-resident advection
+state and operators).
+
+Arrays use the reference's Matlab shapes (e.g. cDisc is K x N) and are
+exchanged in column-major (Fortran) order, which is the memory layout the
+C-ABI expects, so no transposes are made.  Ids are 0-based.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib, ptr
+
+# shapes of the grid fields as functions of (numT, numV, numE)
+_FIELD_SHAPES = {
+    'V0T': lambda T, V, E: (T, 3), 'V0E': lambda T, V, E: (E, 2), 'T0E': lambda T, V, E: (E, 2),
+    'E0E': lambda T, V, E: (E, 2), 'E0T': lambda T, V, E: (T, 3), 'idE': lambda T, V, E: (E,),
+    'idE0T': lambda T, V, E: (T, 3), 'nbr': lambda T, V, E: (T, 3), 'nbrEdge': lambda T, V, E: (T, 3),
+    'v2tPtr': lambda T, V, E: (V + 1,), 'v2tIdx': lambda T, V, E: (3 * T,),
+    'markE0Tint': lambda T, V, E: (T, 3), 'markE0TbdrD': lambda T, V, E: (T, 3),
+    'markE0TbdrN': lambda T, V, E: (T, 3), 'markV0TbdrD': lambda T, V, E: (T, 3),
+    'coordV': lambda T, V, E: (V, 2), 'areaE': lambda T, V, E: (E,), 'nuE': lambda T, V, E: (E, 2),
+    'baryE': lambda T, V, E: (E, 2), 'areaT': lambda T, V, E: (T,), 'baryT': lambda T, V, E: (T, 2),
+    'B': lambda T, V, E: (T, 2, 2), 'areaE0T': lambda T, V, E: (T, 3), 'sigE0T': lambda T, V, E: (T, 3),
+    'nuE0T': lambda T, V, E: (T, 3, 2), 'coordV0T': lambda T, V, E: (T, 3, 2),
+}
+_BOOL_FIELDS = {'markE0Tint', 'markE0TbdrD', 'markE0TbdrN', 'markV0TbdrD'}
+
+
+def fortran(a, shape, dtype=np.float64):
+    a = np.asfortranarray(a, dtype=dtype)
+    if a.shape != tuple(shape):
+        raise ValueError('expected shape %s, got %s' % (tuple(shape), a.shape))
+    return a
+
+
+class Grid:
+    """Mesh with the reference's numbering (core/generateGridData.m:130-198).
+    Fields are read lazily as attributes: ``g.E0T``, ``g.nuE0T``, ..."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        T, V, E = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().festung_grid_sizes(self._h, C.byref(T), C.byref(V), C.byref(E)))
+        self.numT, self.numV, self.numE = T.value, V.value, E.value
+        self._cache = {}
+
+    @classmethod
+    def domainSquare(cls, hmax, a=0.0, b=1.0):
+        """core/domainSquare.m:67-105 (Friedrichs-Keller triangulation)."""
+        h = C.c_void_p()
+        check(lib().festung_domain_square(float(hmax), float(a), float(b), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def generateGridData(cls, coordV, V0T):
+        """core/generateGridData.m:130-198 for a CCW mesh (V0T 0-based)."""
+        coordV = fortran(coordV, (np.shape(coordV)[0], 2))
+        V0T = fortran(V0T, (np.shape(V0T)[0], 3), np.int32)
+        h = C.c_void_p()
+        check(lib().festung_grid_create(ptr(coordV), coordV.shape[0], ptr(V0T), V0T.shape[0], C.byref(h)))
+        return cls(h.value)
+
+    def set_boundary(self, idE, markE0TbdrN=None):
+        idE = fortran(idE, (self.numE,), np.int32)
+        mN = None if markE0TbdrN is None else fortran(markE0TbdrN, (self.numT, 3), np.uint8)
+        check(lib().festung_grid_set_boundary(self._h, ptr(idE), ptr(mN)))
+        self._cache.clear()
+
+    def field(self, name):
+        if name in self._cache:
+            return self._cache[name]
+        cnt = C.c_int64()
+        isd = C.c_int()
+        check(lib().festung_grid_field_size(self._h, name.encode(), C.byref(cnt), C.byref(isd)))
+        shape = _FIELD_SHAPES[name](self.numT, self.numV, self.numE)
+        if isd.value:
+            out = np.empty(shape, dtype=np.float64, order='F')
+            check(lib().festung_grid_get_f64(self._h, name.encode(), ptr(out)))
+        else:
+            out = np.empty(shape, dtype=np.int32, order='F')
+            check(lib().festung_grid_get_i32(self._h, name.encode(), ptr(out)))
+            if name in _BOOL_FIELDS:
+                out = out.astype(bool)
+        assert out.size == cnt.value
+        self._cache[name] = out
+        return out
+
+    def __getattr__(self, name):
+        if name in _FIELD_SHAPES:
+            return self.field(name)
+        raise AttributeError(name)
+
+    def mapRef2Phy(self, Q1, Q2):
+        """Physical coordinates of reference points (g.mapRef2Phy):
+        returns X1, X2 of shape K x len(Q1)."""
+        Q1 = np.ascontiguousarray(np.atleast_1d(Q1), dtype=np.float64)
+        Q2 = np.ascontiguousarray(np.atleast_1d(Q2), dtype=np.float64)
+        nq = Q1.size
+        X1 = np.empty((self.numT, nq), order='F')
+        X2 = np.empty((self.numT, nq), order='F')
+        check(lib().festung_grid_map_ref_to_phy(self._h, nq, ptr(Q1), ptr(Q2), ptr(X1), ptr(X2)))
+        return X1, X2
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value and L._lib is not None:
+            L._lib.festung_grid_destroy(h)
+            self._h = None
+
+
+def ref_table(p, name, ord_=0):
+    """Reference tables computed by the library (quadrature rules, basis
+    tables, reference blocks), column-major with the reference's shapes."""
+    cnt = C.c_int64()
+    check(lib().festung_ref_size(int(p), name.encode(), int(ord_), C.byref(cnt)))
+    out = np.empty(cnt.value, dtype=np.float64)
+    check(lib().festung_ref_get(int(p), name.encode(), int(ord_), ptr(out)))
+    return out
+
+
+def quadRule1D(qOrd):
+    v = ref_table(0, 'quad1D', qOrd)
+    R = v.size // 2
+    return v[:R], v[R:]
+
+
+def quadRule2D(qOrd):
+    v = ref_table(0, 'quad2D', qOrd)
+    R = v.size // 3
+    return v[:R], v[R:2 * R], v[2 * R:]
+
+
+class Problem:
+    """Device-resident advection problem (festung_problem_* of the C-ABI)."""
+
+    def __init__(self, grid, p, ordRK, limiter=False, graph=False):
+        flags = (L.FLAG_LIMITER if limiter else 0) | (L.FLAG_GRAPH if graph else 0)
+        h = C.c_void_p()
+        check(lib().festung_problem_create(grid._h, int(p), int(ordRK), flags, C.byref(h)))
+        self._h = h
+        self.grid = grid
+        self.p, self.ordRK, self.limiter = p, ordRK, limiter
+        K, N, RE, RF, Re = (C.c_int32() for _ in range(5))
+        check(lib().festung_problem_sizes(h, C.byref(K), C.byref(N), C.byref(RE), C.byref(RF), C.byref(Re)))
+        self.K, self.N, self.RE, self.RF, self.Re = K.value, N.value, RE.value, RF.value, Re.value
+        self.ordE = max(2 * p, 1)
+        self.ordF = 2 * p + 1
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().festung_problem_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        if L._lib is not None and getattr(self, '_h', None) is not None:
+            self.close()
+
+    @property
+    def stream(self):
+        s = C.c_void_p()
+        check(lib().festung_problem_stream(self._h, C.byref(s)))
+        return s.value
+
+    def set_velocity_disc(self, u1Disc, u2Disc, vNormalOnQuadEdge):
+        K, N = self.K, self.N
+        a = fortran(u1Disc, (K, N)); b = fortran(u2Disc, (K, N))
+        v = fortran(vNormalOnQuadEdge, (K, 3, self.Re))
+        check(lib().festung_set_velocity_disc(self._h, ptr(a), ptr(b), ptr(v)))
+
+    def set_velocity_points(self, u1q, u2q, u1e, u2e):
+        K = self.K
+        a = fortran(u1q, (K, self.RE)); b = fortran(u2q, (K, self.RE))
+        c = fortran(u1e, (K, 3, self.Re)); d = fortran(u2e, (K, 3, self.Re))
+        check(lib().festung_set_velocity_points(self._h, ptr(a), ptr(b), ptr(c), ptr(d)))
+
+    def set_source_points(self, fq=None, cDe=None):
+        f = None if fq is None else fortran(fq, (self.K, self.RE))
+        c = None if cDe is None else fortran(cDe, (self.K, 3, self.Re))
+        check(lib().festung_set_source_points(self._h, ptr(f), ptr(c)))
+
+    def set_vertex_dirichlet(self, vertexD):
+        v = None if vertexD is None else fortran(vertexD, (self.grid.numV,))
+        check(lib().festung_set_vertex_dirichlet(self._h, ptr(v)))
+
+    def project(self, ord_, fq, set_state=False):
+        R = self.RE if ord_ == self.ordE else self.RF
+        f = fortran(fq, (self.K, R))
+        out = np.empty((self.K, self.N), order='F')
+        check(lib().festung_project(self._h, int(ord_), ptr(f), ptr(out), int(bool(set_state))))
+        return out
+
+    def set_state(self, cDisc):
+        c = fortran(cDisc, (self.K, self.N))
+        check(lib().festung_set_state(self._h, ptr(c)))
+
+    def get_state(self):
+        out = np.empty((self.K, self.N), order='F')
+        check(lib().festung_get_state(self._h, ptr(out)))
+        return out
+
+    def set_state_dev(self, t):
+        """State from a device tensor laid out [N][K] (contiguous)."""
+        assert t.is_contiguous() and t.numel() == self.K * self.N
+        check(lib().festung_set_state_dev(self._h, C.c_void_p(t.data_ptr())))
+
+    def get_state_dev(self, t):
+        assert t.is_contiguous() and t.numel() == self.K * self.N
+        check(lib().festung_get_state_dev(self._h, C.c_void_p(t.data_ptr())))
+
+    def begin_step(self):
+        check(lib().festung_begin_step(self._h))
+
+    def stage(self, nSubStep, tau):
+        check(lib().festung_stage(self._h, int(nSubStep), float(tau)))
+
+    def step(self, nsteps, tau):
+        check(lib().festung_step(self._h, int(nsteps), float(tau)))
+
+    def limit(self):
+        check(lib().festung_limit(self._h))
+
+    def rhs(self):
+        out = np.empty((self.K, self.N), order='F')
+        check(lib().festung_rhs(self._h, ptr(out)))
+        return out
+
+    def limiter_info(self):
+        K = self.K
+        a = np.empty(K); mn = np.empty((K, 3), order='F'); mx = np.empty((K, 3), order='F')
+        check(lib().festung_limiter_info(self._h, ptr(a), ptr(mn), ptr(mx)))
+        return a, mn, mx
+
+    def l2_error(self, ord_, cq):
+        R = self.RE if ord_ == self.ordE else self.RF
+        q = fortran(cq, (self.K, R))
+        e = C.c_double()
+        check(lib().festung_l2_error(self._h, int(ord_), ptr(q), C.byref(e)))
+        return e.value
+
+    KERNEL_KINDS = ('stage', 'vertex', 'limit_dot', 'limit', 'copy')
+
+    def profile(self, enable):
+        check(lib().festung_profile(self._h, int(bool(enable))))
+
+    def profile_read(self):
+        ms = np.zeros(5)
+        cnt = np.zeros(5, dtype=np.int64)
+        check(lib().festung_profile_read(self._h, ptr(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {k: (ms[i], int(cnt[i])) for i, k in enumerate(self.KERNEL_KINDS)}
+
+    def sync(self):
+        check(lib().festung_sync(self._h))

@@ -1,0 +1,53 @@
+// Shared definitions of the festung_hip library: status codes, the
+// thread-local error message, and small helpers.
+#pragma once
+#include <cstdint>
+#include <cstdio>
+#include <cstdarg>
+#include <string>
+#include <stdexcept>
+#include <vector>
+
+#include "festung_hip.h"
+
+namespace festung {
+
+// Thrown inside the library; converted to an int status at the C-ABI.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw Error(code, buf);
+}
+
+void set_last_error(const std::string& s);
+
+// Run f, translating exceptions into status codes (no exception crosses the C-ABI).
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    set_last_error("");
+    return FESTUNG_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return FESTUNG_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return FESTUNG_ERR_INTERNAL;
+  }
+}
+
+inline int nbasis(int p) { return (p + 1) * (p + 2) / 2; }
+
+}  // namespace festung

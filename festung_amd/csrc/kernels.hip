@@ -1,0 +1,799 @@
+// gfx950 kernels of the explicit DG advection step on triangles (FP64).
+//
+// One thread owns one element k.  Coefficients are structure-of-arrays
+// c[i*K + k] (the column-major K x N layout of the reference's cDisc), so
+// every per-element load and store of a wavefront is a contiguous 512-byte
+// segment.  Reference tables are read through uniform (scalar) loads from
+// one device buffer; the neighbour-trace table, whose index depends on the
+// neighbour's local edge, is staged in LDS.
+//
+// Kernels:
+//   stage_kernel       element term (core/assembleMatElemDphiPhiFuncDisc.m)
+//                      + upwind edge term (core/assembleMatEdgePhiPhiValUpwind.m)
+//                      + source + block mass solve + RK combination
+//                      (advection/solveSubStep.m:62-83), optionally followed by
+//                      the change to the Taylor basis of the limiter
+//   vertex_minmax      per-vertex patch min / max of centroid values with the
+//                      reference's global shift (core/computeMinMaxV0TElementPatch.m)
+//   limit_kernel       alpha (core/computeVertexBasedLimiter.m), limited Taylor
+//                      coefficients (core/applySlopeLimiterTaylorLinear.m), the
+//                      dc/dt mass-lumping correction (advection/solveSubStep.m:75-78),
+//                      change back to the DG basis, and for dc/dt the RK combination
+//   to_taylor_kernel   DG -> Taylor for a stand-alone limiter call
+//   project / source / vnormal / l2 kernels for setup and checks.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace festung {
+
+constexpr int BLOCK = 256;
+
+__device__ __forceinline__ int gtid() { return blockIdx.x * BLOCK + threadIdx.x; }
+
+// ---------------------------------------------------------------- Taylor --
+// E_D(A): the degree-D symmetric power of a 2x2 matrix A acting on
+// derivative multi-indices.  Row b (target derivative d1^(D-b) d2^b) holds
+// the coefficients of (A11 X + A21)^(D-b) (A12 X + A22)^b, column b' the
+// coefficient of X^(D-b') (source derivative d1^(D-b') d2^b').
+template <int D>
+__device__ __forceinline__ void sym_power(double A11, double A21, double A12, double A22, double (&E)[D + 1][D + 1]) {
+#pragma unroll
+  for (int b = 0; b <= D; ++b) {
+    double pw[D + 1];
+    pw[0] = 1.0;
+#pragma unroll
+    for (int i = 1; i <= D; ++i) pw[i] = 0.0;
+#pragma unroll
+    for (int t = 0; t < D; ++t) {
+      const double c0 = t < D - b ? A21 : A22;
+      const double c1 = t < D - b ? A11 : A12;
+#pragma unroll
+      for (int i = D; i >= 1; --i) pw[i] = pw[i] * c0 + pw[i - 1] * c1;
+      pw[0] = pw[0] * c0;
+    }
+#pragma unroll
+    for (int bp = 0; bp <= D; ++bp) E[b][bp] = pw[D - bp];
+  }
+}
+
+// y[off .. off+D] = E_D(A) x[off .. off+D]
+template <int D>
+__device__ __forceinline__ void apply_sym(double A11, double A21, double A12, double A22, const double* x, double* y) {
+  double E[D + 1][D + 1];
+  sym_power<D>(A11, A21, A12, A22, E);
+#pragma unroll
+  for (int b = 0; b <= D; ++b) {
+    double s = 0.0;
+#pragma unroll
+    for (int bp = 0; bp <= D; ++bp) s += E[b][bp] * x[bp];
+    y[b] = s;
+  }
+}
+
+template <int P>
+struct Geo {
+  double B11, B21, B12, B22, det, dX;
+  __device__ __forceinline__ void load(const double* __restrict__ Bp, int k, int K) {
+    B11 = Bp[k];
+    B21 = Bp[k + K];
+    B12 = Bp[k + 2 * K];
+    B22 = Bp[k + 3 * K];
+    det = B11 * B22 - B12 * B21;  // = 2|T| for a counter-clockwise element
+    dX = 2.0 / sqrt(det);         // core/phiTaylorPhy.m:80
+  }
+};
+
+// DG -> Taylor coefficients (exact change of basis between the orthonormal
+// basis and the element's centroid-scaled, mean-free Taylor basis of
+// core/phiTaylorPhy.m: T_0 = mean, T_(a,b) = d_xi^a d_eta^b u at the centroid,
+// xi = dX (x - x_c)).  F maps DG coefficients to [mean; reference-coordinate
+// derivatives at the centroid].
+template <int P>
+__device__ __forceinline__ void dg_to_taylor(const double (&x)[Dims<P>::N], const Geo<P>& g,
+                                             const double* __restrict__ F, double (&T)[Dims<P>::N]) {
+  constexpr int N = Dims<P>::N;
+  double y[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) s += F[r + j * N] * x[j];
+    y[r] = s;
+  }
+  const double s = 1.0 / (g.det * g.dX);
+  const double C11 = g.B22 * s, C12 = -g.B12 * s, C21 = -g.B21 * s, C22 = g.B11 * s;
+  T[0] = y[0];
+  if constexpr (P >= 1) apply_sym<1>(C11, C21, C12, C22, y + 1, T + 1);
+  if constexpr (P >= 2) apply_sym<2>(C11, C21, C12, C22, y + 3, T + 3);
+  if constexpr (P >= 3) apply_sym<3>(C11, C21, C12, C22, y + 6, T + 6);
+  if constexpr (P >= 4) apply_sym<4>(C11, C21, C12, C22, y + 10, T + 10);
+}
+
+template <int P>
+__device__ __forceinline__ void taylor_to_dg(const double (&T)[Dims<P>::N], const Geo<P>& g,
+                                             const double* __restrict__ Finv, double (&x)[Dims<P>::N]) {
+  constexpr int N = Dims<P>::N;
+  double z[N];
+  const double A11 = g.dX * g.B11, A21 = g.dX * g.B21, A12 = g.dX * g.B12, A22 = g.dX * g.B22;
+  z[0] = T[0];
+  if constexpr (P >= 1) apply_sym<1>(A11, A21, A12, A22, T + 1, z + 1);
+  if constexpr (P >= 2) apply_sym<2>(A11, A21, A12, A22, T + 3, z + 3);
+  if constexpr (P >= 3) apply_sym<3>(A11, A21, A12, A22, T + 6, z + 6);
+  if constexpr (P >= 4) apply_sym<4>(A11, A21, A12, A22, T + 10, z + 10);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) s += Finv[i + j * N] * z[j];
+    x[i] = s;
+  }
+}
+
+// Element moments mu_(a,b) = mean over the element of xi^a eta^b for
+// a + b <= 2P, stored at d(d+1)/2 + b (d = a + b).
+template <int P, int D>
+__device__ __forceinline__ void moments_deg(const Geo<P>& g, const double* __restrict__ nu, double* mu) {
+  double E[D + 1][D + 1];
+  sym_power<D>(g.dX * g.B11, g.dX * g.B12, g.dX * g.B21, g.dX * g.B22, E);
+#pragma unroll
+  for (int b = 0; b <= D; ++b) {
+    double s = 0.0;
+#pragma unroll
+    for (int bp = 0; bp <= D; ++bp) s += E[b][bp] * nu[(D - bp) * 9 + bp];
+    mu[D * (D + 1) / 2 + b] = s;
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void element_moments(const Geo<P>& g, const double* __restrict__ nu,
+                                                double (&mu)[(2 * P + 1) * (2 * P + 2) / 2]) {
+  mu[0] = 1.0;
+  if constexpr (2 * P >= 1) moments_deg<P, 1>(g, nu, mu);
+  if constexpr (2 * P >= 2) moments_deg<P, 2>(g, nu, mu);
+  if constexpr (2 * P >= 3) moments_deg<P, 3>(g, nu, mu);
+  if constexpr (2 * P >= 4) moments_deg<P, 4>(g, nu, mu);
+  if constexpr (2 * P >= 5) moments_deg<P, 5>(g, nu, mu);
+  if constexpr (2 * P >= 6) moments_deg<P, 6>(g, nu, mu);
+  if constexpr (2 * P >= 7) moments_deg<P, 7>(g, nu, mu);
+  if constexpr (2 * P >= 8) moments_deg<P, 8>(g, nu, mu);
+}
+
+__host__ __device__ constexpr int tdeg(int i) { return i == 0 ? 0 : i < 3 ? 1 : i < 6 ? 2 : i < 10 ? 3 : 4; }
+__host__ __device__ constexpr int tb(int i) { return i - tdeg(i) * (tdeg(i) + 1) / 2; }
+__host__ __device__ constexpr double tfact(int n) { return n <= 1 ? 1.0 : n * tfact(n - 1); }
+
+// ------------------------------------------------------ min / max reduce --
+__device__ __forceinline__ void minmax_block_finish(double mn, double mx, const MinMaxSink& s) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    mn = fmin(mn, __shfl_xor(mn, off));
+    mx = fmax(mx, __shfl_xor(mx, off));
+  }
+  __shared__ double smn[BLOCK / 64], smx[BLOCK / 64];
+  __shared__ int isLast;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    smn[w] = mn;
+    smx[w] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = smn[0], b = smx[0];
+#pragma unroll
+    for (int i = 1; i < BLOCK / 64; ++i) {
+      a = fmin(a, smn[i]);
+      b = fmax(b, smx[i]);
+    }
+    s.partials[2 * blockIdx.x] = a;
+    s.partials[2 * blockIdx.x + 1] = b;
+    __threadfence();
+    unsigned t = atomicAdd(s.counter, 1u);
+    isLast = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (isLast) {
+    __threadfence();
+    double a = INFINITY, b = -INFINITY;
+    const volatile double* pp = s.partials;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += BLOCK) {
+      a = fmin(a, pp[2 * i]);
+      b = fmax(b, pp[2 * i + 1]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      a = fmin(a, __shfl_xor(a, off));
+      b = fmax(b, __shfl_xor(b, off));
+    }
+    __syncthreads();
+    if (lane == 0) {
+      smn[w] = a;
+      smx[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int i = 1; i < BLOCK / 64; ++i) {
+        a = fmin(a, smn[i]);
+        b = fmax(b, smx[i]);
+      }
+      s.gmm[0] = a;
+      s.gmm[1] = b;
+      *s.counter = 0u;
+      __threadfence();
+    }
+  }
+}
+
+// ----------------------------------------------------------------- stage --
+
+template <int P, int MODE>
+__global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
+  constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
+  __shared__ double sTheta[Re * N * 9];
+  for (int i = threadIdx.x; i < Re * N * 9; i += BLOCK) sTheta[i] = a.tab[a.off.theta + i];
+  __syncthreads();
+
+  const int K = a.m.K;
+  const int k = gtid();
+  const bool active = k < K;
+  const int kk = active ? k : 0;
+  const double* __restrict__ T = a.tab;
+
+  double c[N], u1[N], u2[N], acc[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    c[i] = a.c[i * K + kk];
+    u1[i] = a.u1[i * K + kk];
+    u2[i] = a.u2[i * K + kk];
+    acc[i] = 0.0;
+  }
+  Geo<P> g;
+  g.load(a.m.B, kk, K);
+
+  // element term  (G1 + G2) c  in quadrature form on quadRule2D(max(2p,1))
+  if constexpr (P >= 1) {
+    constexpr int UQ = RE <= 16 ? RE : 4;
+#pragma unroll UQ
+    for (int q = 0; q < RE; ++q) {
+      double cq = 0.0, u1q = 0.0, u2q = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const double ph = T[a.off.phiE + q + j * RE];
+        cq += c[j] * ph;
+        u1q += u1[j] * ph;
+        u2q += u2[j] * ph;
+      }
+      const double wc = T[a.off.wE + q] * cq;
+      const double f1 = wc * (g.B22 * u1q - g.B12 * u2q);
+      const double f2 = wc * (g.B11 * u2q - g.B21 * u1q);
+#pragma unroll
+      for (int i = 1; i < N; ++i)
+        acc[i] += T[a.off.gradE + q + i * RE] * f1 + T[a.off.gradE + q + i * RE + RE * N] * f2;
+    }
+  }
+
+  // edge term  -R c  (upwind; interior neighbours gathered, no atomics)
+  const int fl = a.m.flags[kk];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const double area = a.m.areaE[n * K + kk];
+    const int nb = a.m.nbr[n * K + kk];
+    const int np = flag_np(fl, n);
+    const bool outOn = flag_out(fl, n);
+    double cn[N];
+    if (nb >= 0) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) cn[j] = a.c[j * K + nb];
+    }
+#pragma unroll
+    for (int r = 0; r < Re; ++r) {
+      const double v = a.vn[(r * 3 + n) * K + kk];
+      double tr = 0.0;
+      if (v > 0.0 && outOn) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) tr += c[j] * T[a.off.phi1D + r + j * Re + n * Re * N];
+      } else if (v < 0.0 && nb >= 0) {
+        const double* th = sTheta + r + n * Re * N + np * Re * N * 3;
+#pragma unroll
+        for (int j = 0; j < N; ++j) tr += cn[j] * th[j * Re];
+      }
+      const double fx = area * v * T[a.off.wEdge + r] * tr;
+#pragma unroll
+      for (int i = 0; i < N; ++i) acc[i] -= T[a.off.phi1D + r + i * Re + n * Re * N] * fx;
+    }
+  }
+  if (a.rhs) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] += a.rhs[i * K + kk];
+  }
+  // block mass solve: cdot = (detJ hatM)^-1 rhs
+  double cdot[N];
+  const double idet = 1.0 / g.det;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) s += T[a.off.hatMinv + i + j * N] * acc[j];
+    cdot[i] = s * idet;
+  }
+
+  if constexpr (MODE == OUT_DOT_DG) {
+    if (active)
+#pragma unroll
+      for (int i = 0; i < N; ++i) a.out[i * K + k] = cdot[i];
+    return;
+  } else if constexpr (MODE == OUT_DOT_TAYLOR) {
+    double Tc[N];
+    dg_to_taylor<P>(cdot, g, T + a.off.F, Tc);
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
+      a.sink.centroid[k] = Tc[0];
+    }
+    minmax_block_finish(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+  } else {
+    // RK combination (advection/solveSubStep.m:83 for the SSP weights)
+    double nx[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) nx[i] = a.rk.a1 * c[i] + a.rk.b * a.tau * cdot[i];
+    if (a.rk.a0 != 0.0 || a.rk.d0 != 0.0) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double c0 = a.c0[i * K + kk];
+        nx[i] += a.rk.a0 * c0;
+        acc[i] = a.rk.d0 * c0;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < N; ++i) acc[i] = 0.0;
+    }
+    if (a.rk.g != 0.0 || a.rk.d1 != 0.0) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double ac = a.acc[i * K + kk];
+        nx[i] += a.rk.g * ac;
+        acc[i] += a.rk.d1 * ac;
+      }
+    }
+    if (a.accOut && active) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) a.accOut[i * K + k] = acc[i] + a.rk.e * a.tau * cdot[i];
+    }
+    if constexpr (MODE == OUT_RK_DG) {
+      if (active)
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.out[i * K + k] = nx[i];
+    } else {
+      double Tc[N];
+      dg_to_taylor<P>(nx, g, T + a.off.F, Tc);
+      if (active) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
+        a.sink.centroid[k] = Tc[0];
+      }
+      minmax_block_finish(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+    }
+  }
+}
+
+// -------------------------------------------------------- vertex min/max --
+
+__global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
+  const int v = gtid();
+  if (v >= a.numV) return;
+  const double sNeg = fabs(a.gmm[1]) + 1.0;  // shiftCentroidToNeg
+  const double sPos = fabs(a.gmm[0]) + 1.0;  // shiftCentroidToPos
+  double m = INFINITY, M = -INFINITY;
+  for (int i = a.v2tPtr[v]; i < a.v2tPtr[v + 1]; ++i) {
+    const double c = a.centroid[a.v2tIdx[i]];
+    m = fmin(m, c);
+    M = fmax(M, c);
+  }
+  // min over the patch of (c - sNeg), then + sNeg (rounding of the reference kept)
+  double lo = (m - sNeg) + sNeg;
+  double hi = (M + sPos) - sPos;
+  if (a.vertexD) {
+    const double d = a.vertexD[v];  // NaN where not a Dirichlet vertex
+    lo = fmin(lo, d);
+    hi = fmax(hi, d);
+  }
+  a.vmin[v] = lo;
+  a.vmax[v] = hi;
+}
+
+// ----------------------------------------------------------------- limit --
+
+template <int P, int MODE>
+__global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
+  constexpr int N = Dims<P>::N;
+  const int K = a.m.K;
+  const int k = gtid();
+  const bool active = k < K;
+  const int kk = active ? k : 0;
+  const double* __restrict__ T = a.tab;
+
+  double x[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] = a.taylor[i * K + kk];
+  Geo<P> g;
+  g.load(a.m.B, kk, K);
+
+  // vertex values of the linear part, core/applySlopeLimiterTaylorLinear.m:117
+  // (phi^Taylor_2,3 at vertex n = dX * (x_n - x_c))
+  const double ex = (g.B11 + g.B12) / 3.0, ey = (g.B21 + g.B22) / 3.0;
+  const double px[3] = {-ex, g.B11 - ex, g.B12 - ex};
+  const double py[3] = {-ey, g.B21 - ey, g.B22 - ey};
+  const double xc = x[0];
+  const double tol = 1.e-8;
+  double alpha = 1.0;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const double val = x[0] + x[1] * (px[n] * g.dX) + x[2] * (py[n] * g.dX);
+    const int v = a.m.V0T[n * K + kk];
+    const double d = val - xc;
+    const double dMin = a.vmin[v] - xc;
+    const double dMax = a.vmax[v] - xc;
+    double al = 1.0;
+    if (d < dMin + tol) al = fmax(0.0, fmin(1.0, dMin / (d - tol)));
+    if (d > dMax - tol) al = fmax(0.0, fmin(1.0, dMax / (d + tol)));
+    alpha = fmin(alpha, al);
+  }
+  if (a.alphaOut && active) a.alphaOut[k] = alpha;
+
+  double lim[N];
+  lim[0] = x[0];
+  lim[1] = alpha * x[1];
+  lim[2] = alpha * x[2];
+  const double keep = alpha == 1.0 ? 1.0 : 0.0;
+#pragma unroll
+  for (int i = 3; i < N; ++i) lim[i] = keep * x[i];
+
+  if constexpr (MODE != LIM_SOLUTION && P >= 2) {
+    // selective mass lumping: lim + diag(MT)^-1 MT (orig - lim)
+    if (alpha != 1.0) {
+      constexpr int NM = (2 * P + 1) * (2 * P + 2) / 2;
+      double mu[NM];
+      element_moments<P>(g, T + a.off.nu, mu);
+      double diff[N];
+#pragma unroll
+      for (int j = 0; j < N; ++j) diff[j] = x[j] - lim[j];
+      double corr[N];
+#pragma unroll
+      for (int i = 1; i < N; ++i) {
+        const int di = tdeg(i), bi = tb(i), ai = di - bi;
+        const double mui = mu[i];
+        const int i2 = (2 * di) * (2 * di + 1) / 2 + 2 * bi;
+        const double fi = tfact(ai) * tfact(bi);
+        const double den = (mu[i2] - mui * mui) / (fi * fi);
+        double s = 0.0;
+#pragma unroll
+        for (int j = 1; j < N; ++j) {
+          const int dj = tdeg(j), bj = tb(j), aj = dj - bj;
+          const int ij = (di + dj) * (di + dj + 1) / 2 + (bi + bj);
+          const double fj = tfact(aj) * tfact(bj);
+          s += ((mu[ij] - mui * mu[j]) / (fi * fj)) / den * diff[j];
+        }
+        corr[i] = s;
+      }
+#pragma unroll
+      for (int i = 1; i < N; ++i) lim[i] += corr[i];
+    }
+  }
+  double y[N];
+  taylor_to_dg<P>(lim, g, T + a.off.Finv, y);
+
+  if constexpr (MODE == LIM_SOLUTION) {
+    if (active)
+#pragma unroll
+      for (int i = 0; i < N; ++i) a.out[i * K + k] = y[i];
+  } else {
+    double nx[N], acc[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) nx[i] = a.rk.a1 * a.c[i * K + kk] + a.rk.b * a.tau * y[i];
+    if (a.rk.a0 != 0.0 || a.rk.d0 != 0.0) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double c0 = a.c0[i * K + kk];
+        nx[i] += a.rk.a0 * c0;
+        acc[i] = a.rk.d0 * c0;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < N; ++i) acc[i] = 0.0;
+    }
+    if (a.rk.g != 0.0 || a.rk.d1 != 0.0) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double ac = a.acc[i * K + kk];
+        nx[i] += a.rk.g * ac;
+        acc[i] += a.rk.d1 * ac;
+      }
+    }
+    if (a.accOut && active) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) a.accOut[i * K + k] = acc[i] + a.rk.e * a.tau * y[i];
+    }
+    if constexpr (MODE == LIM_DOT_RK_DG) {
+      if (active)
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.out[i * K + k] = nx[i];
+    } else {
+      double Tc[N];
+      dg_to_taylor<P>(nx, g, T + a.off.F, Tc);
+      if (active) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
+        a.sink.centroid[k] = Tc[0];
+      }
+      minmax_block_finish(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+    }
+  }
+}
+
+// ------------------------------------------------------------ to_taylor --
+
+template <int P>
+__global__ __launch_bounds__(BLOCK) void to_taylor_kernel(ToTaylorArgs a) {
+  constexpr int N = Dims<P>::N;
+  const int K = a.K;
+  const int k = gtid();
+  const bool active = k < K;
+  const int kk = active ? k : 0;
+  double x[N], Tc[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] = a.c[i * K + kk];
+  Geo<P> g;
+  g.load(a.B, kk, K);
+  dg_to_taylor<P>(x, g, a.tab + a.off.F, Tc);
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
+    a.sink.centroid[k] = Tc[0];
+  }
+  minmax_block_finish(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+}
+
+// ------------------------------------------------------ setup kernels --
+// L2 projection of point values, core/projectFuncCont2DataDisc.m:98-101:
+// out = (sum_q f_q w_q phi(q)) / hatM.
+template <int P, int R>
+__global__ __launch_bounds__(BLOCK) void project_kernel(int K, const double* __restrict__ fq,
+                                                        const double* __restrict__ tab, int offW, int offPhi,
+                                                        int offMinv, double* __restrict__ out) {
+  constexpr int N = Dims<P>::N;
+  const int k = gtid();
+  if (k >= K) return;
+  double r[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) r[i] = 0.0;
+  for (int q = 0; q < R; ++q) {
+    const double f = fq[q * K + k] * tab[offW + q];
+#pragma unroll
+    for (int i = 0; i < N; ++i) r[i] += f * tab[offPhi + q + i * R];
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) s += r[j] * tab[offMinv + j + i * N];
+    out[i * K + k] = s;
+  }
+}
+
+// u . nu at the edge points, core/computeFuncContNuOnQuadEdge.m:61-66
+__global__ __launch_bounds__(BLOCK) void vnormal_kernel(int K, int Re, const double* __restrict__ nuE0T,
+                                                        const double* __restrict__ u1e,
+                                                        const double* __restrict__ u2e, double* __restrict__ vn) {
+  const int k = gtid();
+  if (k >= K) return;
+  for (int n = 0; n < 3; ++n) {
+    const double n1 = nuE0T[k + n * K], n2 = nuE0T[k + n * K + 3 * K];
+    for (int r = 0; r < Re; ++r) {
+      const int idx = (r * 3 + n) * K + k;
+      vn[idx] = n1 * u1e[idx] + n2 * u2e[idx];
+    }
+  }
+}
+
+// globL - globKD: detJ hatM fDisc minus the Dirichlet inflow term,
+// advection/preprocessSubStep.m:88-98, core/assembleVecEdgePhiIntFuncContVal.m:173-184
+template <int P>
+__global__ __launch_bounds__(BLOCK) void source_kernel(Mesh m, const double* __restrict__ fq,
+                                                       const double* __restrict__ cDe, const double* __restrict__ vn,
+                                                       const double* __restrict__ tab, TabOff off,
+                                                       double* __restrict__ rhs) {
+  constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
+  const int K = m.K;
+  const int k = gtid();
+  if (k >= K) return;
+  double r[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) r[i] = 0.0;
+  if (fq) {
+    Geo<P> g;
+    g.load(m.B, k, K);
+    double pr[N], fd[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) pr[i] = 0.0;
+    for (int q = 0; q < RE; ++q) {
+      const double f = fq[q * K + k] * tab[off.wE + q];
+#pragma unroll
+      for (int i = 0; i < N; ++i) pr[i] += f * tab[off.phiE + q + i * RE];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s += pr[j] * tab[off.hatMinv + j + i * N];
+      fd[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s += tab[off.hatM + i + j * N] * fd[j];
+      r[i] = g.det * s;
+    }
+  }
+  if (cDe) {
+    const int fl = m.flags[k];
+    for (int n = 0; n < 3; ++n) {
+      if (!flag_dir(fl, n)) continue;
+      const double area = m.areaE[n * K + k];
+      for (int r2 = 0; r2 < Re; ++r2) {
+        const int idx = (r2 * 3 + n) * K + k;
+        const double v = vn[idx];
+        const double fx = area * (cDe[idx] * ((v < 0.0) ? v : 0.0)) * tab[off.wEdge + r2];
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[i] -= fx * tab[off.phi1D + r2 + i * Re + n * Re * N];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) rhs[i * K + k] = r[i];
+}
+
+// per-block partial sums of 2 |T_k| sum_r W_r (c_h - c)^2, core/computeL2Error.m:103-113
+template <int P, int R>
+__global__ __launch_bounds__(BLOCK) void l2_kernel(int K, const double* __restrict__ B, const double* __restrict__ c,
+                                                   const double* __restrict__ cq, const double* __restrict__ tab,
+                                                   int offW, int offPhi, double* __restrict__ partial) {
+  constexpr int N = Dims<P>::N;
+  const int k = gtid();
+  double e = 0.0;
+  if (k < K) {
+    double x[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = c[i * K + k];
+    for (int q = 0; q < R; ++q) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s += x[j] * tab[offPhi + q + j * R];
+      const double d = s - cq[q * K + k];
+      e += d * d * tab[offW + q];
+    }
+    const double det = B[k] * B[k + 3 * K] - B[k + 2 * K] * B[k + K];
+    e *= det;  // 2 |T_k|
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off);
+  __shared__ double s[BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < BLOCK / 64; ++i) t += s[i];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// ---------------------------------------------------------- launchers --
+#define FESTUNG_DISPATCH_P(P_, BODY)                           \
+  switch (P_) {                                                \
+    case 0: { constexpr int PP = 0; BODY; } break;             \
+    case 1: { constexpr int PP = 1; BODY; } break;             \
+    case 2: { constexpr int PP = 2; BODY; } break;             \
+    case 3: { constexpr int PP = 3; BODY; } break;             \
+    case 4: { constexpr int PP = 4; BODY; } break;             \
+    default: return hipErrorInvalidValue;                      \
+  }
+
+#define FESTUNG_DISPATCH_P1(P_, BODY)                          \
+  switch (P_) {                                                \
+    case 1: { constexpr int PP = 1; BODY; } break;             \
+    case 2: { constexpr int PP = 2; BODY; } break;             \
+    case 3: { constexpr int PP = 3; BODY; } break;             \
+    case 4: { constexpr int PP = 4; BODY; } break;             \
+    default: return hipErrorInvalidValue;                      \
+  }
+
+static inline dim3 grid_for(int n) { return dim3((n + BLOCK - 1) / BLOCK); }
+
+hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s) {
+  dim3 gr = grid_for(a.m.K);
+#define L_STAGE(MODE_)                                                                  \
+  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((stage_kernel<PP, MODE_>), gr, dim3(BLOCK), 0, s, a))
+#define L_STAGE1(MODE_)                                                                 \
+  FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((stage_kernel<PP, MODE_>), gr, dim3(BLOCK), 0, s, a))
+  switch (mode) {
+    case OUT_RK_DG: L_STAGE(OUT_RK_DG); break;
+    case OUT_RK_TAYLOR: L_STAGE1(OUT_RK_TAYLOR); break;
+    case OUT_DOT_TAYLOR: L_STAGE1(OUT_DOT_TAYLOR); break;
+    case OUT_DOT_DG: L_STAGE(OUT_DOT_DG); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef L_STAGE
+#undef L_STAGE1
+  return hipGetLastError();
+}
+
+hipError_t launch_vertex(const VertexArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(vertex_minmax_kernel, grid_for(a.numV), dim3(BLOCK), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_limit(int p, int mode, const LimitArgs& a, hipStream_t s) {
+  if (p < 1) return hipErrorInvalidValue;
+  dim3 gr = grid_for(a.m.K);
+#define L_LIM(MODE_) FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((limit_kernel<PP, MODE_>), gr, dim3(BLOCK), 0, s, a))
+  switch (mode) {
+    case LIM_SOLUTION: L_LIM(LIM_SOLUTION); break;
+    case LIM_DOT_RK_DG: L_LIM(LIM_DOT_RK_DG); break;
+    case LIM_DOT_RK_TAYLOR: L_LIM(LIM_DOT_RK_TAYLOR); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef L_LIM
+  return hipGetLastError();
+}
+
+hipError_t launch_to_taylor(int p, const ToTaylorArgs& a, hipStream_t s) {
+  dim3 gr = grid_for(a.K);
+  FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((to_taylor_kernel<PP>), gr, dim3(BLOCK), 0, s, a));
+  return hipGetLastError();
+}
+
+hipError_t launch_project(int p, bool ordF, int K, const double* fq, const double* tab, const TabOff& off,
+                          double* out, hipStream_t s) {
+  dim3 gr = grid_for(K);
+  if (ordF) {
+    FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((project_kernel<PP, Dims<PP>::RF>), gr, dim3(BLOCK), 0, s, K, fq, tab,
+                                             off.wF, off.phiF, off.hatMinv, out));
+  } else {
+    FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((project_kernel<PP, Dims<PP>::RE>), gr, dim3(BLOCK), 0, s, K, fq, tab,
+                                             off.wE, off.phiE, off.hatMinv, out));
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_vnormal(int K, int Re, const double* nuE0T, const double* u1e, const double* u2e, double* vn,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(vnormal_kernel, grid_for(K), dim3(BLOCK), 0, s, K, Re, nuE0T, u1e, u2e, vn);
+  return hipGetLastError();
+}
+
+hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* cDe, const double* vn,
+                         const double* tab, const TabOff& off, double* rhs, hipStream_t s) {
+  dim3 gr = grid_for(m.K);
+  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((source_kernel<PP>), gr, dim3(BLOCK), 0, s, m, fq, cDe, vn, tab, off, rhs));
+  return hipGetLastError();
+}
+
+hipError_t launch_l2(int p, bool ordF, int K, const double* B, const double* c, const double* cq, const double* tab,
+                     const TabOff& off, double* partial, hipStream_t s) {
+  dim3 gr = grid_for(K);
+  if (ordF) {
+    FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((l2_kernel<PP, Dims<PP>::RF>), gr, dim3(BLOCK), 0, s, K, B, c, cq, tab,
+                                             off.wF, off.phiF, partial));
+  } else {
+    FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((l2_kernel<PP, Dims<PP>::RE>), gr, dim3(BLOCK), 0, s, K, B, c, cq, tab,
+                                             off.wE, off.phiE, partial));
+  }
+  return hipGetLastError();
+}
+
+int block_size() { return BLOCK; }
+
+}  // namespace festung

@@ -1,0 +1,139 @@
+// Device-side data structures shared by the kernels and the problem object.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace festung {
+
+// Compile-time sizes for polynomial degree P.
+template <int P>
+struct Dims {
+  static constexpr int N = (P + 1) * (P + 2) / 2;
+  // points of quadRule2D(max(2P, 1)) and quadRule2D(2P + 1)
+  static constexpr int RE = P == 0 ? 1 : P == 1 ? 3 : P == 2 ? 6 : P == 3 ? 12 : 64;
+  static constexpr int RF = P == 0 ? 1 : P == 1 ? 4 : P == 2 ? 7 : P == 3 ? 49 : 81;
+  static constexpr int Re = P + 1;  // points of quadRule1D(2P + 1)
+};
+
+// Offsets (in doubles) of the reference tables inside one device buffer.
+struct TabOff {
+  int wE, phiE, gradE;      // quadRule2D(max(2p,1)): weights, phi (RE x N), gradPhi (RE x N x 2)
+  int wF, phiF;             // quadRule2D(2p+1)
+  int wEdge, phi1D, theta;  // quadRule1D(2p+1): weights, phi1D (Re x N x 3), thetaPhi1D (Re x N x 3 x 3)
+  int hatM, hatMinv;        // N x N
+  int F, Finv;              // Taylor change of basis, N x N
+  int nu;                   // 9 x 9 centred reference moments
+  int total;
+};
+
+// Per-element flag word: bits 2n..2n+1 = neighbour's local edge across
+// local edge n; bit 6+n = outflow (diagonal) term active on edge n; bit 9+n =
+// Dirichlet inflow on edge n.
+__host__ __device__ inline int flag_np(int f, int n) { return (f >> (2 * n)) & 3; }
+__host__ __device__ inline bool flag_out(int f, int n) { return (f >> (6 + n)) & 1; }
+__host__ __device__ inline bool flag_dir(int f, int n) { return (f >> (9 + n)) & 1; }
+
+// Coefficients of one explicit RK stage:
+//   out = a0*c0 + a1*c + b*tau*cdot + g*acc
+//   acc_out = d0*c0 + d1*acc + e*tau*cdot           (only when acc_out != null)
+struct RKCoef {
+  double a0, a1, b, g, d0, d1, e;
+};
+
+// Global min / max of the centroid values of one limiter application, by a
+// single-pass block reduction: per-block partials, the last block to finish
+// folds them into gmm[0] = min, gmm[1] = max and resets the counter.
+struct MinMaxSink {
+  double* centroid;   // K
+  double* partials;   // 2 x gridDim.x
+  unsigned* counter;  // 1, zero between launches
+  double* gmm;        // 2
+};
+
+struct Mesh {
+  int K, numV;
+  const double* B;       // 4 x K: B11, B21, B12, B22
+  const double* areaE;   // 3 x K
+  const int* nbr;        // 3 x K
+  const int* flags;      // K
+  const int* V0T;        // 3 x K
+  const int* v2tPtr;     // numV + 1
+  const int* v2tIdx;     // 3K
+};
+
+enum StageOut { OUT_RK_DG = 0, OUT_RK_TAYLOR = 1, OUT_DOT_TAYLOR = 2, OUT_DOT_DG = 3 };
+enum LimitMode { LIM_SOLUTION = 0, LIM_DOT_RK_DG = 1, LIM_DOT_RK_TAYLOR = 2 };
+
+struct StageArgs {
+  Mesh m;
+  const double* c;       // stage state (neighbours are read from it)
+  const double* c0;      // state at the start of the step
+  const double* acc;     // RK4 accumulator (in)
+  double* accOut;        // RK4 accumulator (out) or null
+  const double* u1;
+  const double* u2;
+  const double* vn;      // Re x 3 x K
+  const double* rhs;     // N x K or null (globL - globKD - globKN)
+  const double* tab;
+  TabOff off;
+  double tau;
+  RKCoef rk;
+  double* out;           // DG or Taylor coefficients
+  MinMaxSink sink;       // for the Taylor outputs
+};
+
+struct VertexArgs {
+  int numV;
+  const int* v2tPtr;
+  const int* v2tIdx;
+  const double* centroid;
+  const double* gmm;
+  const double* vertexD;  // numV or null
+  double* vmin;
+  double* vmax;
+};
+
+struct LimitArgs {
+  Mesh m;
+  const double* taylor;  // Taylor coefficients to limit (N x K)
+  const double* vmin;
+  const double* vmax;
+  const double* tab;
+  TabOff off;
+  // RK combination (dc/dt modes)
+  const double* c;
+  const double* c0;
+  const double* acc;
+  double* accOut;
+  double tau;
+  RKCoef rk;
+  double* out;           // DG (LIM_SOLUTION, LIM_DOT_RK_DG) or Taylor (LIM_DOT_RK_TAYLOR)
+  MinMaxSink sink;
+  double* alphaOut;      // K or null
+};
+
+struct ToTaylorArgs {
+  int K;
+  const double* B;
+  const double* c;
+  const double* tab;
+  TabOff off;
+  double* out;
+  MinMaxSink sink;
+};
+
+hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s);
+hipError_t launch_vertex(const VertexArgs& a, hipStream_t s);
+hipError_t launch_limit(int p, int mode, const LimitArgs& a, hipStream_t s);
+hipError_t launch_to_taylor(int p, const ToTaylorArgs& a, hipStream_t s);
+hipError_t launch_project(int p, bool ordF, int K, const double* fq, const double* tab, const TabOff& off,
+                          double* out, hipStream_t s);
+hipError_t launch_vnormal(int K, int Re, const double* nuE0T, const double* u1e, const double* u2e, double* vn,
+                          hipStream_t s);
+hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* cDe, const double* vn,
+                         const double* tab, const TabOff& off, double* rhs, hipStream_t s);
+hipError_t launch_l2(int p, bool ordF, int K, const double* B, const double* c, const double* cq, const double* tab,
+                     const TabOff& off, double* partial, hipStream_t s);
+int block_size();
+
+}  // namespace festung

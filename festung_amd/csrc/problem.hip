@@ -1,0 +1,728 @@
+// Device-resident advection problem: buffers, the per-stage kernel
+// sequence, hipGraph capture of whole steps, and the problem part of the
+// C-ABI.  The stage sequence replaces advection/solveSubStep.m:57-99 with the
+// operators of advection/preprocessSubStep.m:56-99 applied matrix-free.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "grid.hpp"
+#include "kernels.hpp"
+#include "tables.hpp"
+
+namespace festung {
+
+#define HIPCHK(expr)                                                                           \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) fail(FESTUNG_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// Owning device allocation.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t b) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+    bytes = b;
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      fail(FESTUNG_ERR_ALLOC, "device allocation of %zu bytes failed: %s", b, hipGetErrorString(e));
+    }
+  }
+  double* d() const { return static_cast<double*>(p); }
+  int* i() const { return static_cast<int*>(p); }
+};
+
+struct StagePlan {
+  RKCoef rk;
+  int in, out;   // buffer ids: 0 = X (step start), 1 = Y, 2 = Z
+  bool accOut;
+};
+
+static std::vector<StagePlan> make_plan(int ordRK) {
+  std::vector<StagePlan> v;
+  auto ssp = [](double w) { return RKCoef{w, 1 - w, 1 - w, 0, 0, 0, 0}; };
+  switch (ordRK) {
+    case 1: v = {{ssp(0.0), 0, 1, false}}; break;                                     // copy back after
+    case 2: v = {{ssp(0.0), 0, 1, false}, {ssp(0.5), 1, 0, false}}; break;
+    case 3: v = {{ssp(0.0), 0, 1, false}, {ssp(3.0 / 4), 1, 2, false}, {ssp(1.0 / 3), 2, 0, false}}; break;
+    case 4:  // classic RK4 in accumulator form (extension)
+      v = {{RKCoef{1, 0, 0.5, 0, 1, 0, 1.0 / 6}, 0, 1, true},
+           {RKCoef{1, 0, 0.5, 0, 0, 1, 1.0 / 3}, 1, 2, true},
+           {RKCoef{1, 0, 1.0, 0, 0, 1, 1.0 / 3}, 2, 1, true},
+           {RKCoef{0, 0, 1.0 / 6, 1, 0, 0, 0}, 1, 0, false}};
+      break;
+    default: fail(FESTUNG_ERR_UNSUPPORTED, "Runge-Kutta order %d not supported (1..4)", ordRK);
+  }
+  return v;
+}
+
+}  // namespace festung
+
+using namespace festung;
+
+struct festung_problem {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int p = 0, N = 0, K = 0, numV = 0, ordRK = 0, flags = 0;
+  RefTables tabs;
+  TabOff off{};
+  std::vector<double> areaT;  // host copy (L2 error)
+  std::vector<StagePlan> plan;
+  DBuf tab, B, area, nbr, flg, V0T, v2tPtr, v2tIdx, nuE0T;
+  DBuf X, Y, Z, acc, T1, T2, cent, partials, gmm, counter, vmin, vmax, vertexD, u1, u2, vn, rhs, alpha;
+  bool hasRhs = false, hasVertexD = false, hasVelocity = false;
+  int cur = 0;           // buffer holding the current state (0 = X)
+  int nextStage = -1;    // next sub-step of a step begun with festung_begin_step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  double graphTau = std::numeric_limits<double>::quiet_NaN();
+  int nblocks = 0;
+  // per-launch timing (HIP events on the problem's stream)
+  bool prof = false;
+  struct Rec {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+
+  hipEvent_t get_event() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+  template <class F>
+  void timed(int kind, F&& f) {
+    if (!prof) {
+      f();
+      return;
+    }
+    Rec r{kind, get_event(), get_event()};
+    HIPCHK(hipEventRecord(r.a, stream));
+    f();
+    HIPCHK(hipEventRecord(r.b, stream));
+    recs.push_back(r);
+  }
+
+  double* buf(int id) const { return id == 0 ? X.d() : id == 1 ? Y.d() : Z.d(); }
+  Mesh mesh() const {
+    return Mesh{K, numV, B.d(), area.d(), nbr.i(), flg.i(), V0T.i(), v2tPtr.i(), v2tIdx.i()};
+  }
+  MinMaxSink sink() const {
+    return MinMaxSink{cent.d(), partials.d(), static_cast<unsigned*>(counter.p), gmm.d()};
+  }
+  void set_device() const { HIPCHK(hipSetDevice(device)); }
+  void drop_graph() {
+    if (graphExec) (void)hipGraphExecDestroy(graphExec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graphExec = nullptr;
+    graph = nullptr;
+    graphTau = std::numeric_limits<double>::quiet_NaN();
+  }
+  ~festung_problem() {
+    drop_graph();
+    for (auto& r : recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  template <class T>
+  void upload(DBuf& b, const T* h, size_t n) {
+    b.alloc(n * sizeof(T));
+    HIPCHK(hipMemcpy(b.p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  }
+
+  // vertex min / max for the centroid values just written to `cent`
+  void enqueue_vertex(bool withDirichlet) {
+    VertexArgs va{numV, v2tPtr.i(), v2tIdx.i(), cent.d(), gmm.d(),
+                  (withDirichlet && hasVertexD) ? vertexD.d() : nullptr, vmin.d(), vmax.d()};
+    timed(FESTUNG_KERNEL_VERTEX, [&] { HIPCHK(launch_vertex(va, stream)); });
+  }
+
+  // One RK stage: state buffer `in` -> `out`, step start in X.
+  void enqueue_stage(const StagePlan& sp, double tau) {
+    const bool lim = (flags & FESTUNG_FLAG_LIMITER) != 0;
+    const double* cin = buf(sp.in);
+    double* cout = buf(sp.out);
+    StageArgs a{};
+    a.m = mesh();
+    a.c = cin;
+    a.c0 = X.d();
+    a.acc = acc.d();
+    a.accOut = sp.accOut ? acc.d() : nullptr;
+    a.u1 = u1.d();
+    a.u2 = u2.d();
+    a.vn = vn.d();
+    a.rhs = hasRhs ? rhs.d() : nullptr;
+    a.tab = tab.d();
+    a.off = off;
+    a.tau = tau;
+    a.rk = sp.rk;
+    a.sink = sink();
+    if (!lim) {
+      a.out = cout;
+      timed(FESTUNG_KERNEL_STAGE, [&] { HIPCHK(launch_stage(p, OUT_RK_DG, a, stream)); });
+      return;
+    }
+    LimitArgs l{};
+    l.m = mesh();
+    l.vmin = vmin.d();
+    l.vmax = vmax.d();
+    l.tab = tab.d();
+    l.off = off;
+    l.sink = sink();
+    l.alphaOut = alpha.d();
+    if (p >= 2) {
+      // dc/dt limiting (advection/solveSubStep.m:69-79), then RK, then
+      // solution limiting (:86-97)
+      a.out = T1.d();
+      timed(FESTUNG_KERNEL_STAGE, [&] { HIPCHK(launch_stage(p, OUT_DOT_TAYLOR, a, stream)); });
+      enqueue_vertex(false);
+      l.taylor = T1.d();
+      l.c = cin;
+      l.c0 = X.d();
+      l.acc = acc.d();
+      l.accOut = sp.accOut ? acc.d() : nullptr;
+      l.tau = tau;
+      l.rk = sp.rk;
+      l.out = T2.d();
+      l.alphaOut = nullptr;
+      timed(FESTUNG_KERNEL_LIMIT_DOT, [&] { HIPCHK(launch_limit(p, LIM_DOT_RK_TAYLOR, l, stream)); });
+    } else {
+      a.out = T2.d();
+      timed(FESTUNG_KERNEL_STAGE, [&] { HIPCHK(launch_stage(p, OUT_RK_TAYLOR, a, stream)); });
+    }
+    enqueue_vertex(true);
+    l.taylor = T2.d();
+    l.out = cout;
+    l.alphaOut = alpha.d();
+    timed(FESTUNG_KERNEL_LIMIT, [&] { HIPCHK(launch_limit(p, LIM_SOLUTION, l, stream)); });
+  }
+
+  void enqueue_step(double tau) {
+    for (const auto& sp : plan) enqueue_stage(sp, tau);
+    if (plan.back().out != 0)
+      timed(FESTUNG_KERNEL_COPY, [&] {
+        HIPCHK(hipMemcpyAsync(X.p, buf(plan.back().out), size_t(N) * K * sizeof(double), hipMemcpyDeviceToDevice,
+                              stream));
+      });
+  }
+
+  void require_velocity() const {
+    if (!hasVelocity) fail(FESTUNG_ERR_STATE, "velocity not set (festung_set_velocity_disc / _points)");
+  }
+
+  void finish_stage_sequence() {
+    if (nextStage >= 0) fail(FESTUNG_ERR_STATE, "a step begun with festung_begin_step is not finished");
+  }
+};
+
+namespace {
+
+void check_handle(const festung_problem* h) {
+  if (!h) fail(FESTUNG_ERR_ARG, "null problem handle");
+}
+
+void sync(festung_problem* h) { HIPCHK(hipStreamSynchronize(h->stream)); }
+
+}  // namespace
+
+extern "C" {
+
+int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, festung_problem** out) {
+  return guard([&] {
+    if (!gh || !out) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    *out = nullptr;
+    if (p < 0 || p > 4) fail(FESTUNG_ERR_UNSUPPORTED, "polynomial degree p = %d (must be 0..4)", p);
+    if ((flags & FESTUNG_FLAG_LIMITER) && p < 1)
+      fail(FESTUNG_ERR_UNSUPPORTED, "slope limiting needs p >= 1 (advection/configureProblem.m:103)");
+    const Grid& g = gh->g;
+    std::unique_ptr<festung_problem> h(new festung_problem);
+    h->plan = make_plan(ordRK);
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) fail(FESTUNG_ERR_HIP, "no HIP device");
+    HIPCHK(hipGetDevice(&h->device));
+    HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->p = p;
+    h->ordRK = ordRK;
+    h->flags = flags;
+    h->tabs = build_ref_tables(p);
+    const int N = h->N = h->tabs.N;
+    const int K = h->K = g.numT;
+    h->numV = g.numV;
+    h->areaT = g.areaT;
+    if (int64_t(K) * N > (int64_t(1) << 31) - 1) fail(FESTUNG_ERR_ARG, "K*N exceeds int32 indexing");
+
+    // reference tables, one buffer
+    std::vector<double> tb;
+    auto put = [&](const std::vector<double>& v) {
+      int o = int(tb.size());
+      tb.insert(tb.end(), v.begin(), v.end());
+      return o;
+    };
+    const RefTables& t = h->tabs;
+    TabOff& o = h->off;
+    o.wE = put(t.ruleE.w);
+    o.phiE = put(t.phiE);
+    o.gradE = put(t.gradE);
+    o.wF = put(t.ruleF.w);
+    o.phiF = put(t.phiF);
+    o.wEdge = put(t.ruleEdge.w);
+    o.phi1D = put(t.phi1D);
+    o.theta = put(t.thetaPhi1D);
+    o.hatM = put(t.hatM);
+    o.hatMinv = put(t.hatMinv);
+    o.F = put(t.F);
+    o.Finv = put(t.Finv);
+    o.nu = put(t.nu);
+    o.total = int(tb.size());
+    h->upload(h->tab, tb.data(), tb.size());
+
+    // mesh
+    h->upload(h->B, g.B.data(), g.B.size());
+    h->upload(h->area, g.areaE0T.data(), g.areaE0T.size());
+    h->upload(h->nuE0T, g.nuE0T.data(), g.nuE0T.size());
+    h->upload(h->V0T, g.V0T.data(), g.V0T.size());
+    h->upload(h->v2tPtr, g.v2tPtr.data(), g.v2tPtr.size());
+    h->upload(h->v2tIdx, g.v2tIdx.data(), g.v2tIdx.size());
+    // neighbours and flags (markE0TE0T, markElemEdges, markE0TbdrN/D of
+    // core/assembleMatEdgePhiPhiValUpwind.m:261-299)
+    std::vector<int32_t> nb(3 * size_t(K)), fl(K, 0);
+    for (int32_t k = 0; k < K; ++k)
+      for (int n = 0; n < 3; ++n) {
+        size_t kn = k + size_t(n) * K;
+        const bool notN = !g.markE0TbdrN[kn];
+        const bool hasNb = g.nbr[kn] >= 0;
+        nb[kn] = (hasNb && notN) ? g.nbr[kn] : -1;
+        const bool markElemEdges = hasNb || g.idE0T[kn] != 0;
+        if (hasNb) fl[k] |= (g.nbrEdge[kn] & 3) << (2 * n);
+        if (notN && markElemEdges) fl[k] |= 1 << (6 + n);
+        if (g.markE0TbdrD[kn]) fl[k] |= 1 << (9 + n);
+      }
+    h->upload(h->nbr, nb.data(), nb.size());
+    h->upload(h->flg, fl.data(), fl.size());
+
+    // state and work buffers
+    const size_t nk = size_t(N) * K * sizeof(double);
+    h->X.alloc(nk);
+    h->Y.alloc(nk);
+    h->Z.alloc(nk);
+    if (ordRK == 4) h->acc.alloc(nk);
+    h->u1.alloc(nk);
+    h->u2.alloc(nk);
+    h->vn.alloc(size_t(3) * t.Re * K * sizeof(double));
+    HIPCHK(hipMemset(h->X.p, 0, nk));
+    h->nblocks = (K + block_size() - 1) / block_size();
+    if (flags & FESTUNG_FLAG_LIMITER) {
+      h->T1.alloc(nk);
+      h->T2.alloc(nk);
+      h->cent.alloc(size_t(K) * sizeof(double));
+      h->alpha.alloc(size_t(K) * sizeof(double));
+      h->vmin.alloc(size_t(g.numV) * sizeof(double));
+      h->vmax.alloc(size_t(g.numV) * sizeof(double));
+      h->vertexD.alloc(size_t(g.numV) * sizeof(double));
+    } else {
+      h->cent.alloc(size_t(K) * sizeof(double));
+    }
+    h->partials.alloc(size_t(2) * (h->nblocks + 1) * sizeof(double));
+    h->gmm.alloc(2 * sizeof(double));
+    h->counter.alloc(sizeof(unsigned));
+    HIPCHK(hipMemset(h->counter.p, 0, sizeof(unsigned)));
+    HIPCHK(hipDeviceSynchronize());
+    *out = h.release();
+  });
+}
+
+void festung_problem_destroy(festung_problem* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  delete h;
+}
+
+int festung_problem_sizes(const festung_problem* h, int32_t* K, int32_t* N, int32_t* RE, int32_t* RF, int32_t* Re) {
+  return guard([&] {
+    check_handle(h);
+    if (K) *K = h->K;
+    if (N) *N = h->N;
+    if (RE) *RE = h->tabs.RE;
+    if (RF) *RF = h->tabs.RF;
+    if (Re) *Re = h->tabs.Re;
+  });
+}
+
+int festung_problem_stream(const festung_problem* h, void** stream) {
+  return guard([&] {
+    check_handle(h);
+    if (!stream) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    *stream = h->stream;
+  });
+}
+
+int festung_set_velocity_disc(festung_problem* h, const double* u1Disc, const double* u2Disc, const double* vNormal) {
+  return guard([&] {
+    check_handle(h);
+    if (!u1Disc || !u2Disc || !vNormal) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    const size_t nk = size_t(h->N) * h->K * sizeof(double);
+    HIPCHK(hipMemcpy(h->u1.p, u1Disc, nk, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->u2.p, u2Disc, nk, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->vn.p, vNormal, size_t(3) * h->tabs.Re * h->K * sizeof(double), hipMemcpyHostToDevice));
+    h->hasVelocity = true;
+  });
+}
+
+int festung_set_velocity_points(festung_problem* h, const double* u1q, const double* u2q, const double* u1e,
+                                const double* u2e) {
+  return guard([&] {
+    check_handle(h);
+    if (!u1q || !u2q || !u1e || !u2e) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    const int K = h->K;
+    DBuf q, e1, e2;
+    h->upload(q, u1q, size_t(h->tabs.RE) * K);
+    HIPCHK(launch_project(h->p, false, K, q.d(), h->tab.d(), h->off, h->u1.d(), h->stream));
+    sync(h);
+    HIPCHK(hipMemcpy(q.p, u2q, size_t(h->tabs.RE) * K * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(launch_project(h->p, false, K, q.d(), h->tab.d(), h->off, h->u2.d(), h->stream));
+    h->upload(e1, u1e, size_t(3) * h->tabs.Re * K);
+    h->upload(e2, u2e, size_t(3) * h->tabs.Re * K);
+    HIPCHK(launch_vnormal(K, h->tabs.Re, h->nuE0T.d(), e1.d(), e2.d(), h->vn.d(), h->stream));
+    sync(h);
+    h->hasVelocity = true;
+  });
+}
+
+int festung_set_source_points(festung_problem* h, const double* fq, const double* cDe) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    if (!fq && !cDe) {
+      h->hasRhs = false;
+      return;
+    }
+    if (cDe) h->require_velocity();
+    const int K = h->K;
+    if (!h->rhs.p) h->rhs.alloc(size_t(h->N) * K * sizeof(double));
+    DBuf f, c;
+    if (fq) h->upload(f, fq, size_t(h->tabs.RE) * K);
+    if (cDe) h->upload(c, cDe, size_t(3) * h->tabs.Re * K);
+    HIPCHK(launch_source(h->p, h->mesh(), f.d(), c.d(), h->vn.d(), h->tab.d(), h->off, h->rhs.d(), h->stream));
+    sync(h);
+    h->hasRhs = true;
+  });
+}
+
+int festung_set_vertex_dirichlet(festung_problem* h, const double* vertexD) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    if (!(h->flags & FESTUNG_FLAG_LIMITER)) fail(FESTUNG_ERR_STATE, "problem created without the limiter");
+    if (!vertexD) {
+      h->hasVertexD = false;
+      return;
+    }
+    HIPCHK(hipMemcpy(h->vertexD.p, vertexD, size_t(h->numV) * sizeof(double), hipMemcpyHostToDevice));
+    h->hasVertexD = true;
+  });
+}
+
+int festung_project(festung_problem* h, int ord, const double* fq, double* cDisc_out, int set_state) {
+  return guard([&] {
+    check_handle(h);
+    if (!fq) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    bool ordF;
+    if (ord == h->tabs.ordE) ordF = false;
+    else if (ord == h->tabs.ordF) ordF = true;
+    else fail(FESTUNG_ERR_ARG, "projection order %d not available (use %d or %d)", ord, h->tabs.ordE, h->tabs.ordF);
+    const int R = ordF ? h->tabs.RF : h->tabs.RE;
+    DBuf q, o;
+    h->upload(q, fq, size_t(R) * h->K);
+    o.alloc(size_t(h->N) * h->K * sizeof(double));
+    HIPCHK(launch_project(h->p, ordF, h->K, q.d(), h->tab.d(), h->off, o.d(), h->stream));
+    sync(h);
+    if (cDisc_out) HIPCHK(hipMemcpy(cDisc_out, o.p, o.bytes, hipMemcpyDeviceToHost));
+    if (set_state) {
+      h->finish_stage_sequence();
+      HIPCHK(hipMemcpy(h->X.p, o.p, o.bytes, hipMemcpyDeviceToDevice));
+      h->cur = 0;
+    }
+  });
+}
+
+int festung_set_state(festung_problem* h, const double* c) {
+  return guard([&] {
+    check_handle(h);
+    if (!c) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    h->finish_stage_sequence();
+    HIPCHK(hipMemcpy(h->X.p, c, size_t(h->N) * h->K * sizeof(double), hipMemcpyHostToDevice));
+    h->cur = 0;
+  });
+}
+
+int festung_get_state(festung_problem* h, double* c) {
+  return guard([&] {
+    check_handle(h);
+    if (!c) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    sync(h);
+    HIPCHK(hipMemcpy(c, h->buf(h->cur), size_t(h->N) * h->K * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int festung_set_state_dev(festung_problem* h, const double* d) {
+  return guard([&] {
+    check_handle(h);
+    if (!d) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    h->finish_stage_sequence();
+    HIPCHK(hipMemcpyAsync(h->X.p, d, size_t(h->N) * h->K * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    h->cur = 0;
+  });
+}
+
+int festung_get_state_dev(festung_problem* h, double* d) {
+  return guard([&] {
+    check_handle(h);
+    if (!d) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    HIPCHK(hipMemcpyAsync(d, h->buf(h->cur), size_t(h->N) * h->K * sizeof(double), hipMemcpyDeviceToDevice,
+                          h->stream));
+  });
+}
+
+int festung_begin_step(festung_problem* h) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    h->require_velocity();
+    if (h->nextStage >= 0) fail(FESTUNG_ERR_STATE, "previous step not finished");
+    if (h->cur != 0) fail(FESTUNG_ERR_STATE, "state not in the step buffer");
+    h->nextStage = 0;
+  });
+}
+
+int festung_stage(festung_problem* h, int nSubStep, double tau) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    if (h->nextStage < 0) fail(FESTUNG_ERR_STATE, "festung_begin_step not called");
+    if (nSubStep != h->nextStage + 1) fail(FESTUNG_ERR_ARG, "expected sub-step %d, got %d", h->nextStage + 1, nSubStep);
+    const StagePlan& sp = h->plan[nSubStep - 1];
+    h->enqueue_stage(sp, tau);
+    h->cur = sp.out;
+    if (nSubStep == int(h->plan.size())) {
+      if (sp.out != 0)
+        HIPCHK(hipMemcpyAsync(h->X.p, h->buf(sp.out), size_t(h->N) * h->K * sizeof(double), hipMemcpyDeviceToDevice,
+                              h->stream));
+      h->cur = 0;
+      h->nextStage = -1;
+    } else {
+      h->nextStage = nSubStep;
+    }
+  });
+}
+
+int festung_step(festung_problem* h, int nsteps, double tau) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    h->require_velocity();
+    h->finish_stage_sequence();
+    if (nsteps < 0) fail(FESTUNG_ERR_ARG, "negative step count");
+    if (nsteps == 0) return;
+    if ((h->flags & FESTUNG_FLAG_GRAPH) && !h->prof) {
+      if (!h->graphExec || !(h->graphTau == tau)) {
+        h->drop_graph();
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        try {
+          h->enqueue_step(tau);
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(h->stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIPCHK(hipStreamEndCapture(h->stream, &h->graph));
+        HIPCHK(hipGraphInstantiate(&h->graphExec, h->graph, nullptr, nullptr, 0));
+        h->graphTau = tau;
+      }
+      for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(h->graphExec, h->stream));
+    } else {
+      for (int s = 0; s < nsteps; ++s) h->enqueue_step(tau);
+    }
+    h->cur = 0;
+  });
+}
+
+int festung_limit(festung_problem* h) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    if (!(h->flags & FESTUNG_FLAG_LIMITER)) fail(FESTUNG_ERR_STATE, "problem created without the limiter");
+    h->finish_stage_sequence();
+    ToTaylorArgs ta{h->K, h->B.d(), h->X.d(), h->tab.d(), h->off, h->T2.d(), h->sink()};
+    HIPCHK(launch_to_taylor(h->p, ta, h->stream));
+    h->enqueue_vertex(true);
+    LimitArgs l{};
+    l.m = h->mesh();
+    l.taylor = h->T2.d();
+    l.vmin = h->vmin.d();
+    l.vmax = h->vmax.d();
+    l.tab = h->tab.d();
+    l.off = h->off;
+    l.out = h->X.d();
+    l.sink = h->sink();
+    l.alphaOut = h->alpha.d();
+    HIPCHK(launch_limit(h->p, LIM_SOLUTION, l, h->stream));
+    sync(h);
+    h->cur = 0;
+  });
+}
+
+int festung_rhs(festung_problem* h, double* cDot) {
+  return guard([&] {
+    check_handle(h);
+    if (!cDot) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    h->require_velocity();
+    DBuf o;
+    o.alloc(size_t(h->N) * h->K * sizeof(double));
+    StageArgs a{};
+    a.m = h->mesh();
+    a.c = h->buf(h->cur);
+    a.c0 = h->X.d();
+    a.u1 = h->u1.d();
+    a.u2 = h->u2.d();
+    a.vn = h->vn.d();
+    a.rhs = h->hasRhs ? h->rhs.d() : nullptr;
+    a.tab = h->tab.d();
+    a.off = h->off;
+    a.out = o.d();
+    a.sink = h->sink();
+    HIPCHK(launch_stage(h->p, OUT_DOT_DG, a, h->stream));
+    sync(h);
+    HIPCHK(hipMemcpy(cDot, o.p, o.bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int festung_limiter_info(festung_problem* h, double* alphaE, double* minV0T, double* maxV0T) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    if (!(h->flags & FESTUNG_FLAG_LIMITER)) fail(FESTUNG_ERR_STATE, "problem created without the limiter");
+    sync(h);
+    const int K = h->K;
+    if (alphaE) HIPCHK(hipMemcpy(alphaE, h->alpha.p, size_t(K) * sizeof(double), hipMemcpyDeviceToHost));
+    if (minV0T || maxV0T) {
+      std::vector<int32_t> v0t(3 * size_t(K));
+      std::vector<double> vmn(h->numV), vmx(h->numV);
+      HIPCHK(hipMemcpy(v0t.data(), h->V0T.p, v0t.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(vmn.data(), h->vmin.p, vmn.size() * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(vmx.data(), h->vmax.p, vmx.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < v0t.size(); ++i) {
+        if (minV0T) minV0T[i] = vmn[v0t[i]];
+        if (maxV0T) maxV0T[i] = vmx[v0t[i]];
+      }
+    }
+  });
+}
+
+int festung_l2_error(festung_problem* h, int ord, const double* cq, double* err) {
+  return guard([&] {
+    check_handle(h);
+    if (!cq || !err) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    bool ordF;
+    if (ord == h->tabs.ordE) ordF = false;
+    else if (ord == h->tabs.ordF) ordF = true;
+    else fail(FESTUNG_ERR_ARG, "L2 error order %d not available (use %d or %d)", ord, h->tabs.ordE, h->tabs.ordF);
+    const int R = ordF ? h->tabs.RF : h->tabs.RE;
+    DBuf q, part;
+    h->upload(q, cq, size_t(R) * h->K);
+    part.alloc(size_t(h->nblocks) * sizeof(double));
+    HIPCHK(launch_l2(h->p, ordF, h->K, h->B.d(), h->buf(h->cur), q.d(), h->tab.d(), h->off, part.d(), h->stream));
+    sync(h);
+    std::vector<double> hp(h->nblocks);
+    HIPCHK(hipMemcpy(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double s = 0;
+    for (double v : hp) s += v;
+    *err = std::sqrt(s);
+  });
+}
+
+int festung_profile(festung_problem* h, int enable) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    sync(h);
+    for (auto& r : h->recs) {
+      h->pool.push_back(r.a);
+      h->pool.push_back(r.b);
+    }
+    h->recs.clear();
+    h->prof = enable != 0;
+  });
+}
+
+int festung_profile_read(festung_problem* h, double* ms, int64_t* count) {
+  return guard([&] {
+    check_handle(h);
+    if (!ms || !count) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    sync(h);
+    for (int i = 0; i < FESTUNG_KERNEL_KINDS; ++i) {
+      ms[i] = 0;
+      count[i] = 0;
+    }
+    for (auto& r : h->recs) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+      ms[r.kind] += t;
+      count[r.kind] += 1;
+      h->pool.push_back(r.a);
+      h->pool.push_back(r.b);
+    }
+    h->recs.clear();
+  });
+}
+
+int festung_sync(festung_problem* h) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    sync(h);
+  });
+}
+
+}  // extern "C"

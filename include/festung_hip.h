@@ -1,0 +1,216 @@
+/*
+ * festung_hip.h -- C-ABI of the MI355X (gfx950) implementation of FESTUNG's
+ * explicit DG advection step on triangles.
+ *
+ * Conventions
+ *  - Every function returns an int status (FESTUNG_OK == 0); the message of
+ *    the last failure on the calling thread is festung_last_error().
+ *  - Arrays are column-major with the reference's Matlab shapes, so a
+ *    Matlab/Octave gateway passes them without copies: a K x N coefficient
+ *    matrix cDisc is stored cDisc[k + i*K] (structure-of-arrays), a K x 3 x R
+ *    array valOnQuad is valOnQuad[k + n*K + r*3*K].
+ *  - Ids are 0-based int32, "none" is -1 (the gateway subtracts 1 from
+ *    Matlab's 1-based ids).
+ *  - Host pointers are read synchronously and never retained; the caller
+ *    owns them.  "_dev" entry points take device pointers instead (same
+ *    layout) and enqueue on the problem's stream.
+ *  - A problem handle is not re-entrant: calls on one handle must not
+ *    overlap.  Handles on different devices are independent.
+ */
+#ifndef FESTUNG_HIP_H
+#define FESTUNG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FESTUNG_OK = 0,
+  FESTUNG_ERR_ARG = 1,          /* invalid argument (null, size, range)       */
+  FESTUNG_ERR_HIP = 2,          /* HIP runtime error (no device, launch, ...) */
+  FESTUNG_ERR_ALLOC = 3,        /* host or device allocation failed           */
+  FESTUNG_ERR_STATE = 4,        /* call not valid in the handle's state       */
+  FESTUNG_ERR_UNSUPPORTED = 5,  /* p, RK order or option not supported        */
+  FESTUNG_ERR_INTERNAL = 6
+};
+
+typedef struct festung_grid festung_grid;
+typedef struct festung_problem festung_problem;
+
+/* Message of the last failing call on this thread ("" after a success). */
+const char* festung_last_error(void);
+/* Library version string. */
+const char* festung_version(void);
+
+/* ---------------------------------------------------------------- grid --
+ * Host-only (no device needed). */
+
+/* Friedrichs-Keller triangulation of [a,b]^2 with boundary ids and marks.
+ * Replaces core/domainSquare.m:67-105 (+ core/generateGridData.m:130-198,
+ * advection/preprocessProblem.m:72-76). */
+int festung_domain_square(double hmax, double a, double b, festung_grid** out);
+
+/* Connectivity and geometry of a counter-clockwise triangle mesh.
+ * coordV: numV x 2, V0T: numT x 3 (0-based).  Boundary ids are all 0 until
+ * festung_grid_set_boundary is called.
+ * Replaces core/generateGridData.m:130-198. */
+int festung_grid_create(const double* coordV, int32_t numV, const int32_t* V0T, int32_t numT,
+                        festung_grid** out);
+
+/* Set edge boundary ids idE (numE, 0 = interior) and optional Neumann marks
+ * markE0TbdrN (numT x 3 bytes, may be NULL); recomputes markE0Tint,
+ * markE0TbdrD and markV0TbdrD.
+ * Replaces advection/configureProblem.m:148-150 and
+ * advection/preprocessProblem.m:72-76. */
+int festung_grid_set_boundary(festung_grid* g, const int32_t* idE, const uint8_t* markE0TbdrN);
+
+int festung_grid_sizes(const festung_grid* g, int32_t* numT, int32_t* numV, int32_t* numE);
+
+/* Copy a named grid field out.  int32 fields: "V0T" (numT x 3), "V0E",
+ * "T0E", "E0E" (numE x 2), "E0T", "idE0T", "nbr", "nbrEdge" (numT x 3),
+ * "idE" (numE), "v2tPtr" (numV+1), "v2tIdx" (3 numT), "markE0Tint",
+ * "markE0TbdrD", "markE0TbdrN", "markV0TbdrD" (numT x 3, 0/1).
+ * double fields: "coordV" (numV x 2), "areaE" (numE), "nuE", "baryE"
+ * (numE x 2), "areaT" (numT), "baryT" (numT x 2), "B" (numT x 2 x 2),
+ * "areaE0T", "sigE0T" (numT x 3), "nuE0T", "coordV0T" (numT x 3 x 2).
+ * Mirrors the struct g of core/generateGridData.m:130-198. */
+int festung_grid_field_size(const festung_grid* g, const char* field, int64_t* count, int* is_double);
+int festung_grid_get_i32(const festung_grid* g, const char* field, int32_t* out);
+int festung_grid_get_f64(const festung_grid* g, const char* field, double* out);
+
+/* Physical coordinates of nq reference points on every element:
+ * X1, X2 are numT x nq.  Replaces g.mapRef2Phy (core/generateGridData.m:195-196). */
+int festung_grid_map_ref_to_phy(const festung_grid* g, int32_t nq, const double* q1, const double* q2,
+                                double* X1, double* X2);
+
+void festung_grid_destroy(festung_grid* g);
+
+/* ------------------------------------------------------ reference tables --
+ * Host-only.  Named tables for polynomial degree p (0..4):
+ *  "quad1D"   (R x 2: nodes, weights) for quadRule1D(ord),   core/quadRule1D.m:48-95
+ *  "quad2D"   (R x 3: Q1, Q2, W)      for quadRule2D(ord),   core/quadRule2D.m:64-109
+ *  "hatM" (N x N)                     core/integrateRefElemPhiPhi.m:49-59
+ *  "hatG" (N x N x N x 2)             core/integrateRefElemDphiPhiPhi.m:55-82
+ *  "hatRdiag" (N x N x 3 x R)         core/integrateRefEdgePhiIntPhiIntPerQuad.m:52-66
+ *  "hatRoff" (N x N x 3 x 3 x R)      core/integrateRefEdgePhiIntPhiExtPerQuad.m:54-70
+ *  "phi2D", "gradPhi2D", "phi1D", "thetaPhi1D" for order ord
+ *                                     core/computeBasesOnQuad.m:72-134
+ *  "taylorF", "taylorFinv" (N x N)    Taylor change-of-basis tables
+ * `ord` selects the quadrature order for the quad/phi tables (ignored
+ * otherwise).  festung_ref_size gives the element count. */
+int festung_ref_size(int p, const char* name, int ord, int64_t* count);
+int festung_ref_get(int p, const char* name, int ord, double* out);
+
+/* ------------------------------------------------------------- problem --
+ * Device-resident advection problem on the current HIP device. */
+
+enum {
+  FESTUNG_FLAG_LIMITER = 1,      /* Taylor-linear vertex-based slope limiter (typeSlopeLim 'linear') */
+  FESTUNG_FLAG_GRAPH = 2         /* capture whole steps into a hipGraph in festung_step */
+};
+
+/* Upload the grid (its connectivity and marks as they are now) and build the
+ * device tables for degree p (0..4) and RK order ordRK (1..3 SSP as in
+ * core/rungeKuttaExplicit.m:64-78; 4 = classic RK4, an extension).
+ * Replaces advection/preprocessProblem.m:48-112. */
+int festung_problem_create(const festung_grid* g, int p, int ordRK, int flags, festung_problem** out);
+void festung_problem_destroy(festung_problem* h);
+int festung_problem_sizes(const festung_problem* h, int32_t* K, int32_t* N, int32_t* RE, int32_t* RF,
+                          int32_t* Re);
+/* HIP stream the problem enqueues on (hipStream_t as void*). */
+int festung_problem_stream(const festung_problem* h, void** stream);
+
+/* Velocity as DG coefficients u1Disc, u2Disc (K x N) and normal velocity at
+ * the edge quadrature points vNormalOnQuadEdge (K x 3 x Re), as computed in
+ * advection/preprocessSubStep.m:71-79.  Host arrays. */
+int festung_set_velocity_disc(festung_problem* h, const double* u1Disc, const double* u2Disc,
+                              const double* vNormalOnQuadEdge);
+/* Velocity from point values: u1q, u2q at the element points of
+ * quadRule2D(2p) (K x RE, mapped by festung_grid_map_ref_to_phy), u1e, u2e
+ * at the edge points of quadRule1D(2p+1) (K x 3 x Re).  The device projects
+ * (core/projectFuncCont2DataDisc.m:94-102) and forms u.nu
+ * (core/computeFuncContNuOnQuadEdge.m:46-67). */
+int festung_set_velocity_points(festung_problem* h, const double* u1q, const double* u2q, const double* u1e,
+                                const double* u2e);
+
+/* Right-hand side of the stage: fq = f at the element points of
+ * quadRule2D(2p) (K x RE, may be NULL for f = 0) and cDe = c_D at the edge
+ * points of quadRule1D(2p+1) (K x 3 x Re, may be NULL for c_D = 0).  The
+ * device forms globL - globKD (advection/preprocessSubStep.m:69-98:
+ * core/projectFuncCont2DataDisc.m, core/assembleMatElemPhiPhi.m,
+ * core/assembleVecEdgePhiIntFuncContVal.m).  With both NULL the source is
+ * cleared.  Takes effect for the following stages. */
+int festung_set_source_points(festung_problem* h, const double* fq, const double* cDe);
+
+/* Dirichlet data at the vertices for the limiter: vertexD (numV values, NaN
+ * where not given), used for vertices with markV0TbdrD
+ * (core/computeMinMaxV0TElementPatch.m:81-82). NULL clears (all NaN). */
+int festung_set_vertex_dirichlet(festung_problem* h, const double* vertexD);
+
+/* L2 projection of point values fq (K x R of quadRule2D(ord), ord in
+ * {2p, 2p+1}) to DG coefficients (K x N): core/projectFuncCont2DataDisc.m:94-102.
+ * If set_state != 0 the result also becomes the current state. */
+int festung_project(festung_problem* h, int ord, const double* fq, double* cDisc_out, int set_state);
+
+/* Current solution cDisc (K x N). */
+int festung_set_state(festung_problem* h, const double* cDisc);
+int festung_get_state(festung_problem* h, double* cDisc);
+int festung_set_state_dev(festung_problem* h, const double* d_cDisc);
+int festung_get_state_dev(festung_problem* h, double* d_cDisc);
+
+/* One RK sub-step nSubStep (1-based) of the time step that started from the
+ * state saved by festung_begin_step: rhs, optional dc/dt limiting, stage
+ * combination, optional solution limiting.
+ * Replaces advection/solveSubStep.m:57-99 (with preprocessSubStep.m's
+ * operators applied matrix-free). */
+int festung_begin_step(festung_problem* h);
+int festung_stage(festung_problem* h, int nSubStep, double tau);
+
+/* nsteps complete time steps of size tau on the device (all RK stages; the
+ * source and vertex Dirichlet data stay as set).  Replaces the main loop
+ * main.m:151-166 over advection/solveStep.m:57-73 for time-independent data. */
+int festung_step(festung_problem* h, int nsteps, double tau);
+
+/* Apply the slope limiter to the current state.
+ * Replaces core/applySlopeLimiterDisc.m:77-86 ('linear'). */
+int festung_limit(festung_problem* h);
+
+/* Evaluate the DG right-hand side cDiscDot = M \ (sysV - sysA c) for the
+ * current state without limiting (K x N) (advection/solveSubStep.m:62-66). */
+int festung_rhs(festung_problem* h, double* cDiscDot);
+
+/* Per-element alpha and per-(element, vertex) min / max of the last
+ * limiter application (K, K x 3, K x 3).
+ * Mirrors the outputs of core/computeVertexBasedLimiter.m:78-106. */
+int festung_limiter_info(festung_problem* h, double* alphaE, double* minV0T, double* maxV0T);
+
+/* L2 error against point values cq of the exact solution at the element
+ * points of quadRule2D(ord) (K x R): core/computeL2Error.m:97-114. */
+int festung_l2_error(festung_problem* h, int ord, const double* cq, double* err);
+
+/* Per-launch timing with HIP events on the problem's stream (forces eager
+ * launches instead of the step graph while enabled).  festung_profile_read
+ * returns, per kernel kind, the summed milliseconds and launch counts since
+ * the last read, and clears them.  No counterpart in the reference (its
+ * only timing is tic/toc around the loop, main.m:151-166). */
+enum {
+  FESTUNG_KERNEL_STAGE = 0,      /* stage_kernel: rhs + mass solve + RK (+ Taylor) */
+  FESTUNG_KERNEL_VERTEX = 1,     /* vertex patch min / max                          */
+  FESTUNG_KERNEL_LIMIT_DOT = 2,  /* dc/dt limiting + RK + Taylor of the new state   */
+  FESTUNG_KERNEL_LIMIT = 3,      /* solution limiting                               */
+  FESTUNG_KERNEL_COPY = 4,       /* end-of-step copy (ordRK = 1 only)               */
+  FESTUNG_KERNEL_KINDS = 5
+};
+int festung_profile(festung_problem* h, int enable);
+int festung_profile_read(festung_problem* h, double* ms, int64_t* count);
+
+/* Synchronise the problem's stream. */
+int festung_sync(festung_problem* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FESTUNG_HIP_H */

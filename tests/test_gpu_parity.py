@@ -1,0 +1,147 @@
+"""Parity of the device path (through the C-ABI) with the oracle on the same
+seeded inputs: projection, the DG right-hand side, single RK steps with and
+without the slope limiter, multi-step runs, the analytical 'transient' case
+with time-dependent source and Dirichlet data, limiter outputs, and
+run-to-run determinism.  Tolerance: 1e-12 relative L2 per step (FP64)."""
+import numpy as np
+import pytest
+
+from festung_amd.advection import steps as S
+from festung_amd.advection import testcases as TC
+from helpers import oracle_problem, oracle_stage_operators, oracle_step, oracle_limit, rel_l2
+from oracle import advection as OA
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def product_problem(tc, p, hmax, ordRK, lim, numSteps=10, tEnd=None, graph=False):
+    pd = S.Struct(testcase='custom', p=p, hmax=hmax, ordRK=ordRK, isSlopeLim=lim, numSteps=numSteps,
+                  useGraph=graph)
+    if tEnd is not None:
+        pd.tEnd = tEnd
+    pd.update(tc)
+    pd = S.configureProblem(pd)
+    pd = S.preprocessProblem(pd)
+    pd = S.initializeProblem(pd)
+    return pd
+
+
+def product_step(pd, nStep):
+    pd.t, pd.omega = S.rungeKuttaExplicit(pd.ordRK, pd.tau, (nStep - 1) * pd.tau)
+    pd.isSubSteppingFinished = False
+    pd.problem.begin_step()
+    pd = S.iterateSubSteps(pd, nStep)
+    return pd.problem.get_state()
+
+
+CASES = [(1, 2, False), (2, 3, False), (1, 2, True), (2, 3, True), (3, 3, False), (3, 3, True),
+         (0, 1, False), (4, 3, False), (4, 3, True), (2, 4, False), (2, 4, True)]
+
+
+@pytest.mark.parametrize('p,ordRK,lim', CASES)
+def test_initial_projection(gpu, p, ordRK, lim):
+    tc = TC.rotating_humps()
+    o = oracle_problem(tc, p, 1 / 8, ordRK, lim)
+    d = product_problem(tc, p, 1 / 8, ordRK, lim)
+    c = d.problem.get_state()
+    assert rel_l2(c, o.cDisc) < TOL
+    assert abs(d.initialError - o.initialError) <= 1e-12 * max(o.initialError, 1e-300)
+
+
+@pytest.mark.parametrize('p', [0, 1, 2, 3, 4])
+def test_rhs(gpu, p):
+    tc = TC.rotating_humps()
+    o = oracle_problem(tc, p, 1 / 8, min(p + 1, 3), False)
+    d = product_problem(tc, p, 1 / 8, min(p + 1, 3), False)
+    rng = np.random.default_rng(7 + p)
+    c = rng.standard_normal((o.K, o.N))
+    o = oracle_stage_operators(o, 0.0)
+    ref = OA.computeDot(o, c.reshape(-1)).reshape(o.K, o.N)
+    d.t = np.array([0.0])
+    S.preprocessSubStep(d, 1, 1)
+    d.problem.set_state(c)
+    got = d.problem.rhs()
+    assert rel_l2(got, ref) < TOL
+
+
+@pytest.mark.parametrize('p,ordRK,lim', CASES)
+def test_one_step(gpu, p, ordRK, lim):
+    tc = TC.rotating_humps()
+    o = oracle_problem(tc, p, 1 / 8, ordRK, lim, numSteps=20, tEnd=0.5)
+    d = product_problem(tc, p, 1 / 8, ordRK, lim, numSteps=20, tEnd=0.5)
+    c = o.cDisc.copy()
+    d.problem.set_state(c)
+    got = product_step(d, 1)
+    ref = oracle_step(o, c, 1)
+    assert rel_l2(got, ref) < TOL
+
+
+@pytest.mark.parametrize('p,ordRK,lim,graph', [(1, 2, False, True), (2, 3, True, True), (2, 3, True, False),
+                                               (1, 2, True, True), (3, 3, True, True)])
+def test_main_loop(gpu, p, ordRK, lim, graph):
+    """20 steps through main('advection', ...) against the oracle's main."""
+    import festung_amd
+    tc = TC.rotating_humps()
+    kw = dict(p=p, hmax=1 / 12, ordRK=ordRK, isSlopeLim=lim, numSteps=20, tEnd=1.0)
+    d = festung_amd.main('advection', dict(testcase='custom', useGraph=graph, **tc, **kw))
+    ocfg = OA.Struct(testcase='custom', **kw)
+    for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):
+        ocfg[k] = tc[k]
+    o = OA.main('advection', ocfg)
+    assert rel_l2(d.cDisc, o.cDisc) < 20 * TOL
+    assert abs(d.error - o.error) <= 1e-10 * o.error
+
+
+@pytest.mark.parametrize('p,lim', [(1, False), (2, False), (2, True)])
+def test_transient_source(gpu, p, lim):
+    """Time-dependent f and c_D ('transient', advection/configureProblem.m:119-132)."""
+    import festung_amd
+    kw = dict(p=p, hmax=1 / 8, isSlopeLim=lim, numSteps=8, tEnd=0.08)
+    d = festung_amd.main('advection', dict(testcase='transient', **kw))
+    o = OA.main('advection', OA.Struct(testcase='transient', **kw))
+    assert rel_l2(d.cDisc, o.cDisc) < 10 * TOL
+    assert abs(d.error - o.error) <= 1e-10 * o.error
+
+
+@pytest.mark.parametrize('p', [1, 2, 3, 4])
+def test_limiter_alone(gpu, p):
+    """applySlopeLimiterDisc on a rough state: result, alpha and patch bounds."""
+    tc = TC.rotating_humps()
+    o = oracle_problem(tc, p, 1 / 8, min(p + 1, 3), True)
+    d = product_problem(tc, p, 1 / 8, min(p + 1, 3), True)
+    rng = np.random.default_rng(11 + p)
+    c = o.cDisc + 0.05 * rng.standard_normal(o.cDisc.shape)
+    ref, (mn, mx) = oracle_limit(o, c, 0.0)
+    d.problem.set_state(c)
+    d.problem.limit()
+    got = d.problem.get_state()
+    alpha, gmn, gmx = d.problem.limiter_info()
+    assert rel_l2(got, ref) < TOL
+    assert np.max(np.abs(gmn - mn)) < 1e-14 and np.max(np.abs(gmx - mx)) < 1e-14
+    assert np.all((alpha >= 0) & (alpha <= 1))
+
+
+def test_determinism(gpu):
+    tc = TC.rotating_humps()
+    outs = []
+    for _ in range(2):
+        d = product_problem(tc, 2, 1 / 16, 3, True, graph=True)
+        for n in range(1, 4):
+            product_step(d, n)
+        d.problem.step(3, d.tau)
+        outs.append(d.problem.get_state())
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_graph_matches_stagewise(gpu):
+    tc = TC.rotating_humps()
+    a = product_problem(tc, 2, 1 / 16, 3, True, graph=True)
+    b = product_problem(tc, 2, 1 / 16, 3, True, graph=False)
+    product_step(a, 1)
+    product_step(b, 1)
+    a.problem.step(4, a.tau)
+    for n in range(2, 6):
+        product_step(b, n)
+    assert np.array_equal(a.problem.get_state(), b.problem.get_state())
