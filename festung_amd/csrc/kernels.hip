@@ -38,8 +38,9 @@ __device__ __forceinline__ int gtid() { return blockIdx.x * BLOCK + threadIdx.x;
 // derivative multi-indices.  Row b (target derivative d1^(D-b) d2^b) holds
 // the coefficients of (A11 X + A21)^(D-b) (A12 X + A22)^b, column b' the
 // coefficient of X^(D-b') (source derivative d1^(D-b') d2^b').
+// y[0 .. D] = E_D(A) x[0 .. D], each row formed and contracted in turn
 template <int D>
-__device__ __forceinline__ void sym_power(double A11, double A21, double A12, double A22, double (&E)[D + 1][D + 1]) {
+__device__ __forceinline__ void apply_sym(double A11, double A21, double A12, double A22, const double* x, double* y) {
 #pragma unroll
   for (int b = 0; b <= D; ++b) {
     double pw[D + 1];
@@ -54,21 +55,9 @@ __device__ __forceinline__ void sym_power(double A11, double A21, double A12, do
       for (int i = D; i >= 1; --i) pw[i] = pw[i] * c0 + pw[i - 1] * c1;
       pw[0] = pw[0] * c0;
     }
-#pragma unroll
-    for (int bp = 0; bp <= D; ++bp) E[b][bp] = pw[D - bp];
-  }
-}
-
-// y[off .. off+D] = E_D(A) x[off .. off+D]
-template <int D>
-__device__ __forceinline__ void apply_sym(double A11, double A21, double A12, double A22, const double* x, double* y) {
-  double E[D + 1][D + 1];
-  sym_power<D>(A11, A21, A12, A22, E);
-#pragma unroll
-  for (int b = 0; b <= D; ++b) {
     double s = 0.0;
 #pragma unroll
-    for (int bp = 0; bp <= D; ++bp) s += E[b][bp] * x[bp];
+    for (int bp = 0; bp <= D; ++bp) s += pw[D - bp] * x[bp];
     y[b] = s;
   }
 }
@@ -136,13 +125,26 @@ __device__ __forceinline__ void taylor_to_dg(const double (&T)[Dims<P>::N], cons
 // a + b <= 2P, stored at d(d+1)/2 + b (d = a + b).
 template <int P, int D>
 __device__ __forceinline__ void moments_deg(const Geo<P>& g, const double* __restrict__ nu, double* mu) {
-  double E[D + 1][D + 1];
-  sym_power<D>(g.dX * g.B11, g.dX * g.B12, g.dX * g.B21, g.dX * g.B22, E);
+  // xi^a eta^b = (dX B11 s + dX B12 t)^a (dX B21 s + dX B22 t)^b, s, t centred
+  // reference coordinates; contract the coefficients with the reference moments
+  const double A11 = g.dX * g.B11, A21 = g.dX * g.B12, A12 = g.dX * g.B21, A22 = g.dX * g.B22;
 #pragma unroll
   for (int b = 0; b <= D; ++b) {
+    double pw[D + 1];
+    pw[0] = 1.0;
+#pragma unroll
+    for (int i = 1; i <= D; ++i) pw[i] = 0.0;
+#pragma unroll
+    for (int t = 0; t < D; ++t) {
+      const double c0 = t < D - b ? A21 : A22;
+      const double c1 = t < D - b ? A11 : A12;
+#pragma unroll
+      for (int i = D; i >= 1; --i) pw[i] = pw[i] * c0 + pw[i - 1] * c1;
+      pw[0] = pw[0] * c0;
+    }
     double s = 0.0;
 #pragma unroll
-    for (int bp = 0; bp <= D; ++bp) s += E[b][bp] * nu[(D - bp) * 9 + bp];
+    for (int bp = 0; bp <= D; ++bp) s += pw[D - bp] * nu[(D - bp) * 9 + bp];
     mu[D * (D + 1) / 2 + b] = s;
   }
 }
@@ -161,19 +163,23 @@ __device__ __forceinline__ void element_moments(const Geo<P>& g, const double* _
   if constexpr (2 * P >= 8) moments_deg<P, 8>(g, nu, mu);
 }
 
+// Taylor index i -> total degree d, exponent b of the second variable, and
+// a! b! (a = d - b), for i = 0 .. 14 in the order (0,0),(1,0),(0,1),(2,0),...
 __host__ __device__ constexpr int tdeg(int i) { return i == 0 ? 0 : i < 3 ? 1 : i < 6 ? 2 : i < 10 ? 3 : 4; }
 __host__ __device__ constexpr int tb(int i) { return i - tdeg(i) * (tdeg(i) + 1) / 2; }
-__host__ __device__ constexpr double tfact(int n) { return n <= 1 ? 1.0 : n * tfact(n - 1); }
+__host__ __device__ constexpr double fact4(int n) { return n == 4 ? 24.0 : n == 3 ? 6.0 : n == 2 ? 2.0 : 1.0; }
+__host__ __device__ constexpr double tfab(int i) { return fact4(tdeg(i) - tb(i)) * fact4(tb(i)); }
 
 // ------------------------------------------------------ min / max reduce --
-__device__ __forceinline__ void minmax_block_finish(double mn, double mx, const MinMaxSink& s) {
+// Block min / max of the centroid values -> partials[2 blockIdx .. +1].
+// Plain stores: the consumer is a later kernel on the same stream.
+__device__ __forceinline__ void minmax_block_partial(double mn, double mx, const MinMaxSink& s) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     mn = fmin(mn, __shfl_xor(mn, off));
     mx = fmax(mx, __shfl_xor(mx, off));
   }
   __shared__ double smn[BLOCK / 64], smx[BLOCK / 64];
-  __shared__ int isLast;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) {
     smn[w] = mn;
@@ -189,40 +195,32 @@ __device__ __forceinline__ void minmax_block_finish(double mn, double mx, const 
     }
     s.partials[2 * blockIdx.x] = a;
     s.partials[2 * blockIdx.x + 1] = b;
-    __threadfence();
-    unsigned t = atomicAdd(s.counter, 1u);
-    isLast = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (isLast) {
-    __threadfence();
-    double a = INFINITY, b = -INFINITY;
-    const volatile double* pp = s.partials;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += BLOCK) {
-      a = fmin(a, pp[2 * i]);
-      b = fmax(b, pp[2 * i + 1]);
-    }
+}
+
+// ------------------------------------------------------------------- RK --
+// out = a0 c0 + a1 c + b tau cdot + g acc;  accOut = d0 c0 + d1 acc + e tau cdot
+template <int N>
+__device__ __forceinline__ void rk_combine(const RKCoef& rk, double tau, const double (&c)[N], const double (&cdot)[N],
+                                           const double* __restrict__ c0, const double* __restrict__ accIn,
+                                           double* __restrict__ accOut, int K, int kk, bool active, double (&nx)[N]) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      a = fmin(a, __shfl_xor(a, off));
-      b = fmax(b, __shfl_xor(b, off));
-    }
-    __syncthreads();
-    if (lane == 0) {
-      smn[w] = a;
-      smx[w] = b;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
+  for (int i = 0; i < N; ++i) nx[i] = rk.a1 * c[i] + rk.b * tau * cdot[i];
+  if (rk.a0 != 0.0) {
 #pragma unroll
-      for (int i = 1; i < BLOCK / 64; ++i) {
-        a = fmin(a, smn[i]);
-        b = fmax(b, smx[i]);
-      }
-      s.gmm[0] = a;
-      s.gmm[1] = b;
-      *s.counter = 0u;
-      __threadfence();
+    for (int i = 0; i < N; ++i) nx[i] += rk.a0 * c0[i * K + kk];
+  }
+  if (rk.g != 0.0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) nx[i] += rk.g * accIn[i * K + kk];
+  }
+  if (accOut && active) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double v = rk.e * tau * cdot[i];
+      if (rk.d0 != 0.0) v += rk.d0 * c0[i * K + kk];
+      if (rk.d1 != 0.0) v += rk.d1 * accIn[i * K + kk];
+      accOut[i * K + kk] = v;
     }
   }
 }
@@ -277,7 +275,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 
   // edge term  -R c  (upwind; interior neighbours gathered, no atomics)
   const int fl = a.m.flags[kk];
-#pragma unroll
+#pragma unroll 1
   for (int n = 0; n < 3; ++n) {
     const double area = a.m.areaE[n * K + kk];
     const int nb = a.m.nbr[n * K + kk];
@@ -333,35 +331,11 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
       for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
       a.sink.centroid[k] = Tc[0];
     }
-    minmax_block_finish(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+    minmax_block_partial(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
   } else {
     // RK combination (advection/solveSubStep.m:83 for the SSP weights)
     double nx[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) nx[i] = a.rk.a1 * c[i] + a.rk.b * a.tau * cdot[i];
-    if (a.rk.a0 != 0.0 || a.rk.d0 != 0.0) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const double c0 = a.c0[i * K + kk];
-        nx[i] += a.rk.a0 * c0;
-        acc[i] = a.rk.d0 * c0;
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < N; ++i) acc[i] = 0.0;
-    }
-    if (a.rk.g != 0.0 || a.rk.d1 != 0.0) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const double ac = a.acc[i * K + kk];
-        nx[i] += a.rk.g * ac;
-        acc[i] += a.rk.d1 * ac;
-      }
-    }
-    if (a.accOut && active) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) a.accOut[i * K + k] = acc[i] + a.rk.e * a.tau * cdot[i];
-    }
+    rk_combine<N>(a.rk, a.tau, c, cdot, a.c0, a.acc, a.accOut, K, kk, active, nx);
     if constexpr (MODE == OUT_RK_DG) {
       if (active)
 #pragma unroll
@@ -374,7 +348,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
         for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
         a.sink.centroid[k] = Tc[0];
       }
-      minmax_block_finish(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+      minmax_block_partial(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
     }
   }
 }
@@ -382,26 +356,45 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 // -------------------------------------------------------- vertex min/max --
 
 __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
+  if (blockIdx.x == 0) {
+    // global min / max of the centroid values from the producer's partials
+    double mn = INFINITY, mx = -INFINITY;
+    for (int i = threadIdx.x; i < a.nPartials; i += BLOCK) {
+      mn = fmin(mn, a.partials[2 * i]);
+      mx = fmax(mx, a.partials[2 * i + 1]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      mn = fmin(mn, __shfl_xor(mn, off));
+      mx = fmax(mx, __shfl_xor(mx, off));
+    }
+    __shared__ double smn[BLOCK / 64], smx[BLOCK / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+      smn[w] = mn;
+      smx[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int i = 1; i < BLOCK / 64; ++i) {
+        mn = fmin(mn, smn[i]);
+        mx = fmax(mx, smx[i]);
+      }
+      a.gmm[0] = mn;
+      a.gmm[1] = mx;
+    }
+  }
   const int v = gtid();
   if (v >= a.numV) return;
-  const double sNeg = fabs(a.gmm[1]) + 1.0;  // shiftCentroidToNeg
-  const double sPos = fabs(a.gmm[0]) + 1.0;  // shiftCentroidToPos
   double m = INFINITY, M = -INFINITY;
   for (int i = a.v2tPtr[v]; i < a.v2tPtr[v + 1]; ++i) {
     const double c = a.centroid[a.v2tIdx[i]];
     m = fmin(m, c);
     M = fmax(M, c);
   }
-  // min over the patch of (c - sNeg), then + sNeg (rounding of the reference kept)
-  double lo = (m - sNeg) + sNeg;
-  double hi = (M + sPos) - sPos;
-  if (a.vertexD) {
-    const double d = a.vertexD[v];  // NaN where not a Dirichlet vertex
-    lo = fmin(lo, d);
-    hi = fmax(hi, d);
-  }
-  a.vmin[v] = lo;
-  a.vmax[v] = hi;
+  a.vmin[v] = m;
+  a.vmax[v] = M;
 }
 
 // ----------------------------------------------------------------- limit --
@@ -428,14 +421,26 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
   const double py[3] = {-ey, g.B21 - ey, g.B22 - ey};
   const double xc = x[0];
   const double tol = 1.e-8;
+  // patch bounds with the reference's global shift: min over the patch of
+  // (c - sNeg), then + sNeg (and max with sPos); the Dirichlet vertex value
+  // joins NaN-ignoring afterwards (core/computeMinMaxV0TElementPatch.m:75-82,107-110)
+  const double sNeg = fabs(a.gmm[1]) + 1.0;
+  const double sPos = fabs(a.gmm[0]) + 1.0;
   double alpha = 1.0;
 #pragma unroll
   for (int n = 0; n < 3; ++n) {
     const double val = x[0] + x[1] * (px[n] * g.dX) + x[2] * (py[n] * g.dX);
     const int v = a.m.V0T[n * K + kk];
+    double lo = (a.vmin[v] - sNeg) + sNeg;
+    double hi = (a.vmax[v] + sPos) - sPos;
+    if (a.vertexD) {
+      const double dv = a.vertexD[v];
+      lo = fmin(lo, dv);
+      hi = fmax(hi, dv);
+    }
     const double d = val - xc;
-    const double dMin = a.vmin[v] - xc;
-    const double dMax = a.vmax[v] - xc;
+    const double dMin = lo - xc;
+    const double dMax = hi - xc;
     double al = 1.0;
     if (d < dMin + tol) al = fmax(0.0, fmin(1.0, dMin / (d - tol)));
     if (d > dMax - tol) al = fmax(0.0, fmin(1.0, dMax / (d + tol)));
@@ -457,26 +462,29 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
       constexpr int NM = (2 * P + 1) * (2 * P + 2) / 2;
       double mu[NM];
       element_moments<P>(g, T + a.off.nu, mu);
-      double diff[N];
+      // corr_i = f_i / (mu_2i - mu_i^2) * (sum_j mu_(i+j) d_j - mu_i sum_j mu_j d_j),
+      // d_j = (orig_j - lim_j) / f_j, f = a! b!: row i of diag(MT)^-1 MT applied
+      // to (orig - lim), with MT the element's Taylor mass matrix
+      double dj[N];
 #pragma unroll
-      for (int j = 0; j < N; ++j) diff[j] = x[j] - lim[j];
+      for (int j = 1; j < N; ++j) dj[j] = (x[j] - lim[j]) * (1.0 / tfab(j));
+      double S0 = 0.0;
+#pragma unroll
+      for (int j = 1; j < N; ++j) S0 += mu[j] * dj[j];
       double corr[N];
 #pragma unroll
       for (int i = 1; i < N; ++i) {
-        const int di = tdeg(i), bi = tb(i), ai = di - bi;
-        const double mui = mu[i];
-        const int i2 = (2 * di) * (2 * di + 1) / 2 + 2 * bi;
-        const double fi = tfact(ai) * tfact(bi);
-        const double den = (mu[i2] - mui * mui) / (fi * fi);
+        const int di = tdeg(i), bi = tb(i);
         double s = 0.0;
 #pragma unroll
         for (int j = 1; j < N; ++j) {
-          const int dj = tdeg(j), bj = tb(j), aj = dj - bj;
-          const int ij = (di + dj) * (di + dj + 1) / 2 + (bi + bj);
-          const double fj = tfact(aj) * tfact(bj);
-          s += ((mu[ij] - mui * mu[j]) / (fi * fj)) / den * diff[j];
+          const int dj_ = tdeg(j), bj = tb(j);
+          s += mu[(di + dj_) * (di + dj_ + 1) / 2 + (bi + bj)] * dj[j];
         }
-        corr[i] = s;
+        s -= mu[i] * S0;
+        const double fi = tfab(i);
+        const double den = mu[(2 * di) * (2 * di + 1) / 2 + 2 * bi] - mu[i] * mu[i];
+        corr[i] = s * fi / den;
       }
 #pragma unroll
       for (int i = 1; i < N; ++i) lim[i] += corr[i];
@@ -490,32 +498,10 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
 #pragma unroll
       for (int i = 0; i < N; ++i) a.out[i * K + k] = y[i];
   } else {
-    double nx[N], acc[N];
+    double cs[N], nx[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) nx[i] = a.rk.a1 * a.c[i * K + kk] + a.rk.b * a.tau * y[i];
-    if (a.rk.a0 != 0.0 || a.rk.d0 != 0.0) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const double c0 = a.c0[i * K + kk];
-        nx[i] += a.rk.a0 * c0;
-        acc[i] = a.rk.d0 * c0;
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < N; ++i) acc[i] = 0.0;
-    }
-    if (a.rk.g != 0.0 || a.rk.d1 != 0.0) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const double ac = a.acc[i * K + kk];
-        nx[i] += a.rk.g * ac;
-        acc[i] += a.rk.d1 * ac;
-      }
-    }
-    if (a.accOut && active) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) a.accOut[i * K + k] = acc[i] + a.rk.e * a.tau * y[i];
-    }
+    for (int i = 0; i < N; ++i) cs[i] = a.c[i * K + kk];
+    rk_combine<N>(a.rk, a.tau, cs, y, a.c0, a.acc, a.accOut, K, kk, active, nx);
     if constexpr (MODE == LIM_DOT_RK_DG) {
       if (active)
 #pragma unroll
@@ -528,7 +514,7 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
         for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
         a.sink.centroid[k] = Tc[0];
       }
-      minmax_block_finish(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+      minmax_block_partial(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
     }
   }
 }
@@ -553,7 +539,7 @@ __global__ __launch_bounds__(BLOCK) void to_taylor_kernel(ToTaylorArgs a) {
     for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
     a.sink.centroid[k] = Tc[0];
   }
-  minmax_block_finish(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+  minmax_block_partial(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
 }
 
 // ------------------------------------------------------ setup kernels --

@@ -40,14 +40,12 @@ struct RKCoef {
   double a0, a1, b, g, d0, d1, e;
 };
 
-// Global min / max of the centroid values of one limiter application, by a
-// single-pass block reduction: per-block partials, the last block to finish
-// folds them into gmm[0] = min, gmm[1] = max and resets the counter.
+// Centroid values of one limiter application and their per-block min / max
+// (partials[2b], partials[2b+1]); the vertex kernel, launched after the
+// producer, folds the partials into the global min / max.
 struct MinMaxSink {
   double* centroid;   // K
-  double* partials;   // 2 x gridDim.x
-  unsigned* counter;  // 1, zero between launches
-  double* gmm;        // 2
+  double* partials;   // 2 x gridDim.x of the producer
 };
 
 struct Mesh {
@@ -87,17 +85,20 @@ struct VertexArgs {
   const int* v2tPtr;
   const int* v2tIdx;
   const double* centroid;
-  const double* gmm;
-  const double* vertexD;  // numV or null
-  double* vmin;
+  const double* partials;  // producer's per-block min / max
+  int nPartials;
+  double* gmm;             // out: global min, max (written by block 0)
+  double* vmin;            // out: raw patch min / max per vertex
   double* vmax;
 };
 
 struct LimitArgs {
   Mesh m;
   const double* taylor;  // Taylor coefficients to limit (N x K)
-  const double* vmin;
+  const double* vmin;    // raw patch min / max per vertex
   const double* vmax;
+  const double* gmm;     // global min / max of the centroid values
+  const double* vertexD; // Dirichlet vertex values (NaN = none) or null
   const double* tab;
   TabOff off;
   // RK combination (dc/dt modes)

@@ -88,7 +88,7 @@ struct festung_problem {
   std::vector<double> areaT;  // host copy (L2 error)
   std::vector<StagePlan> plan;
   DBuf tab, B, area, nbr, flg, V0T, v2tPtr, v2tIdx, nuE0T;
-  DBuf X, Y, Z, acc, T1, T2, cent, partials, gmm, counter, vmin, vmax, vertexD, u1, u2, vn, rhs, alpha;
+  DBuf X, Y, Z, acc, T1, T2, cent, partials, gmm, vmin, vmax, vertexD, u1, u2, vn, rhs, alpha;
   bool hasRhs = false, hasVertexD = false, hasVelocity = false;
   int cur = 0;           // buffer holding the current state (0 = X)
   int nextStage = -1;    // next sub-step of a step begun with festung_begin_step
@@ -133,7 +133,7 @@ struct festung_problem {
     return Mesh{K, numV, B.d(), area.d(), nbr.i(), flg.i(), V0T.i(), v2tPtr.i(), v2tIdx.i()};
   }
   MinMaxSink sink() const {
-    return MinMaxSink{cent.d(), partials.d(), static_cast<unsigned*>(counter.p), gmm.d()};
+    return MinMaxSink{cent.d(), partials.d()};
   }
   void set_device() const { HIPCHK(hipSetDevice(device)); }
   void drop_graph() {
@@ -159,10 +159,10 @@ struct festung_problem {
     HIPCHK(hipMemcpy(b.p, h, n * sizeof(T), hipMemcpyHostToDevice));
   }
 
-  // vertex min / max for the centroid values just written to `cent`
-  void enqueue_vertex(bool withDirichlet) {
-    VertexArgs va{numV, v2tPtr.i(), v2tIdx.i(), cent.d(), gmm.d(),
-                  (withDirichlet && hasVertexD) ? vertexD.d() : nullptr, vmin.d(), vmax.d()};
+  // raw vertex-patch min / max of the centroid values just written to `cent`,
+  // and the global min / max from the producer's per-block partials
+  void enqueue_vertex() {
+    VertexArgs va{numV, v2tPtr.i(), v2tIdx.i(), cent.d(), partials.d(), nblocks, gmm.d(), vmin.d(), vmax.d()};
     timed(FESTUNG_KERNEL_VERTEX, [&] { HIPCHK(launch_vertex(va, stream)); });
   }
 
@@ -195,6 +195,8 @@ struct festung_problem {
     l.m = mesh();
     l.vmin = vmin.d();
     l.vmax = vmax.d();
+    l.gmm = gmm.d();
+    l.vertexD = nullptr;
     l.tab = tab.d();
     l.off = off;
     l.sink = sink();
@@ -204,7 +206,7 @@ struct festung_problem {
       // solution limiting (:86-97)
       a.out = T1.d();
       timed(FESTUNG_KERNEL_STAGE, [&] { HIPCHK(launch_stage(p, OUT_DOT_TAYLOR, a, stream)); });
-      enqueue_vertex(false);
+      enqueue_vertex();
       l.taylor = T1.d();
       l.c = cin;
       l.c0 = X.d();
@@ -219,9 +221,10 @@ struct festung_problem {
       a.out = T2.d();
       timed(FESTUNG_KERNEL_STAGE, [&] { HIPCHK(launch_stage(p, OUT_RK_TAYLOR, a, stream)); });
     }
-    enqueue_vertex(true);
+    enqueue_vertex();
     l.taylor = T2.d();
     l.out = cout;
+    l.vertexD = hasVertexD ? vertexD.d() : nullptr;
     l.alphaOut = alpha.d();
     timed(FESTUNG_KERNEL_LIMIT, [&] { HIPCHK(launch_limit(p, LIM_SOLUTION, l, stream)); });
   }
@@ -354,8 +357,6 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     }
     h->partials.alloc(size_t(2) * (h->nblocks + 1) * sizeof(double));
     h->gmm.alloc(2 * sizeof(double));
-    h->counter.alloc(sizeof(unsigned));
-    HIPCHK(hipMemset(h->counter.p, 0, sizeof(unsigned)));
     HIPCHK(hipDeviceSynchronize());
     *out = h.release();
   });
@@ -525,7 +526,6 @@ int festung_begin_step(festung_problem* h) {
   return guard([&] {
     check_handle(h);
     h->set_device();
-    h->require_velocity();
     if (h->nextStage >= 0) fail(FESTUNG_ERR_STATE, "previous step not finished");
     if (h->cur != 0) fail(FESTUNG_ERR_STATE, "state not in the step buffer");
     h->nextStage = 0;
@@ -537,6 +537,7 @@ int festung_stage(festung_problem* h, int nSubStep, double tau) {
     check_handle(h);
     h->set_device();
     if (h->nextStage < 0) fail(FESTUNG_ERR_STATE, "festung_begin_step not called");
+    h->require_velocity();
     if (nSubStep != h->nextStage + 1) fail(FESTUNG_ERR_ARG, "expected sub-step %d, got %d", h->nextStage + 1, nSubStep);
     const StagePlan& sp = h->plan[nSubStep - 1];
     h->enqueue_stage(sp, tau);
@@ -593,12 +594,14 @@ int festung_limit(festung_problem* h) {
     h->finish_stage_sequence();
     ToTaylorArgs ta{h->K, h->B.d(), h->X.d(), h->tab.d(), h->off, h->T2.d(), h->sink()};
     HIPCHK(launch_to_taylor(h->p, ta, h->stream));
-    h->enqueue_vertex(true);
+    h->enqueue_vertex();
     LimitArgs l{};
     l.m = h->mesh();
     l.taylor = h->T2.d();
     l.vmin = h->vmin.d();
     l.vmax = h->vmax.d();
+    l.gmm = h->gmm.d();
+    l.vertexD = h->hasVertexD ? h->vertexD.d() : nullptr;
     l.tab = h->tab.d();
     l.off = h->off;
     l.out = h->X.d();
@@ -645,14 +648,24 @@ int festung_limiter_info(festung_problem* h, double* alphaE, double* minV0T, dou
     const int K = h->K;
     if (alphaE) HIPCHK(hipMemcpy(alphaE, h->alpha.p, size_t(K) * sizeof(double), hipMemcpyDeviceToHost));
     if (minV0T || maxV0T) {
+      // bounds of the last (solution) limiter application, as the limit
+      // kernel forms them: shifted patch min / max joined with c_D
       std::vector<int32_t> v0t(3 * size_t(K));
-      std::vector<double> vmn(h->numV), vmx(h->numV);
+      std::vector<double> vmn(h->numV), vmx(h->numV), vD(h->numV, std::nan("")), gm(2);
       HIPCHK(hipMemcpy(v0t.data(), h->V0T.p, v0t.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(vmn.data(), h->vmin.p, vmn.size() * sizeof(double), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(vmx.data(), h->vmax.p, vmx.size() * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(gm.data(), h->gmm.p, 2 * sizeof(double), hipMemcpyDeviceToHost));
+      if (h->hasVertexD)
+        HIPCHK(hipMemcpy(vD.data(), h->vertexD.p, vD.size() * sizeof(double), hipMemcpyDeviceToHost));
+      volatile double sNeg = std::fabs(gm[1]) + 1.0, sPos = std::fabs(gm[0]) + 1.0;
       for (size_t i = 0; i < v0t.size(); ++i) {
-        if (minV0T) minV0T[i] = vmn[v0t[i]];
-        if (maxV0T) maxV0T[i] = vmx[v0t[i]];
+        volatile double lo = vmn[v0t[i]] - sNeg;
+        lo = lo + sNeg;
+        volatile double hi = vmx[v0t[i]] + sPos;
+        hi = hi - sPos;
+        if (minV0T) minV0T[i] = std::fmin(lo, vD[v0t[i]]);
+        if (maxV0T) maxV0T[i] = std::fmax(hi, vD[v0t[i]]);
       }
     }
   });

@@ -81,10 +81,12 @@ def test_one_step(gpu, p, ordRK, lim):
 @pytest.mark.parametrize('p,ordRK,lim,graph', [(1, 2, False, True), (2, 3, True, True), (2, 3, True, False),
                                                (1, 2, True, True), (3, 3, True, True)])
 def test_main_loop(gpu, p, ordRK, lim, graph):
-    """20 steps through main('advection', ...) against the oracle's main."""
+    """20 steps through main('advection', ...) against the oracle's main, at a
+    stable time step (CFL 0.4 / (2p+1) with max|u| <= 0.8)."""
     import festung_amd
     tc = TC.rotating_humps()
-    kw = dict(p=p, hmax=1 / 12, ordRK=ordRK, isSlopeLim=lim, numSteps=20, tEnd=1.0)
+    h = 1 / 12
+    kw = dict(p=p, hmax=h, ordRK=ordRK, isSlopeLim=lim, numSteps=20, tEnd=20 * 0.4 * h / (0.8 * (2 * p + 1)))
     d = festung_amd.main('advection', dict(testcase='custom', useGraph=graph, **tc, **kw))
     ocfg = OA.Struct(testcase='custom', **kw)
     for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):

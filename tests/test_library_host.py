@@ -1,0 +1,196 @@
+"""Host-side checks of the C-ABI library (no GPU): it loads, exports every
+symbol the header declares, builds grids bit-identical to the oracle's
+(connectivity, numbering and geometry), computes the same reference tables,
+and reports errors as status codes."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy.spatial import Delaunay
+
+from festung_amd import _lib
+from festung_amd.core import Grid, quadRule1D, quadRule2D, ref_table
+from oracle import basis as OB
+from oracle import grid as OG
+from oracle import quadrature as OQ
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INT_FIELDS = ('V0T', 'V0E', 'T0E', 'E0T', 'E0E', 'idE', 'idE0T')
+MARK_FIELDS = ('markE0Tint', 'markE0TbdrD', 'markV0TbdrD')
+F64_FIELDS = ('coordV', 'areaE', 'nuE', 'baryE', 'areaT', 'baryT', 'B', 'areaE0T', 'sigE0T', 'nuE0T',
+              'coordV0T')
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, 'include', 'festung_hip.h')).read()
+    txt = re.sub(r'/\*.*?\*/', '', txt, flags=re.S)
+    return sorted(set(re.findall(r'\b(festung_\w+)\s*\(', txt)))
+
+
+def test_library_exports_header_symbols():
+    L = _lib.lib()
+    syms = header_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(L, s), s
+        assert s in _lib.SIGNATURES, s
+    assert L.festung_version().startswith(b'festung-hip')
+
+
+@pytest.mark.parametrize('h', [1 / 3, 1 / 8, 1 / 10, 1 / 12, 2 ** -6])
+def test_domain_square_bit_exact(h):
+    g = Grid.domainSquare(h)
+    o = OG.domainSquare(h)
+    OG.markBoundaries(o)
+    assert (g.numT, g.numV, g.numE) == (o.numT, o.numV, o.numE)
+    for f in INT_FIELDS + MARK_FIELDS:
+        assert np.array_equal(getattr(g, f), o[f]), f
+    for f in F64_FIELDS:
+        assert np.array_equal(getattr(g, f), o[f]), f
+    # neighbour lists are the nonzeros of markE0TE0T{nn, np}
+    nbr, nbe = g.nbr, g.nbrEdge
+    for (nn, np_), M in o.markE0TE0T.items():
+        r, c = M.nonzero()
+        assert np.array_equal(nbr[r, nn], c) and np.all(nbe[r, nn] == np_)
+    # vertex patches == the rows of markV0TT0V
+    OG.computeDerivedGridData(o)
+    ptr, idx = g.v2tPtr, g.v2tIdx
+    for n in range(3):
+        M = o.markV0TT0V[n].tocsr()
+        for k in range(0, g.numT, max(1, g.numT // 50)):
+            v = g.V0T[k, n]
+            assert np.array_equal(np.sort(idx[ptr[v]:ptr[v + 1]]), np.sort(M.indices[M.indptr[k]:M.indptr[k + 1]]))
+
+
+def _delaunay_mesh(n, seed):
+    rng = np.random.default_rng(seed)
+    pts = np.vstack([rng.uniform(0, 1, (n, 2)), [[0, 0], [1, 0], [0, 1], [1, 1]]])
+    tri = Delaunay(pts).simplices.astype(np.int64)
+    a, b, c = pts[tri[:, 0]], pts[tri[:, 1]], pts[tri[:, 2]]
+    cw = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]) < 0
+    tri[cw] = tri[cw][:, [0, 2, 1]]
+    return pts, tri
+
+
+@pytest.mark.parametrize('seed', [0, 1, 2])
+def test_unstructured_grid_bit_exact(seed):
+    pts, tri = _delaunay_mesh(300, seed)
+    g = Grid.generateGridData(pts, tri)
+    o = OG.generateGridData(pts, tri)
+    for f in ('V0T', 'V0E', 'T0E', 'E0T', 'E0E'):
+        assert np.array_equal(getattr(g, f), o[f]), f
+    for f in F64_FIELDS:
+        assert np.array_equal(getattr(g, f), o[f]), f
+    # boundary ids from the caller, as in the reference (generateGridData leaves idE to the caller)
+    idE = (o.E0E[:, 1] < 0).astype(np.int32)
+    g.set_boundary(idE)
+    o.idE = idE
+    o.idE0T = idE[o.E0T]
+    OG.markBoundaries(o)
+    for f in MARK_FIELDS:
+        assert np.array_equal(getattr(g, f), o[f]), f
+
+
+def test_quadrature_and_tables_match_oracle():
+    for q in range(0, 18):
+        a = quadRule1D(q); b = OQ.quadRule1D(q)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    for q in range(0, 13):
+        a = quadRule2D(q); b = OQ.quadRule2D(q)
+        tol = 0 if q <= 6 else 2e-15       # collapsed rules: eigen-solver last bits
+        for x, y in zip(a, b):
+            assert np.max(np.abs(x - y)) <= tol, q
+    for p in range(5):
+        N = (p + 1) * (p + 2) // 2
+        bq = OB.computeBasesOnQuad(N)
+        refs = {'hatM': OB.integrateRefElemPhiPhi(N, bq), 'hatG': OB.integrateRefElemDphiPhiPhi(N, bq),
+                'hatRdiag': OB.integrateRefEdgePhiIntPhiIntPerQuad(N, bq),
+                'hatRoff': OB.integrateRefEdgePhiIntPhiExtPerQuad(N, bq)}
+        for name, ref in refs.items():
+            got = ref_table(p, name)
+            assert np.allclose(got, ref.flatten(order='F'), rtol=0, atol=1e-12 * max(1, np.abs(ref).max())), (p, name)
+        for ord_ in ([2 * p, 2 * p + 1] if p > 0 else [1]):
+            assert np.allclose(ref_table(p, 'phi2D', ord_), bq['phi2D'][ord_].flatten(order='F'), atol=1e-13)
+            assert np.allclose(ref_table(p, 'gradPhi2D', ord_), bq['gradPhi2D'][ord_].flatten(order='F'), atol=1e-12)
+            assert np.array_equal(ref_table(p, 'phi1D', ord_), bq['phi1D'][ord_].flatten(order='F'))
+            assert np.array_equal(ref_table(p, 'thetaPhi1D', ord_), bq['thetaPhi1D'][ord_].flatten(order='F'))
+
+
+@pytest.mark.parametrize('p', [1, 2, 3, 4])
+def test_taylor_tables(p):
+    """F maps DG coefficients to [mean; reference derivatives at the centroid]:
+    row 0 is sqrt(2) e_1, rows of degree 1 are gradPhi at (1/3, 1/3), rows of
+    degree d >= 2 are finite differences of gradPhi; Finv is its inverse."""
+    N = (p + 1) * (p + 2) // 2
+    F = ref_table(p, 'taylorF').reshape(N, N, order='F')
+    Fi = ref_table(p, 'taylorFinv').reshape(N, N, order='F')
+    assert np.allclose(F @ Fi, np.eye(N), atol=1e-10)
+    assert np.allclose(F[0], np.eye(N)[0] * np.sqrt(2))
+    c = 1 / 3
+    for j in range(N):
+        assert abs(F[1, j] - OB.gradPhi(j + 1, 1, c, c)) < 1e-12
+        assert abs(F[2, j] - OB.gradPhi(j + 1, 2, c, c)) < 1e-12
+        if p >= 2:
+            h = 1e-5
+            d11 = (OB.gradPhi(j + 1, 1, c + h, c) - OB.gradPhi(j + 1, 1, c - h, c)) / (2 * h)
+            d12 = (OB.gradPhi(j + 1, 1, c, c + h) - OB.gradPhi(j + 1, 1, c, c - h)) / (2 * h)
+            d22 = (OB.gradPhi(j + 1, 2, c, c + h) - OB.gradPhi(j + 1, 2, c, c - h)) / (2 * h)
+            assert np.allclose(F[3:6, j], [d11, d12, d22], atol=1e-5 * max(1, np.abs(F[3:6, j]).max()))
+
+
+def test_error_paths():
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.festung_domain_square(-1.0, 0.0, 1.0, C.byref(h)) == _lib.ERR_ARG
+    assert b'h > 0' in L.festung_last_error()
+    assert L.festung_domain_square(0.5, 0.0, 1.0, None) == _lib.ERR_ARG
+    # clockwise element is rejected
+    coord = np.asfortranarray([[0, 0], [1, 0], [0, 1]], dtype=np.float64)
+    V0T = np.asfortranarray([[0, 2, 1]], dtype=np.int32)
+    st = L.festung_grid_create(_lib.ptr(coord), 3, _lib.ptr(V0T), 1, C.byref(h))
+    assert st == _lib.ERR_ARG and b'counter-clockwise' in L.festung_last_error()
+    # out-of-range vertex id
+    V0T = np.asfortranarray([[0, 1, 7]], dtype=np.int32)
+    assert L.festung_grid_create(_lib.ptr(coord), 3, _lib.ptr(V0T), 1, C.byref(h)) == _lib.ERR_ARG
+    # unsupported degree, unknown table
+    cnt = C.c_int64()
+    assert L.festung_ref_size(5, b'hatM', 0, C.byref(cnt)) == _lib.ERR_UNSUPPORTED
+    assert L.festung_ref_size(1, b'nope', 0, C.byref(cnt)) == _lib.ERR_ARG
+    assert L.festung_ref_size(1, b'quad2D', 99, C.byref(cnt)) == _lib.ERR_ARG
+    # unknown grid field
+    g = Grid.domainSquare(0.5)
+    with pytest.raises(_lib.FestungError):
+        g.field('nope') if False else _lib.check(L.festung_grid_field_size(g._h, b'nope', C.byref(cnt), None))
+    # a successful call clears the message
+    assert L.festung_grid_sizes(g._h, None, None, None) == _lib.OK
+    assert L.festung_last_error() == b''
+
+
+def test_problem_needs_device_and_fails_loudly():
+    """Without a visible GPU the device path raises; there is no CPU fallback."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip('GPU visible')
+    from festung_amd.core import Problem
+    g = Grid.domainSquare(0.5)
+    with pytest.raises(_lib.FestungError) as e:
+        Problem(g, 1, 2)
+    assert e.value.code == _lib.ERR_HIP
+
+
+def test_host_mirror_defaults_follow_reference():
+    """advection/configureProblem.m:64-96 defaults (triangles)."""
+    from festung_amd.advection import steps as S
+    pd = S.configureProblem(S.Struct())
+    assert pd.testcase == 'solid_body' and pd.hmax == 2 ** -6 and pd.p == 0
+    assert pd.qOrd == 1 and pd.ordRK == 1 and pd.numSteps == 100
+    assert abs(pd.tEnd - (100 / (3142 * 1.5)) * 2 * np.pi) < 1e-15
+    assert pd.isSlopeLim is False and pd.typeSlopeLim == 'linear'
+    pd = S.configureProblem(S.Struct(p=2))
+    assert pd.qOrd == 5 and pd.ordRK == 3
+    with pytest.raises(AssertionError):
+        S.configureProblem(S.Struct(p=0, isSlopeLim=True))
+    with pytest.raises(NotImplementedError):
+        S.configureProblem(S.Struct(isQuadri=True))

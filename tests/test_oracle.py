@@ -1,0 +1,493 @@
+"""Pins of the oracle (CPU only): each check is chosen so that a plausible
+mistake in the restatement (dropped term, wrong sign, wrong index, transposed
+operand) fails it, independently of the oracle's own formulas."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.special import roots_jacobi, roots_legendre
+
+from oracle import advection as OA
+from oracle import assembly as A
+from oracle import basis as B
+from oracle import grid as G
+from oracle import limiter as L
+from oracle import quadrature as Q
+
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+def exact_tri(a, b):
+    """Integral of x^a y^b over the reference triangle: a! b! / (a+b+2)!."""
+    return math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+
+
+# ------------------------------------------------------------- quadrature --
+@pytest.mark.parametrize('q', range(0, 18))
+def test_quad1d_exact_and_legendre(q):
+    X, W = Q.quadRule1D(q)
+    assert abs(W.sum() - 1) < 1e-14
+    deg = max(q, 1)
+    for d in range(deg + 1):
+        assert abs(np.sum(W * X ** d) - 1 / (d + 1)) < 1e-14
+    x, w = np.polynomial.legendre.leggauss(len(X))
+    o = np.argsort(X)
+    assert np.allclose(np.sort(X), (x + 1) / 2, atol=1e-15)
+    assert np.allclose(W[o], w / 2, atol=1e-15)
+
+
+@pytest.mark.parametrize('q', range(0, 13))
+def test_quad2d_exactness(q):
+    Q1, Q2, W = Q.quadRule2D(q)
+    assert abs(W.sum() - 0.5) < 1e-14
+    assert np.all((Q1 >= 0) & (Q2 >= 0) & (Q1 + Q2 <= 1 + 1e-15))
+    for a in range(max(q, 1) + 1):
+        for b in range(max(q, 1) + 1 - a):
+            assert abs(np.sum(W * Q1 ** a * Q2 ** b) - exact_tri(a, b)) < 5e-14, (q, a, b)
+    # orders 4 and 6 are 15-digit tables: exact up to their printed precision
+    if q in (4, 6):
+        assert abs(np.sum(W * Q1 ** (q + 1))) > 0
+
+
+@pytest.mark.parametrize('n', [7, 8, 9, 10])
+def test_triquad_duffy(n):
+    """core/triquad.m against the textbook collapsed rule built from scipy's
+    Gauss-Jacobi(0, 1) and Gauss-Legendre nodes."""
+    X, Y, Wx, Wy = Q.triquad(n)
+    xj, wj = roots_jacobi(n, 0, 1)         # weight (1+x)
+    yl, wl = roots_legendre(n)
+    # Jacobi nodes mapped to [0, 1] carry the collapse weight; Legendre to [0, 1]
+    x = (xj + 1) / 2
+    wx = wj / 4
+    t = (1 + yl) / 2
+    wy = wl / 2
+    # collapse (x, t) -> (x (1 - t), x t)
+    S = X + Y
+    assert np.allclose(S, S[:, :1], atol=1e-15)
+    assert np.allclose(np.sort(S[:, 0]), np.sort(x), atol=1e-14)
+    assert np.allclose(np.sort(Wx), np.sort(wx), atol=1e-14)
+    assert np.allclose(np.sort(Wy), np.sort(wy), atol=1e-14)
+    assert np.allclose(np.sort(Y[0, :] / S[0, 0]), np.sort(t), atol=1e-13)
+
+
+# ------------------------------------------------------------------ basis --
+@pytest.mark.parametrize('p', range(5))
+def test_basis_orthonormal(p):
+    N = (p + 1) * (p + 2) // 2
+    Q1, Q2, W = Q.quadRule2D(2 * p + 2)
+    Phi = np.stack([B.phi(i, Q1, Q2) for i in range(1, N + 1)], axis=1)
+    M = Phi.T @ (W[:, None] * Phi)
+    assert np.allclose(M, np.eye(N), atol=1e-13)
+
+
+@pytest.mark.parametrize('i', range(1, 16))
+def test_gradphi_finite_differences(i):
+    rng = np.random.default_rng(i)
+    x1 = rng.uniform(0, 0.5, 20); x2 = rng.uniform(0, 0.5, 20)
+    h = 1e-6
+    for m, (d1, d2) in ((1, (h, 0)), (2, (0, h))):
+        fd = (B.phi(i, x1 + d1, x2 + d2) - B.phi(i, x1 - d1, x2 - d2)) / (2 * h)
+        assert np.allclose(B.gradPhi(i, m, x1, x2), fd, atol=2e-6, rtol=1e-7)
+
+
+def test_theta_matches_neighbour_geometry():
+    """theta(nn, np) maps the point of edge nn to the *same physical point*
+    seen from the neighbour across that edge (checked on a real mesh)."""
+    g = G.domainSquare(1 / 4)
+    s = np.array([0.2, 0.7])
+    for (nn, np_), M in g.markE0TE0T.items():
+        r, c = M.nonzero()
+        for k, kp in zip(r, c):
+            a1, a2 = B.gammaMap(nn + 1, s)
+            b1, b2 = B.theta(nn + 1, np_ + 1, a1, a2)
+            X = G.mapRef2Phy(g, 0, a1, a2)[k], G.mapRef2Phy(g, 1, a1, a2)[k]
+            Y = G.mapRef2Phy(g, 0, b1, b2)[kp], G.mapRef2Phy(g, 1, b1, b2)[kp]
+            assert np.allclose(X, Y, atol=1e-15)
+
+
+@pytest.mark.parametrize('p', range(5))
+def test_reference_blocks(p):
+    N = (p + 1) * (p + 2) // 2
+    bq = B.computeBasesOnQuad(N)
+    M = B.integrateRefElemPhiPhi(N, bq)
+    assert np.allclose(M, np.eye(N), atol=1e-13)
+    G4 = B.integrateRefElemDphiPhiPhi(N, bq)
+    # independent: integrate grad phi_i phi_j phi_l with a rule exact for degree 3p-1
+    q = max(3 * p, 2)
+    Q1, Q2, W = Q.quadRule2D(q)
+    ex = np.zeros_like(G4)
+    for i in range(N):
+        for j in range(N):
+            for l in range(N):
+                for m in range(2):
+                    ex[i, j, l, m] = np.sum(W * B.gradPhi(i + 1, m + 1, Q1, Q2) * B.phi(j + 1, Q1, Q2)
+                                            * B.phi(l + 1, Q1, Q2))
+    if p <= 1 or p == 4:
+        # the reference's order-2p rule is exact for degree 3p-1 when p <= 1;
+        # for p = 4 the order-8 collapsed rule is exact up to degree 15
+        assert np.allclose(G4, ex, atol=1e-11 * max(1, np.abs(ex).max()))
+    else:
+        # deliberately inexact in the reference (core/integrateRefElemDphiPhiPhi.m:68-71)
+        assert np.abs(G4 - ex).max() > 1e-8
+    # symmetry in (j, l)
+    assert np.allclose(G4, G4.transpose(0, 2, 1, 3), atol=1e-13)
+    Rd = B.integrateRefEdgePhiIntPhiIntPerQuad(N, bq)
+    Ro = B.integrateRefEdgePhiIntPhiExtPerQuad(N, bq)
+    # sum over r of R-diag = edge mass matrix on edge n (length-normalised):
+    # exact edge integral of phi_i phi_j along gamma_n with a high-order rule
+    s, w = Q.quadRule1D(2 * p + 3)
+    for n in range(3):
+        a1, a2 = B.gammaMap(n + 1, s)
+        P = np.stack([B.phi(i, a1, a2) for i in range(1, N + 1)], axis=1)
+        assert np.allclose(Rd[:, :, n, :].sum(axis=2), P.T @ (w[:, None] * P), atol=1e-12)
+        for np_ in range(3):
+            b1, b2 = B.theta(n + 1, np_ + 1, a1, a2)
+            PT = np.stack([B.phi(i, b1, b2) for i in range(1, N + 1)], axis=1)
+            assert np.allclose(Ro[:, :, n, np_, :].sum(axis=2), P.T @ (w[:, None] * PT), atol=1e-12)
+
+
+# ------------------------------------------------------------------- mesh --
+@pytest.mark.parametrize('h', [1 / 3, 1 / 4, 1 / 7, 1 / 10, 2 ** -5])
+def test_mesh_closed_forms(h):
+    g = G.domainSquare(h)
+    dim = g.dim
+    assert g.numT == 2 * dim * dim
+    assert g.numV == (dim + 1) ** 2
+    assert g.numE == 3 * dim * dim + 2 * dim
+    assert g.numV - g.numE + g.numT == 1
+    assert abs(g.areaT.sum() - 1) < 1e-13
+    assert np.all(g.areaT > 0)
+    # outward unit normals
+    nrm = np.linalg.norm(g.nuE0T, axis=2)
+    assert np.allclose(nrm, 1, atol=1e-15)
+    out = np.einsum('knm,knm->kn', g.nuE0T, g.baryE0T - g.baryT[:, None, :])
+    assert np.all(out > 0)
+    # E0T / T0E / E0E consistency
+    for n in range(3):
+        e = g.E0T[:, n]
+        k = np.arange(g.numT)
+        first = g.T0E[e, 0] == k
+        second = g.T0E[e, 1] == k
+        assert np.all(first | second)
+        assert np.all(np.where(first, g.E0E[e, 0], g.E0E[e, 1]) == n)
+    # markE0TE0T: each interior edge seen from both sides
+    tot = sum(M.nnz for M in g.markE0TE0T.values())
+    nint = int(np.sum(g.E0E[:, 1] >= 0))
+    assert tot == 2 * nint
+    for (nn, np_), M in g.markE0TE0T.items():
+        assert (M - g.markE0TE0T[(np_, nn)].T).nnz == 0
+    # boundary ids: 4 * dim boundary edges, dim on each side
+    assert np.array_equal(np.bincount(g.idE, minlength=5)[1:], [dim] * 4)
+
+
+def test_mesh_bruteforce_neighbours():
+    g = G.domainSquare(1 / 5)
+    edges = {}
+    for k in range(g.numT):
+        for n in range(3):
+            a, b = g.V0T[k, (n + 1) % 3], g.V0T[k, (n + 2) % 3]
+            edges.setdefault(frozenset((a, b)), []).append((k, n))
+    for (nn, np_), M in g.markE0TE0T.items():
+        r, c = M.nonzero()
+        for k, kp in zip(r, c):
+            e = frozenset((g.V0T[k, (nn + 1) % 3], g.V0T[k, (nn + 2) % 3]))
+            assert sorted(edges[e]) == sorted([(k, nn), (kp, np_)])
+    assert len(edges) == g.numE
+
+
+def test_mesh_dense_patches_match_sparse():
+    g1 = G.domainSquare(1 / 6)
+    G.markBoundaries(g1)
+    g2 = G.generateGridData(g1.coordV, g1.V0T, dense_patches=True)
+    g2.idE0T = g1.idE0T
+    G.markBoundaries(g2)
+    G.computeDerivedGridData(g1)
+    G.computeDerivedGridData(g2)
+    for n in range(3):
+        assert (g1.markV0TT0V[n] != g2.markV0TT0V[n]).nnz == 0
+
+
+def test_mesh_golden_domainsquare_picture():
+    gold = json.load(open(os.path.join(GOLDEN, 'domainsquare_dim3.json')))
+    g = G.domainSquare(1 / 3)
+    idx = lambda xy: int(np.flatnonzero(np.all(np.isclose(g.coordV, xy), axis=1))[0]) + 1
+    for key, v in gold['corner_vertices_1based'].items():
+        assert idx(eval(key)) == v
+    for j, v in enumerate(gold['top_row_vertices_1based']):
+        assert idx((j / 3, 1.0)) == v
+    for j, v in enumerate(gold['bottom_row_vertices_1based']):
+        assert idx((j / 3, 0.0)) == v
+    for key, val in gold['square_triangles_1based'].items():
+        if key == 'comment':
+            continue
+        lo, up = val
+        i, j = map(int, key.split(','))
+        cx, cy = (j - 0.5) / 3, (i - 0.5) / 3
+        # lower-left triangle has its centroid below the square's diagonal x + y = const
+        ks = [k for k in range(g.numT)
+              if abs(g.baryT[k, 0] - cx) < 1 / 6 and abs(g.baryT[k, 1] - cy) < 1 / 6]
+        low = [k for k in ks if (g.baryT[k, 0] - cx) + (g.baryT[k, 1] - cy) < 0]
+        upp = [k for k in ks if (g.baryT[k, 0] - cx) + (g.baryT[k, 1] - cy) > 0]
+        assert [low[0] + 1, upp[0] + 1] == [lo, up], key
+    for e, (a, b) in gold['edges_1based'].items():
+        assert sorted(g.V0E[int(e) - 1] + 1) == [a, b]
+
+
+def test_mesh_golden_generategriddata_example():
+    gold = json.load(open(os.path.join(GOLDEN, 'generategriddata_example.json')))
+    g = G.generateGridData(np.array(gold['coordV']), np.array(gold['V0T_1based']) - 1)
+    assert sorted(g.V0E[4] + 1) == gold['edge5_vertices_1based']
+    idE = (np.abs(g.nuE[:, 1]) > 0) * ((g.nuE[:, 0] > 0) + (g.nuE[:, 1] > 0) * 2 + 1)
+    for key, v in gold['idE_by_vertices_1based'].items():
+        a, b = map(int, key.split(','))
+        e = [i for i in range(g.numE) if sorted(g.V0E[i] + 1) == [a, b]][0]
+        assert idE[e] == v, key
+
+
+# ------------------------------------------------------------- operators --
+def _setup(p, h=1 / 4, u=None, lim=False):
+    pd = OA.Struct(testcase='custom', p=p, hmax=h, numSteps=10, isSlopeLim=lim)
+    pd.u1Cont = u[0] if u else (lambda t, x1, x2: 0.5 - x2)
+    pd.u2Cont = u[1] if u else (lambda t, x1, x2: x1 - 0.5)
+    pd.c0Cont = lambda x1, x2: np.sin(3 * x1) * np.cos(2 * x2)
+    z = lambda t, x1, x2: np.zeros_like(x1)
+    pd.fCont = z; pd.cDCont = z; pd.gNCont = z
+    pd = OA.configureProblem(pd)
+    pd = OA.preprocessProblem(pd)
+    return pd
+
+
+def _brute_rhs(pd, c, ts=0.0):
+    """Independent per-element / per-edge evaluation of
+    sum_T int c u . grad phi_i - sum_E int u.nu c_upwind phi_i, using physical
+    points and the inverse map to find the neighbour's reference coordinates
+    (no theta tables, no markE0TE0T)."""
+    g = pd.g; N = pd.N; p = pd.p
+    K = g.numT
+    r = np.zeros((K, N))
+    Q1, Q2, W = Q.quadRule2D(max(2 * p, 1))
+    for k in range(K):
+        Bk = g.B[k]
+        a1 = g.coordV0T[k, 0]
+        X = Bk @ np.vstack([Q1, Q2]) + a1[:, None]
+        u1 = A.projectFuncCont2DataDisc  # placeholder to keep names local
+        # velocity as the reference uses it: DG projection of u (order 2p)
+        u1d = pd.u1Disc[k]; u2d = pd.u2Disc[k]
+        Phi = np.stack([B.phi(i, Q1, Q2) for i in range(1, N + 1)], axis=1)
+        uq1 = Phi @ u1d; uq2 = Phi @ u2d
+        cq = Phi @ c[k]
+        Binv = np.linalg.inv(Bk)
+        for i in range(N):
+            gr = np.vstack([B.gradPhi(i + 1, 1, Q1, Q2), B.gradPhi(i + 1, 2, Q1, Q2)])
+            gx = Binv.T @ gr
+            r[k, i] += np.sum(W * cq * (uq1 * gx[0] + uq2 * gx[1])) * 2 * g.areaT[k]
+    s, w = Q.quadRule1D(2 * p + 1)
+    for k in range(K):
+        for n in range(3):
+            a1, a2 = B.gammaMap(n + 1, s)
+            Xk = g.B[k] @ np.vstack([a1, a2]) + g.coordV0T[k, 0][:, None]
+            nu = g.nuE0T[k, n]
+            un = nu[0] * pd.u1Cont(ts, Xk[0], Xk[1]) + nu[1] * pd.u2Cont(ts, Xk[0], Xk[1])
+            ck = np.stack([B.phi(i, a1, a2) for i in range(1, N + 1)], axis=1) @ c[k]
+            e = g.E0T[k, n]
+            kp = g.T0E[e, 1] if g.T0E[e, 0] == k else g.T0E[e, 0]
+            if kp >= 0:
+                xh = np.linalg.solve(g.B[kp], Xk - g.coordV0T[kp, 0][:, None])
+                cn = np.stack([B.phi(i, xh[0], xh[1]) for i in range(1, N + 1)], axis=1) @ c[kp]
+            else:
+                cn = np.zeros_like(ck)    # c_D = 0
+            up = np.where(un > 0, ck, cn)
+            for i in range(N):
+                r[k, i] -= g.areaE0T[k, n] * np.sum(w * un * up * B.phi(i + 1, a1, a2))
+    return r
+
+
+@pytest.mark.parametrize('p', [1, 2, 3])
+def test_assembled_operator_vs_bruteforce(p):
+    pd = _setup(p)
+    pd.t = np.array([0.0])
+    pd = OA.preprocessSubStep(pd, 1, 1)
+    rng = np.random.default_rng(p)
+    c = rng.standard_normal((pd.K, pd.N))
+    sysA = -pd.globG[0] - pd.globG[1] + pd.globR
+    got = -(sysA @ c.reshape(-1)).reshape(pd.K, pd.N)
+    ref = _brute_rhs(pd, c)
+    assert np.allclose(got, ref, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize('p', [1, 2, 3])
+def test_free_stream_preservation(p):
+    """c = 1, c_D = 1, divergence-free linear u: dc/dt = 0."""
+    pd = _setup(p)
+    pd.cDCont = lambda t, x1, x2: np.ones_like(x1)
+    pd.t = np.array([0.0])
+    pd = OA.preprocessSubStep(pd, 1, 1)
+    one = A.projectFuncCont2DataDisc(pd.g, lambda x1, x2: np.ones_like(x1), 2 * p, pd.hatM, pd.basesOnQuad)
+    dot = OA.computeDot(pd, one.reshape(-1))
+    assert np.abs(dot).max() < 1e-12
+
+
+def test_mass_balance():
+    """Row i = 1 of the operator telescopes: the total mass changes only by
+    the boundary fluxes."""
+    pd = _setup(2, u=(lambda t, x1, x2: 0.3 + 0 * x1, lambda t, x1, x2: 0.2 + 0 * x1))
+    pd.t = np.array([0.0])
+    pd = OA.preprocessSubStep(pd, 1, 1)
+    rng = np.random.default_rng(3)
+    c = rng.standard_normal((pd.K, pd.N))
+    dot = OA.computeDot(pd, c.reshape(-1)).reshape(pd.K, pd.N)
+    mass_rate = np.sum(pd.g.detJ0T * dot[:, 0]) * np.sqrt(2) / 2
+    # boundary outflow with the element's own trace (inflow c_D = 0)
+    s, w = Q.quadRule1D(5)
+    out = 0.0
+    for k in range(pd.K):
+        for n in range(3):
+            if pd.g.idE0T[k, n] == 0:
+                continue
+            a1, a2 = B.gammaMap(n + 1, s)
+            un = pd.vNormalOnQuadEdge[k, n]
+            ck = np.stack([B.phi(i, a1, a2) for i in range(1, pd.N + 1)], axis=1) @ c[k]
+            out += pd.g.areaE0T[k, n] * np.sum(w * np.maximum(un, 0) * ck)
+    assert abs(mass_rate + out) < 1e-12 * max(1, abs(out))
+
+
+def test_projection_and_l2_error_exact_for_polynomials():
+    pd = _setup(2, h=1 / 5)
+    f = lambda x1, x2: 1 + 2 * x1 - 3 * x2 + x1 * x2 - 0.5 * x2 ** 2
+    c = A.projectFuncCont2DataDisc(pd.g, f, 5, pd.hatM, pd.basesOnQuad)
+    assert A.computeL2Error(pd.g, c, f, 5, pd.basesOnQuad) < 1e-13
+    g2 = lambda x1, x2: np.exp(x1) * np.sin(2 * x2)
+    c2 = A.projectFuncCont2DataDisc(pd.g, g2, 5, pd.hatM, pd.basesOnQuad)
+    pd.basesOnQuad.update(B.computeBasesOnQuad(6, [10]))
+    e = A.computeL2Error(pd.g, c2, g2, 10, pd.basesOnQuad)
+    # independent: the same rule element by element from B and phi directly
+    Q1, Q2, W = Q.quadRule2D(10)
+    tot = 0.0
+    for k in range(pd.K):
+        X = pd.g.B[k] @ np.vstack([Q1, Q2]) + pd.g.coordV0T[k, 0][:, None]
+        ch = np.stack([B.phi(i, Q1, Q2) for i in range(1, 7)], axis=1) @ c2[k]
+        tot += 2 * pd.g.areaT[k] * np.sum(W * (ch - g2(X[0], X[1])) ** 2)
+    assert abs(e - np.sqrt(tot)) < 1e-12 * e
+
+
+# -------------------------------------------------------------- RK tables --
+def test_rk_tables_integrate_time_quadrature():
+    """u = 0, c' = f(t): SSP-RK2 is the trapezoidal rule, SSP-RK3 Simpson's
+    rule, classic RK4 Simpson's rule; exact for f = t^2 (RK3, RK4) / t (RK2)."""
+    for ordRK, f, F in ((2, lambda t: t, lambda t: t ** 2 / 2), (3, lambda t: t ** 2, lambda t: t ** 3 / 3),
+                        (4, lambda t: t ** 3, lambda t: t ** 4 / 4), (1, lambda t: 1.0 + 0 * t, lambda t: t)):
+        pd = OA.Struct(testcase='custom', p=1, hmax=1 / 2, numSteps=1, tEnd=0.7, ordRK=ordRK)
+        z = lambda t, x1, x2: np.zeros_like(x1)
+        pd.u1Cont = z; pd.u2Cont = z; pd.cDCont = z; pd.gNCont = z
+        pd.fCont = lambda t, x1, x2, f=f: f(t) + 0 * x1
+        pd.c0Cont = lambda x1, x2: 0 * x1
+        d = OA.main('advection', pd)
+        mean = d.cDisc[:, 0] * np.sqrt(2)
+        assert np.allclose(mean, F(0.7), atol=1e-13), ordRK
+
+
+def test_ssp_rk3_stability_polynomial():
+    """Linear problem c' = L c: one SSP-RK3 step equals 1 + z + z^2/2 + z^3/6."""
+    pd = _setup(1, h=1 / 3)
+    pd.t = np.array([0.0]); pd = OA.preprocessSubStep(pd, 1, 1)
+    Lm = sp.linalg.spsolve(pd.globM.tocsc(), -(-pd.globG[0] - pd.globG[1] + pd.globR).tocsc()).toarray()
+    pd.tau = 0.01
+    rng = np.random.default_rng(0)
+    c = rng.standard_normal(pd.K * pd.N)
+    pd.ordRK = 3
+    pd.cDisc = c.reshape(pd.K, pd.N)
+    pd = OA.solveStep(pd, 1)
+    z = pd.tau * Lm
+    ref = c + z @ c + z @ (z @ c) / 2 + z @ (z @ (z @ c)) / 6
+    assert np.allclose(pd.cDiscRK[-1], ref, atol=1e-13)
+
+
+# ---------------------------------------------------------------- limiter --
+def test_taylor_roundtrip_and_derivatives():
+    pd = _setup(3, h=1 / 3, lim=True)
+    rng = np.random.default_rng(5)
+    c = rng.standard_normal((pd.K, pd.N))
+    t = L.projectDataDisc2DataTaylor(c, pd.globM, pd.globMDiscTaylor)
+    back = L.projectDataTaylor2DataDisc(t, pd.globM, pd.globMDiscTaylor)
+    assert np.allclose(back, c, atol=1e-12)
+    # T_1 is the mean, T_2 / T_3 are dX times... the physical gradient at the
+    # centroid divided by dX: check with finite differences of the DG function
+    k = 4
+    g = pd.g
+    dX = 2 / np.sqrt(2 * g.areaT[k])
+    xc = g.baryT[k]
+    Binv = np.linalg.inv(g.B[k])
+
+    def u(x):
+        xh = Binv @ (x - g.coordV0T[k, 0])
+        return sum(c[k, i] * B.phi(i + 1, xh[0], xh[1]) for i in range(pd.N))
+    h = 1e-5
+    ux = (u(xc + [h, 0]) - u(xc - [h, 0])) / (2 * h)
+    uy = (u(xc + [0, h]) - u(xc - [0, h])) / (2 * h)
+    assert np.allclose([t[k, 1], t[k, 2]], [ux / dX, uy / dX], rtol=1e-6, atol=1e-8)
+    assert abs(t[k, 0] - np.sqrt(2) * c[k, 0]) < 1e-13
+
+
+def test_taylor_basis_mean_free():
+    pd = _setup(4, h=1 / 2, lim=True)
+    Q1, Q2, W = Q.quadRule2D(8)
+    for i in range(2, 16):
+        vals = L.phiTaylorRef(pd.g, i, Q1, Q2)
+        assert np.abs(vals @ W).max() < 1e-14
+
+
+def test_limiter_bruteforce_and_invariants():
+    pd = _setup(2, h=1 / 4, lim=True)
+    rng = np.random.default_rng(9)
+    c = rng.standard_normal((pd.K, pd.N))
+    cD = np.zeros((pd.K, 3))
+    lim, (mn, mx) = L.applySlopeLimiterDisc(pd.g, c, pd.g.markV0TbdrD, cD, pd.globM, pd.globMDiscTaylor,
+                                            pd.basesOnQuad)
+    # brute-force patch min / max of centroid means
+    mean = np.sqrt(2) * c[:, 0]
+    g = pd.g
+    for k in range(g.numT):
+        for n in range(3):
+            v = g.V0T[k, n]
+            patch = [kk for kk in range(g.numT) if v in g.V0T[kk]]
+            lo, hi = mean[patch].min(), mean[patch].max()
+            if g.markV0TbdrD[k, n]:
+                lo, hi = min(lo, 0.0), max(hi, 0.0)
+            assert abs(mn[k, n] - lo) < 1e-13 and abs(mx[k, n] - hi) < 1e-13
+    # means unchanged, alpha in [0, 1], limited vertex values inside the bounds
+    assert np.allclose(lim[:, 0], c[:, 0], atol=1e-13)
+    t = L.projectDataDisc2DataTaylor(lim, pd.globM, pd.globMDiscTaylor)
+    vals = A.computeFuncDiscAtPoints(t[:, :3], pd.basesOnQuad['phiTaylorV0T'][:, :, :3])
+    assert np.all(vals <= mx + 1e-8 + 1e-12) and np.all(vals >= mn - 1e-8 - 1e-12)
+
+
+def test_limiter_keeps_linear_data_inside_bounds():
+    """p = 1, a linear function on a uniform mesh: interior elements whose
+    vertex values lie strictly inside the patch bounds are untouched."""
+    pd = _setup(1, h=1 / 6, lim=True)
+    f = lambda x1, x2: 0.3 * x1 - 0.2 * x2
+    c = A.projectFuncCont2DataDisc(pd.g, f, 3, pd.hatM, pd.basesOnQuad)
+    cD = A.computeFuncContV0T(pd.g, f)
+    t = L.projectDataDisc2DataTaylor(c, pd.globM, pd.globMDiscTaylor)
+    vals = A.computeFuncDiscAtPoints(t[:, :3], pd.basesOnQuad['phiTaylorV0T'][:, :, :3])
+    alpha, (mn, mx) = L.computeVertexBasedLimiter(pd.g, t[:, 0], vals, pd.g.markV0TbdrD, cD)
+    inside = np.all((vals > mn + 1e-8) & (vals < mx - 1e-8), axis=1)
+    assert inside.sum() > pd.K // 3
+    assert np.all(alpha[inside] == 1)
+
+
+# -------------------------------------------------------------------- EOC --
+@pytest.mark.slow
+def test_convergence_order_transient():
+    """'transient' testcase: EOC ~ p+1 on triangles (advection/doConvergenceTest.m)."""
+    for p, lo in ((1, 1.8),):
+        errs = []
+        for lev in (1, 2, 3):
+            d = OA.main('advection', OA.Struct(testcase='transient', p=p, hmax=2 ** -lev / 3,
+                                               numSteps=2 ** p * 50 * 2 ** lev // 8, tEnd=0.25))
+            errs.append(d.error)
+        eoc = np.log(np.array(errs[:-1]) / np.array(errs[1:])) / np.log(2)
+        assert np.all(eoc > lo), eoc
