@@ -32,7 +32,7 @@ METRIC = 'DoF-stage updates/s at p=2 + % HBM roofline; batched & adjoint vs plai
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(kind, p, K, numV, lim, ordRK):
+def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6):
     """Algorithmic HBM bytes of all launches of one kernel kind in one time
     step (each datum read or written once; neighbour and vertex gathers count
     once per element / vertex)."""
@@ -52,7 +52,7 @@ def kernel_bytes(kind, p, K, numV, lim, ordRK):
                 tot += K * (8 * (N + w * N + 2 * N + 3 * Re + 7 + N) + 16)
         elif kind == 'vertex' and lim:
             nv = 2 if p >= 2 else 1
-            tot += nv * (4 * (numV + 1) + 4 * 3 * K + 8 * K + 16 * numV)
+            tot += nv * (4 * maxDeg * numV + 8 * K + 16 * numV)   # padded lists, centroids, bounds
         elif kind == 'limit_dot' and lim and p >= 2:
             tot += K * (8 * (N + 4 + N + w * N + N + 1) + 12) + 16 * numV
         elif kind == 'limit' and lim:
@@ -89,6 +89,7 @@ def main():
     ap.add_argument('--kernel-steps', type=int, default=5, help='eager steps timed per kernel')
     ap.add_argument('--cpu-hmax', type=float, default=2.0 ** -8, help='mesh of the CPU sample')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--copy-mb', type=int, default=1024, help='size of the device copy probe (MiB per buffer)')
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU only: the path runs on a single MI355X; '
@@ -122,6 +123,7 @@ def main():
     pd = S.solveStep(pd, 1)          # first step sets the device velocity
     prob = pd.problem
     K, N, numV = prob.K, prob.N, pd.g.numV
+    maxDeg = int(np.max(np.diff(pd.g.v2tPtr)))
     nstage = {1: 1, 2: 2, 3: 3, 4: 4}[ordRK]
     prob.step(a.warmup, tau)
     prob.sync()
@@ -155,7 +157,7 @@ def main():
     for kind, (ms, cnt) in kt.items():
         if cnt == 0:
             continue
-        per_step_bytes = kernel_bytes(kind, p, K, numV, lim, ordRK)
+        per_step_bytes = kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg)
         launches_per_step = cnt / a.kernel_steps
         avg_ms = ms / cnt
         bytes_per_launch = per_step_bytes / launches_per_step if launches_per_step else 0.0
@@ -167,13 +169,33 @@ def main():
     for v in kernels.values():
         v['share'] = v['avg_us'] * v['launches_per_step'] / tot_ms
     dom = max(kernels, key=lambda k: kernels[k]['share'])
-    step_bytes = sum(kernel_bytes(k, p, K, numV, lim, ordRK) for k in ('stage', 'vertex', 'limit_dot', 'limit'))
+    step_bytes = sum(kernel_bytes(k, p, K, numV, lim, ordRK, maxDeg) for k in ('stage', 'vertex', 'limit_dot', 'limit'))
     secs = wall
     value = K * N * nstage * a.steps / secs
     roof = dict(bound='hbm', kernel=dom, achieved=kernels[dom]['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
                 frac=kernels[dom]['GBs'] / HBM_PEAK_GBS, traffic=None,
                 step_GBs=step_bytes / (secs / a.steps) / 1e9,
                 step_frac=step_bytes / (secs / a.steps) / 1e9 / HBM_PEAK_GBS)
+
+    # plain device copy of a buffer larger than the 256 MiB Infinity Cache:
+    # the attainable read+write bandwidth on this box, and the reference
+    # kernel for scaling the FETCH_SIZE / WRITE_SIZE counters
+    nb = a.copy_mb * (1 << 20)
+    src = torch.ones(nb // 8, dtype=torch.float64, device='cuda')
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    c0 = torch.cuda.Event(enable_timing=True)
+    c1 = torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(10):
+        dst.copy_(src)
+    c1.record()
+    torch.cuda.synchronize()
+    copy_GBs = 2 * nb * 10 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    del src, dst
+    roof['copy_GBs'] = copy_GBs
+    roof['frac_of_copy'] = kernels[dom]['GBs'] / copy_GBs
 
     cpu = None
     if not a.no_cpu_baseline:

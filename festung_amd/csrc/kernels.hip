@@ -31,7 +31,18 @@ namespace festung {
 
 constexpr int BLOCK = 256;
 
-__device__ __forceinline__ int gtid() { return blockIdx.x * BLOCK + threadIdx.x; }
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Remap so that XCD x runs a contiguous range of logical blocks, in
+// order: with elements stored along a space-filling curve, an XCD then works
+// on a compact part of the mesh and neighbour / vertex-patch gathers across
+// block boundaries hit its own L2.
+constexpr int kXCDs = 8;
+__device__ __forceinline__ int logical_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int full = nb - nb % kXCDs;
+  return b < full ? (b % kXCDs) * (full / kXCDs) + b / kXCDs : b;
+}
+__device__ __forceinline__ int gtid() { return logical_block() * BLOCK + threadIdx.x; }
 
 // ---------------------------------------------------------------- Taylor --
 // E_D(A): the degree-D symmetric power of a 2x2 matrix A acting on
@@ -357,11 +368,23 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 
 __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
   if (blockIdx.x == 0) {
-    // global min / max of the centroid values from the producer's partials
+    // global min / max of the centroid values from the producer's partials;
+    // loads are issued in batches of 8 before they are reduced
     double mn = INFINITY, mx = -INFINITY;
-    for (int i = threadIdx.x; i < a.nPartials; i += BLOCK) {
-      mn = fmin(mn, a.partials[2 * i]);
-      mx = fmax(mx, a.partials[2 * i + 1]);
+    for (int i0 = 0; i0 < a.nPartials; i0 += 8 * BLOCK) {
+      double lo[8], hi[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u * BLOCK + threadIdx.x;
+        const bool ok = i < a.nPartials;
+        lo[u] = ok ? a.partials[2 * i] : INFINITY;
+        hi[u] = ok ? a.partials[2 * i + 1] : -INFINITY;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        mn = fmin(mn, lo[u]);
+        mx = fmax(mx, hi[u]);
+      }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -388,10 +411,18 @@ __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
   const int v = gtid();
   if (v >= a.numV) return;
   double m = INFINITY, M = -INFINITY;
-  for (int i = a.v2tPtr[v]; i < a.v2tPtr[v + 1]; ++i) {
-    const double c = a.centroid[a.v2tIdx[i]];
-    m = fmin(m, c);
-    M = fmax(M, c);
+  for (int j0 = 0; j0 < a.maxDeg; j0 += 8) {
+    int id[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) id[u] = j0 + u < a.maxDeg ? a.v2tPad[(j0 + u) * a.numV + v] : -1;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (id[u] >= 0) {
+        const double c = a.centroid[id[u]];
+        m = fmin(m, c);
+        M = fmax(M, c);
+      }
+    }
   }
   a.vmin[v] = m;
   a.vmax[v] = M;
@@ -777,6 +808,20 @@ hipError_t launch_l2(int p, bool ordF, int K, const double* B, const double* c, 
     FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((l2_kernel<PP, Dims<PP>::RE>), gr, dim3(BLOCK), 0, s, K, B, c, cq, tab,
                                              off.wE, off.phiE, partial));
   }
+  return hipGetLastError();
+}
+
+// dst[d + j K] = src[idx[d] + j K], j < cols (reorder a K x cols array)
+__global__ __launch_bounds__(BLOCK) void permute_kernel(int K, int cols, const double* __restrict__ src,
+                                                        const int* __restrict__ idx, double* __restrict__ dst) {
+  const int d = gtid();
+  if (d >= K) return;
+  const int s = idx[d];
+  for (int j = 0; j < cols; ++j) dst[d + size_t(j) * K] = src[s + size_t(j) * K];
+}
+
+hipError_t launch_permute(int K, int cols, const double* src, const int* idx, double* dst, hipStream_t s) {
+  hipLaunchKernelGGL(permute_kernel, grid_for(K), dim3(BLOCK), 0, s, K, cols, src, idx, dst);
   return hipGetLastError();
 }
 

@@ -54,9 +54,7 @@ struct Mesh {
   const double* areaE;   // 3 x K
   const int* nbr;        // 3 x K
   const int* flags;      // K
-  const int* V0T;        // 3 x K
-  const int* v2tPtr;     // numV + 1
-  const int* v2tIdx;     // 3K
+  const int* V0T;        // 3 x K (device vertex ids)
 };
 
 enum StageOut { OUT_RK_DG = 0, OUT_RK_TAYLOR = 1, OUT_DOT_TAYLOR = 2, OUT_DOT_DG = 3 };
@@ -82,8 +80,8 @@ struct StageArgs {
 
 struct VertexArgs {
   int numV;
-  const int* v2tPtr;
-  const int* v2tIdx;
+  int maxDeg;            // columns of the padded vertex -> element lists
+  const int* v2tPad;     // maxDeg x numV, element ids (device order), -1 padded
   const double* centroid;
   const double* partials;  // producer's per-block min / max
   int nPartials;
@@ -135,6 +133,7 @@ hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* c
                          const double* tab, const TabOff& off, double* rhs, hipStream_t s);
 hipError_t launch_l2(int p, bool ordF, int K, const double* B, const double* c, const double* cq, const double* tab,
                      const TabOff& off, double* partial, hipStream_t s);
+hipError_t launch_permute(int K, int cols, const double* src, const int* idx, double* dst, hipStream_t s);
 int block_size();
 
 }  // namespace festung

@@ -75,6 +75,46 @@ static std::vector<StagePlan> make_plan(int ordRK) {
   return v;
 }
 
+// Hilbert-curve index of (x, y) in [0, n)^2, n a power of two (the
+// classic iterative quadrant-rotation construction).
+static uint64_t hilbert_index(uint32_t n, uint32_t x, uint32_t y) {
+  uint64_t d = 0;
+  for (uint32_t s = n / 2; s > 0; s /= 2) {
+    const uint32_t rx = (x & s) ? 1 : 0, ry = (y & s) ? 1 : 0;
+    d += uint64_t(s) * s * ((3 * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = n - 1 - x;
+        y = n - 1 - y;
+      }
+      std::swap(x, y);
+    }
+  }
+  return d;
+}
+
+// Order of points along a Hilbert curve over their bounding box: perm[d] is
+// the id placed at position d.  Ties (same 2^16 grid cell) keep id order.
+static std::vector<int32_t> hilbert_order(const double* x, const double* y, int32_t n) {
+  double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+  for (int32_t i = 0; i < n; ++i) {
+    x0 = std::min(x0, x[i]); x1 = std::max(x1, x[i]);
+    y0 = std::min(y0, y[i]); y1 = std::max(y1, y[i]);
+  }
+  const double span = std::max(std::max(x1 - x0, y1 - y0), 1e-300);
+  const uint32_t G = 1u << 16;
+  std::vector<std::pair<uint64_t, int32_t>> key(n);
+  for (int32_t i = 0; i < n; ++i) {
+    uint32_t qx = uint32_t(std::min(double(G - 1), std::floor((x[i] - x0) / span * G)));
+    uint32_t qy = uint32_t(std::min(double(G - 1), std::floor((y[i] - y0) / span * G)));
+    key[i] = {hilbert_index(G, qx, qy), i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> perm(n);
+  for (int32_t i = 0; i < n; ++i) perm[i] = key[i].second;
+  return perm;
+}
+
 }  // namespace festung
 
 using namespace festung;
@@ -86,8 +126,16 @@ struct festung_problem {
   RefTables tabs;
   TabOff off{};
   std::vector<double> areaT;  // host copy (L2 error)
+  // Device element / vertex order: Hilbert-curve order of the centroids /
+  // vertices, so that neighbours and vertex patches are close in memory.
+  // eperm[d] = reference element at device position d, einv = inverse;
+  // vperm / vinv the same for vertices.  All C-ABI arrays stay in the
+  // reference's order and are permuted here.
+  std::vector<int32_t> eperm, einv, vperm, vinv, V0Tref;
+  DBuf dEperm, dEinv;
   std::vector<StagePlan> plan;
-  DBuf tab, B, area, nbr, flg, V0T, v2tPtr, v2tIdx, nuE0T;
+  DBuf tab, B, area, nbr, flg, V0T, v2tPad, nuE0T;
+  int maxDeg = 0;
   DBuf X, Y, Z, acc, T1, T2, cent, partials, gmm, vmin, vmax, vertexD, u1, u2, vn, rhs, alpha;
   bool hasRhs = false, hasVertexD = false, hasVelocity = false;
   int cur = 0;           // buffer holding the current state (0 = X)
@@ -130,7 +178,7 @@ struct festung_problem {
 
   double* buf(int id) const { return id == 0 ? X.d() : id == 1 ? Y.d() : Z.d(); }
   Mesh mesh() const {
-    return Mesh{K, numV, B.d(), area.d(), nbr.i(), flg.i(), V0T.i(), v2tPtr.i(), v2tIdx.i()};
+    return Mesh{K, numV, B.d(), area.d(), nbr.i(), flg.i(), V0T.i()};
   }
   MinMaxSink sink() const {
     return MinMaxSink{cent.d(), partials.d()};
@@ -158,11 +206,33 @@ struct festung_problem {
     b.alloc(n * sizeof(T));
     HIPCHK(hipMemcpy(b.p, h, n * sizeof(T), hipMemcpyHostToDevice));
   }
+  // K x cols column-major array, reference order -> device order
+  std::vector<double> to_dev(const double* src, int cols) const {
+    std::vector<double> out(size_t(K) * cols);
+    for (int j = 0; j < cols; ++j)
+      for (int d = 0; d < K; ++d) out[d + size_t(j) * K] = src[eperm[d] + size_t(j) * K];
+    return out;
+  }
+  void from_dev(const double* dev, int cols, double* dst) const {
+    for (int j = 0; j < cols; ++j)
+      for (int d = 0; d < K; ++d) dst[eperm[d] + size_t(j) * K] = dev[d + size_t(j) * K];
+  }
+  void upload_elem(DBuf& b, const double* src, int cols, bool realloc = true) {
+    std::vector<double> v = to_dev(src, cols);
+    if (realloc || !b.p) b.alloc(v.size() * sizeof(double));
+    HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  void download_elem(const DBuf& b, const double* devp, int cols, double* dst) {
+    (void)b;
+    std::vector<double> v(size_t(K) * cols);
+    HIPCHK(hipMemcpy(v.data(), devp, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+    from_dev(v.data(), cols, dst);
+  }
 
   // raw vertex-patch min / max of the centroid values just written to `cent`,
   // and the global min / max from the producer's per-block partials
   void enqueue_vertex() {
-    VertexArgs va{numV, v2tPtr.i(), v2tIdx.i(), cent.d(), partials.d(), nblocks, gmm.d(), vmin.d(), vmax.d()};
+    VertexArgs va{numV, maxDeg, v2tPad.i(), cent.d(), partials.d(), nblocks, gmm.d(), vmin.d(), vmax.d()};
     timed(FESTUNG_KERNEL_VERTEX, [&] { HIPCHK(launch_vertex(va, stream)); });
   }
 
@@ -281,7 +351,7 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     const int N = h->N = h->tabs.N;
     const int K = h->K = g.numT;
     h->numV = g.numV;
-    h->areaT = g.areaT;
+    h->areaT = g.areaT;  // reference order
     if (int64_t(K) * N > (int64_t(1) << 31) - 1) fail(FESTUNG_ERR_ARG, "K*N exceeds int32 indexing");
 
     // reference tables, one buffer
@@ -309,26 +379,52 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     o.total = int(tb.size());
     h->upload(h->tab, tb.data(), tb.size());
 
-    // mesh
-    h->upload(h->B, g.B.data(), g.B.size());
-    h->upload(h->area, g.areaE0T.data(), g.areaE0T.size());
-    h->upload(h->nuE0T, g.nuE0T.data(), g.nuE0T.size());
-    h->upload(h->V0T, g.V0T.data(), g.V0T.size());
-    h->upload(h->v2tPtr, g.v2tPtr.data(), g.v2tPtr.size());
-    h->upload(h->v2tIdx, g.v2tIdx.data(), g.v2tIdx.size());
+    // device order of elements and vertices (Hilbert curve)
+    h->eperm = hilbert_order(g.baryT.data(), g.baryT.data() + K, K);
+    h->vperm = hilbert_order(g.coordV.data(), g.coordV.data() + g.numV, g.numV);
+    h->einv.assign(K, 0);
+    h->vinv.assign(g.numV, 0);
+    for (int32_t d = 0; d < K; ++d) h->einv[h->eperm[d]] = d;
+    for (int32_t d = 0; d < g.numV; ++d) h->vinv[h->vperm[d]] = d;
+    h->V0Tref = g.V0T;
+    h->upload(h->dEperm, h->eperm.data(), h->eperm.size());
+    h->upload(h->dEinv, h->einv.data(), h->einv.size());
+
+    // mesh, in device order
+    h->upload_elem(h->B, g.B.data(), 4);
+    h->upload_elem(h->area, g.areaE0T.data(), 3);
+    h->upload_elem(h->nuE0T, g.nuE0T.data(), 6);
+    {
+      std::vector<int32_t> v0t(3 * size_t(K));
+      for (int n = 0; n < 3; ++n)
+        for (int32_t d = 0; d < K; ++d) v0t[d + size_t(n) * K] = h->vinv[g.V0T[h->eperm[d] + size_t(n) * K]];
+      h->upload(h->V0T, v0t.data(), v0t.size());
+      // padded vertex -> element lists, maxDeg x numV (device orders)
+      int md = 0;
+      for (int32_t v = 0; v < g.numV; ++v) md = std::max(md, g.v2tPtr[v + 1] - g.v2tPtr[v]);
+      h->maxDeg = md;
+      std::vector<int32_t> pad(size_t(md) * g.numV, -1);
+      for (int32_t dv = 0; dv < g.numV; ++dv) {
+        const int32_t v = h->vperm[dv];
+        for (int32_t i = g.v2tPtr[v], j = 0; i < g.v2tPtr[v + 1]; ++i, ++j)
+          pad[size_t(j) * g.numV + dv] = h->einv[g.v2tIdx[i]];
+      }
+      h->upload(h->v2tPad, pad.data(), pad.size());
+    }
     // neighbours and flags (markE0TE0T, markElemEdges, markE0TbdrN/D of
     // core/assembleMatEdgePhiPhiValUpwind.m:261-299)
     std::vector<int32_t> nb(3 * size_t(K)), fl(K, 0);
-    for (int32_t k = 0; k < K; ++k)
+    for (int32_t d = 0; d < K; ++d)
       for (int n = 0; n < 3; ++n) {
-        size_t kn = k + size_t(n) * K;
+        const int32_t k = h->eperm[d];
+        const size_t kn = k + size_t(n) * K, dn = d + size_t(n) * K;
         const bool notN = !g.markE0TbdrN[kn];
         const bool hasNb = g.nbr[kn] >= 0;
-        nb[kn] = (hasNb && notN) ? g.nbr[kn] : -1;
+        nb[dn] = (hasNb && notN) ? h->einv[g.nbr[kn]] : -1;
         const bool markElemEdges = hasNb || g.idE0T[kn] != 0;
-        if (hasNb) fl[k] |= (g.nbrEdge[kn] & 3) << (2 * n);
-        if (notN && markElemEdges) fl[k] |= 1 << (6 + n);
-        if (g.markE0TbdrD[kn]) fl[k] |= 1 << (9 + n);
+        if (hasNb) fl[d] |= (g.nbrEdge[kn] & 3) << (2 * n);
+        if (notN && markElemEdges) fl[d] |= 1 << (6 + n);
+        if (g.markE0TbdrD[kn]) fl[d] |= 1 << (9 + n);
       }
     h->upload(h->nbr, nb.data(), nb.size());
     h->upload(h->flg, fl.data(), fl.size());
@@ -393,10 +489,9 @@ int festung_set_velocity_disc(festung_problem* h, const double* u1Disc, const do
     check_handle(h);
     if (!u1Disc || !u2Disc || !vNormal) fail(FESTUNG_ERR_ARG, "null pointer argument");
     h->set_device();
-    const size_t nk = size_t(h->N) * h->K * sizeof(double);
-    HIPCHK(hipMemcpy(h->u1.p, u1Disc, nk, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->u2.p, u2Disc, nk, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->vn.p, vNormal, size_t(3) * h->tabs.Re * h->K * sizeof(double), hipMemcpyHostToDevice));
+    h->upload_elem(h->u1, u1Disc, h->N, false);
+    h->upload_elem(h->u2, u2Disc, h->N, false);
+    h->upload_elem(h->vn, vNormal, 3 * h->tabs.Re, false);
     h->hasVelocity = true;
   });
 }
@@ -409,13 +504,13 @@ int festung_set_velocity_points(festung_problem* h, const double* u1q, const dou
     h->set_device();
     const int K = h->K;
     DBuf q, e1, e2;
-    h->upload(q, u1q, size_t(h->tabs.RE) * K);
+    h->upload_elem(q, u1q, h->tabs.RE);
     HIPCHK(launch_project(h->p, false, K, q.d(), h->tab.d(), h->off, h->u1.d(), h->stream));
     sync(h);
-    HIPCHK(hipMemcpy(q.p, u2q, size_t(h->tabs.RE) * K * sizeof(double), hipMemcpyHostToDevice));
+    h->upload_elem(q, u2q, h->tabs.RE, false);
     HIPCHK(launch_project(h->p, false, K, q.d(), h->tab.d(), h->off, h->u2.d(), h->stream));
-    h->upload(e1, u1e, size_t(3) * h->tabs.Re * K);
-    h->upload(e2, u2e, size_t(3) * h->tabs.Re * K);
+    h->upload_elem(e1, u1e, 3 * h->tabs.Re);
+    h->upload_elem(e2, u2e, 3 * h->tabs.Re);
     HIPCHK(launch_vnormal(K, h->tabs.Re, h->nuE0T.d(), e1.d(), e2.d(), h->vn.d(), h->stream));
     sync(h);
     h->hasVelocity = true;
@@ -434,8 +529,8 @@ int festung_set_source_points(festung_problem* h, const double* fq, const double
     const int K = h->K;
     if (!h->rhs.p) h->rhs.alloc(size_t(h->N) * K * sizeof(double));
     DBuf f, c;
-    if (fq) h->upload(f, fq, size_t(h->tabs.RE) * K);
-    if (cDe) h->upload(c, cDe, size_t(3) * h->tabs.Re * K);
+    if (fq) h->upload_elem(f, fq, h->tabs.RE);
+    if (cDe) h->upload_elem(c, cDe, 3 * h->tabs.Re);
     HIPCHK(launch_source(h->p, h->mesh(), f.d(), c.d(), h->vn.d(), h->tab.d(), h->off, h->rhs.d(), h->stream));
     sync(h);
     h->hasRhs = true;
@@ -451,7 +546,9 @@ int festung_set_vertex_dirichlet(festung_problem* h, const double* vertexD) {
       h->hasVertexD = false;
       return;
     }
-    HIPCHK(hipMemcpy(h->vertexD.p, vertexD, size_t(h->numV) * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> v(h->numV);
+    for (int32_t d = 0; d < h->numV; ++d) v[d] = vertexD[h->vperm[d]];
+    HIPCHK(hipMemcpy(h->vertexD.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
     h->hasVertexD = true;
   });
 }
@@ -467,11 +564,11 @@ int festung_project(festung_problem* h, int ord, const double* fq, double* cDisc
     else fail(FESTUNG_ERR_ARG, "projection order %d not available (use %d or %d)", ord, h->tabs.ordE, h->tabs.ordF);
     const int R = ordF ? h->tabs.RF : h->tabs.RE;
     DBuf q, o;
-    h->upload(q, fq, size_t(R) * h->K);
+    h->upload_elem(q, fq, R);
     o.alloc(size_t(h->N) * h->K * sizeof(double));
     HIPCHK(launch_project(h->p, ordF, h->K, q.d(), h->tab.d(), h->off, o.d(), h->stream));
     sync(h);
-    if (cDisc_out) HIPCHK(hipMemcpy(cDisc_out, o.p, o.bytes, hipMemcpyDeviceToHost));
+    if (cDisc_out) h->download_elem(o, o.d(), h->N, cDisc_out);
     if (set_state) {
       h->finish_stage_sequence();
       HIPCHK(hipMemcpy(h->X.p, o.p, o.bytes, hipMemcpyDeviceToDevice));
@@ -486,7 +583,7 @@ int festung_set_state(festung_problem* h, const double* c) {
     if (!c) fail(FESTUNG_ERR_ARG, "null pointer argument");
     h->set_device();
     h->finish_stage_sequence();
-    HIPCHK(hipMemcpy(h->X.p, c, size_t(h->N) * h->K * sizeof(double), hipMemcpyHostToDevice));
+    h->upload_elem(h->X, c, h->N, false);
     h->cur = 0;
   });
 }
@@ -497,7 +594,7 @@ int festung_get_state(festung_problem* h, double* c) {
     if (!c) fail(FESTUNG_ERR_ARG, "null pointer argument");
     h->set_device();
     sync(h);
-    HIPCHK(hipMemcpy(c, h->buf(h->cur), size_t(h->N) * h->K * sizeof(double), hipMemcpyDeviceToHost));
+    h->download_elem(h->X, h->buf(h->cur), h->N, c);
   });
 }
 
@@ -507,7 +604,7 @@ int festung_set_state_dev(festung_problem* h, const double* d) {
     if (!d) fail(FESTUNG_ERR_ARG, "null pointer argument");
     h->set_device();
     h->finish_stage_sequence();
-    HIPCHK(hipMemcpyAsync(h->X.p, d, size_t(h->N) * h->K * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(launch_permute(h->K, h->N, d, h->dEperm.i(), h->X.d(), h->stream));
     h->cur = 0;
   });
 }
@@ -517,8 +614,7 @@ int festung_get_state_dev(festung_problem* h, double* d) {
     check_handle(h);
     if (!d) fail(FESTUNG_ERR_ARG, "null pointer argument");
     h->set_device();
-    HIPCHK(hipMemcpyAsync(d, h->buf(h->cur), size_t(h->N) * h->K * sizeof(double), hipMemcpyDeviceToDevice,
-                          h->stream));
+    HIPCHK(launch_permute(h->K, h->N, h->buf(h->cur), h->dEinv.i(), d, h->stream));
   });
 }
 
@@ -635,7 +731,7 @@ int festung_rhs(festung_problem* h, double* cDot) {
     a.sink = h->sink();
     HIPCHK(launch_stage(h->p, OUT_DOT_DG, a, h->stream));
     sync(h);
-    HIPCHK(hipMemcpy(cDot, o.p, o.bytes, hipMemcpyDeviceToHost));
+    h->download_elem(o, o.d(), h->N, cDot);
   });
 }
 
@@ -646,13 +742,13 @@ int festung_limiter_info(festung_problem* h, double* alphaE, double* minV0T, dou
     if (!(h->flags & FESTUNG_FLAG_LIMITER)) fail(FESTUNG_ERR_STATE, "problem created without the limiter");
     sync(h);
     const int K = h->K;
-    if (alphaE) HIPCHK(hipMemcpy(alphaE, h->alpha.p, size_t(K) * sizeof(double), hipMemcpyDeviceToHost));
+    if (alphaE) h->download_elem(h->alpha, h->alpha.d(), 1, alphaE);
     if (minV0T || maxV0T) {
       // bounds of the last (solution) limiter application, as the limit
       // kernel forms them: shifted patch min / max joined with c_D
-      std::vector<int32_t> v0t(3 * size_t(K));
+      std::vector<int32_t> v0t(3 * size_t(K));   // device vertex ids, reference element order
+      for (size_t i = 0; i < v0t.size(); ++i) v0t[i] = h->vinv[h->V0Tref[i]];
       std::vector<double> vmn(h->numV), vmx(h->numV), vD(h->numV, std::nan("")), gm(2);
-      HIPCHK(hipMemcpy(v0t.data(), h->V0T.p, v0t.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(vmn.data(), h->vmin.p, vmn.size() * sizeof(double), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(vmx.data(), h->vmax.p, vmx.size() * sizeof(double), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(gm.data(), h->gmm.p, 2 * sizeof(double), hipMemcpyDeviceToHost));
@@ -682,7 +778,7 @@ int festung_l2_error(festung_problem* h, int ord, const double* cq, double* err)
     else fail(FESTUNG_ERR_ARG, "L2 error order %d not available (use %d or %d)", ord, h->tabs.ordE, h->tabs.ordF);
     const int R = ordF ? h->tabs.RF : h->tabs.RE;
     DBuf q, part;
-    h->upload(q, cq, size_t(R) * h->K);
+    h->upload_elem(q, cq, R);
     part.alloc(size_t(h->nblocks) * sizeof(double));
     HIPCHK(launch_l2(h->p, ordF, h->K, h->B.d(), h->buf(h->cur), q.d(), h->tab.d(), h->off, part.d(), h->stream));
     sync(h);

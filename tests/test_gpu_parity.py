@@ -147,3 +147,20 @@ def test_graph_matches_stagewise(gpu):
     for n in range(2, 6):
         product_step(b, n)
     assert np.array_equal(a.problem.get_state(), b.problem.get_state())
+
+
+def test_device_pointer_state(gpu):
+    """State transfer through device pointers ([N][K] in the reference's
+    element order) round-trips exactly through the device's element order."""
+    import torch
+    tc = TC.rotating_humps()
+    d = product_problem(tc, 2, 1 / 8, 3, False)
+    rng = np.random.default_rng(3)
+    c = rng.standard_normal((d.K, d.N))
+    t = torch.tensor(np.ascontiguousarray(c.T), device='cuda')
+    d.problem.set_state_dev(t)
+    assert np.array_equal(d.problem.get_state(), c)
+    out = torch.empty_like(t)
+    d.problem.get_state_dev(out)
+    d.problem.sync()
+    assert np.array_equal(out.cpu().numpy().T, c)
