@@ -77,8 +77,49 @@ def cpu_baseline(hmax, p, ordRK, lim, tc_factory):
     return pd.K * N * S / pd.timeLoop, pd.K, pd.timeLoop
 
 
+def run_batched(nprob, nsteps, seed0=4000):
+    """The batched workload: nprob problems (K log-uniform in [1k, 64k],
+    p uniform in {1, 2, 3}, SSP-RK min(p+1, 3), seeded velocities and humps),
+    nsteps steps each.  Returns (batched seconds, one-launch-per-problem
+    seconds, DoF-stage updates, setup seconds)."""
+    import numpy as np
+    import torch
+    from festung_amd import batch as BT
+    from festung_amd.core import Batch
+    t0 = time.perf_counter()
+    specs = BT.batch_specs(nprob, seed0=seed0)
+    inputs = [BT.ProblemInputs(s, device='cuda') for s in specs]
+    bp = BT.BatchedProblems(inputs)
+    singles = [BT.make_single(inp) for inp in inputs]
+    sb = [Batch([s]) for s in singles]
+    # warm-up: graph capture + one run of each form
+    bp.step(nsteps)
+    bp.sync()
+    for b, inp in zip(sb, inputs):
+        b.step(nsteps, [inp.tau])
+        b.sync()
+    setup = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bp.step(nsteps)
+    bp.sync()
+    t_batched = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for b, inp in zip(sb, inputs):
+        b.step(nsteps, [inp.tau])
+        b.sync()
+    t_single = time.perf_counter() - t0
+    upd = sum(inp.K * (inp.p + 1) * (inp.p + 2) // 2 * inp.spec['ordRK'] for inp in inputs) * nsteps
+    K_tot = sum(inp.K for inp in inputs)
+    for b in sb:
+        b.close()
+    return t_batched, t_single, upd, setup, K_tot
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument('--workload', choices=('step', 'batched'), default='step')
+    ap.add_argument('--nprob', type=int, default=512)
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=100)
     ap.add_argument('--warmup', type=int, default=10)
@@ -101,6 +142,18 @@ def main():
     if not torch.cuda.is_available():
         print('no GPU visible', file=sys.stderr)
         sys.exit(1)
+    if a.workload == 'batched':
+        nsteps = a.steps if a.steps != 100 else 50
+        tb, ts, upd, setup, Ktot = run_batched(a.nprob, nsteps)
+        print(json.dumps({
+            'metric': 'DoF-stage updates/s, batched independent problems (one graph launch) vs one launch per problem',
+            'value': upd / tb, 'unit': 'DoF-stage updates/s', 'n_gpus': 1, 'steps': nsteps, 'warmup': 1,
+            'ms_per_step': 1e3 * tb / nsteps, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+            'dtype': 'f64', 'data': 'synthetic',
+            'config': {'workload': '%d problems, domainSquare K log-uniform [1k, 64k] (total K=%d), p in {1,2,3}, '
+                                   'SSP-RK min(p+1,3), %d steps, limiter off' % (a.nprob, Ktot, nsteps)},
+            'batched_s': tb, 'one_launch_per_problem_s': ts, 'speedup': ts / tb, 'setup_s': setup}))
+        return
     from festung_amd.advection import steps as S
     from festung_amd.advection import testcases as TC
 

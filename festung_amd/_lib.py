@@ -38,6 +38,7 @@ SIGNATURES = {
     'festung_domain_square': (C.c_int, [C.c_double, C.c_double, C.c_double, C.POINTER(vp)]),
     'festung_grid_create': (C.c_int, [dp, C.c_int32, ip, C.c_int32, C.POINTER(vp)]),
     'festung_grid_set_boundary': (C.c_int, [vp, ip, C.POINTER(C.c_uint8)]),
+    'festung_grid_concat': (C.c_int, [C.POINTER(vp), C.c_int32, C.POINTER(vp)]),
     'festung_grid_sizes': (C.c_int, [vp, ip, ip, ip]),
     'festung_grid_field_size': (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     'festung_grid_get_i32': (C.c_int, [vp, C.c_char_p, ip]),
@@ -66,6 +67,11 @@ SIGNATURES = {
     'festung_rhs': (C.c_int, [vp, dp]),
     'festung_limiter_info': (C.c_int, [vp, dp, dp, dp]),
     'festung_l2_error': (C.c_int, [vp, C.c_int, dp, dp]),
+    'festung_set_tau_field': (C.c_int, [vp, dp]),
+    'festung_batch_create': (C.c_int, [C.POINTER(vp), C.c_int32, C.POINTER(vp)]),
+    'festung_batch_step': (C.c_int, [vp, C.c_int32, dp]),
+    'festung_batch_sync': (C.c_int, [vp]),
+    'festung_batch_destroy': (None, [vp]),
     'festung_profile': (C.c_int, [vp, C.c_int]),
     'festung_profile_read': (C.c_int, [vp, dp, C.POINTER(C.c_int64)]),
     'festung_sync': (C.c_int, [vp]),

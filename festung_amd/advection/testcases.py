@@ -15,6 +15,19 @@ class Testcase(dict):
     __getattr__ = dict.__getitem__
 
 
+def _xp(x):
+    """Array namespace of x: the closures accept numpy arrays and, for fast
+    set-up of large batches, torch tensors (evaluated where they live)."""
+    if type(x).__module__.startswith('torch'):
+        import torch
+        return torch
+    return np
+
+
+def _zeros(t, x1, x2):
+    return _xp(x1).zeros_like(x1)
+
+
 def solid_body():
     """LeVeque's solid body rotation; advection/configureProblem.m:106-117."""
     G = lambda x1, x2, a, b: (1 / 0.15) * np.sqrt((x1 - a) ** 2 + (x2 - b) ** 2)
@@ -54,12 +67,20 @@ def stream_velocity(seed, kmax, lmax, amplitude, rotation=True, unit_max_speed=F
     ks = [(k, l) for k in range(1, kmax + 1) for l in range(1, lmax + 1)]
     a = rng.standard_normal(len(ks))
 
+    kx = sorted({k for k, _ in ks}); ly = sorted({l for _, l in ks})
+
     def pert(x1, x2):
-        p1 = np.zeros_like(x1, dtype=float); p2 = np.zeros_like(x1, dtype=float)
+        xp = _xp(x1)
+        sx = {k: xp.sin(k * np.pi * x1) for k in kx}
+        cx = {k: xp.cos(k * np.pi * x1) for k in kx}
+        sy = {l: xp.sin(l * np.pi * x2) for l in ly}
+        cy = {l: xp.cos(l * np.pi * x2) for l in ly}
+        p1 = xp.zeros_like(x1)
+        p2 = xp.zeros_like(x1)
         for (k, l), ak in zip(ks, a):
-            s = ak / (np.pi * np.sqrt(k * k + l * l))
-            p1 += s * np.sin(k * np.pi * x1) * (l * np.pi) * np.cos(l * np.pi * x2)     # d psi / dy
-            p2 -= s * (k * np.pi) * np.cos(k * np.pi * x1) * np.sin(l * np.pi * x2)     # -d psi / dx
+            s = float(ak / (np.pi * np.sqrt(k * k + l * l)))
+            p1 = p1 + s * sx[k] * (l * np.pi) * cy[l]       # d psi / dy
+            p2 = p2 - s * (k * np.pi) * cx[k] * sy[l]       # -d psi / dx
         return p1, p2
 
     xs = np.linspace(0, 1, 257)
@@ -73,6 +94,8 @@ def stream_velocity(seed, kmax, lmax, amplitude, rotation=True, unit_max_speed=F
     else:
         scale_p = amplitude / np.max(np.hypot(q1, q2))
         scale_all = 1.0
+
+    scale_all = float(scale_all); scale_p = float(scale_p)
 
     def u1(t, x1, x2):
         return scale_all * ((0.5 - x2 if rotation else 0.0) + scale_p * pert(x1, x2)[0])
@@ -91,9 +114,11 @@ def gaussian_humps(seed, n=8):
     A = rng.uniform(0.5, 1.0, size=n)
 
     def c0(x1, x2):
-        r = np.zeros_like(x1, dtype=float)
+        xp = _xp(x1)
+        r = xp.zeros_like(x1)
         for j in range(n):
-            r += A[j] * np.exp(-((x1 - xc[j, 0]) ** 2 + (x2 - xc[j, 1]) ** 2) / sg[j] ** 2)
+            r = r + float(A[j]) * xp.exp(-((x1 - float(xc[j, 0])) ** 2 + (x2 - float(xc[j, 1])) ** 2)
+                                         / float(sg[j]) ** 2)
         return r
     return c0
 
@@ -119,7 +144,7 @@ def rotating_humps(vel_seed=21, hump_seed=22, amplitude=0.05, kmax=4, lmax=4):
     """Rotation + 16-mode perturbation (amplitude 0.05) carrying 8 Gaussian
     humps, zero inflow and source: the p = 2 drop-in and batched workloads."""
     u1, u2, _ = stream_velocity(vel_seed, kmax, lmax, amplitude)
-    zero = lambda t, x1, x2: np.zeros_like(x1)
+    zero = _zeros
     return Testcase(isAnalytical=False, c0Cont=gaussian_humps(hump_seed), u1Cont=u1, u2Cont=u2,
                     fCont=zero, cDCont=zero, gNCont=zero, fZero=True, cDZero=True, isStationary=True)
 
@@ -128,7 +153,7 @@ def random_discontinuous(vel_seed=31, c_seed=32):
     """32-mode stream-function velocity scaled to unit maximum speed carrying
     discontinuous indicator data: the p = 4 workload."""
     u1, u2, _ = stream_velocity(vel_seed, 8, 4, 1.0, rotation=False, unit_max_speed=True)
-    zero = lambda t, x1, x2: np.zeros_like(x1)
+    zero = _zeros
     return Testcase(isAnalytical=False, c0Cont=indicators(c_seed), u1Cont=u1, u2Cont=u2,
                     fCont=zero, cDCont=zero, gNCont=zero, fZero=True, cDZero=True, isStationary=True)
 

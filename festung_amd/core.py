@@ -63,6 +63,14 @@ class Grid:
         check(lib().festung_grid_create(ptr(coordV), coordV.shape[0], ptr(V0T), V0T.shape[0], C.byref(h)))
         return cls(h.value)
 
+    @classmethod
+    def concat(cls, grids):
+        """Disjoint union of grids (ids offset in order)."""
+        arr = (C.c_void_p * len(grids))(*[g._h.value for g in grids])
+        h = C.c_void_p()
+        check(lib().festung_grid_concat(arr, len(grids), C.byref(h)))
+        return cls(h.value)
+
     def set_boundary(self, idE, markE0TbdrN=None):
         idE = fortran(idE, (self.numE,), np.int32)
         mN = None if markE0TbdrN is None else fortran(markE0TbdrN, (self.numT, 3), np.uint8)
@@ -181,6 +189,10 @@ class Problem:
         c = None if cDe is None else fortran(cDe, (self.K, 3, self.Re))
         check(lib().festung_set_source_points(self._h, ptr(f), ptr(c)))
 
+    def set_tau_field(self, tauK):
+        t = None if tauK is None else fortran(tauK, (self.K,))
+        check(lib().festung_set_tau_field(self._h, ptr(t)))
+
     def set_vertex_dirichlet(self, vertexD):
         v = None if vertexD is None else fortran(vertexD, (self.grid.numV,))
         check(lib().festung_set_vertex_dirichlet(self._h, ptr(v)))
@@ -253,3 +265,32 @@ class Problem:
 
     def sync(self):
         check(lib().festung_sync(self._h))
+
+
+class Batch:
+    """Several problems advanced together as one hipGraph launch
+    (festung_batch_* of the C-ABI)."""
+
+    def __init__(self, problems):
+        self.problems = list(problems)
+        arr = (C.c_void_p * len(self.problems))(*[p._h.value for p in self.problems])
+        h = C.c_void_p()
+        check(lib().festung_batch_create(arr, len(self.problems), C.byref(h)))
+        self._h = h
+
+    def step(self, nsteps, taus):
+        t = np.ascontiguousarray(taus, dtype=np.float64)
+        assert t.size == len(self.problems)
+        check(lib().festung_batch_step(self._h, int(nsteps), ptr(t)))
+
+    def sync(self):
+        check(lib().festung_batch_sync(self._h))
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().festung_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        if L._lib is not None and getattr(self, '_h', None) is not None:
+            self.close()

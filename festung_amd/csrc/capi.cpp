@@ -161,6 +161,26 @@ int festung_grid_set_boundary(festung_grid* h, const int32_t* idE, const uint8_t
   });
 }
 
+int festung_grid_concat(const festung_grid* const* grids, int32_t n, festung_grid** out) {
+  return guard([&] {
+    if (!grids || !out || n < 1) fail(FESTUNG_ERR_ARG, "invalid argument");
+    *out = nullptr;
+    std::vector<const Grid*> gs;
+    for (int32_t i = 0; i < n; ++i) {
+      if (!grids[i]) fail(FESTUNG_ERR_ARG, "null grid %d", i);
+      gs.push_back(&grids[i]->g);
+    }
+    auto* h = new festung_grid;
+    try {
+      h->g = concat_grids(gs);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
 int festung_grid_sizes(const festung_grid* h, int32_t* numT, int32_t* numV, int32_t* numE) {
   return guard([&] {
     if (!h) fail(FESTUNG_ERR_ARG, "null grid");

@@ -281,3 +281,87 @@ void map_ref_to_phy(const Grid& g, int nq, const double* q1, const double* q2, d
 }
 
 }  // namespace festung
+
+namespace festung {
+
+namespace {
+// Concatenate column-major arrays with `cols` columns; part i has rows[i] rows.
+template <class T, class F>
+std::vector<T> cat_cols(const std::vector<const Grid*>& gs, const std::vector<int64_t>& rows, int cols,
+                        F&& get) {
+  int64_t R = 0;
+  for (int64_t r : rows) R += r;
+  std::vector<T> out(size_t(R) * cols);
+  for (int c = 0; c < cols; ++c) {
+    int64_t off = 0;
+    for (size_t i = 0; i < gs.size(); ++i) {
+      for (int64_t r = 0; r < rows[i]; ++r) out[size_t(c) * R + off + r] = get(i, size_t(c) * rows[i] + r);
+      off += rows[i];
+    }
+  }
+  return out;
+}
+}  // namespace
+
+Grid concat_grids(const std::vector<const Grid*>& gs) {
+  if (gs.empty()) fail(FESTUNG_ERR_ARG, "concat: no grids");
+  const size_t n = gs.size();
+  std::vector<int64_t> T(n), V(n), E(n), V1(n);
+  std::vector<int32_t> offT(n), offV(n), offE(n);
+  int64_t sT = 0, sV = 0, sE = 0;
+  for (size_t i = 0; i < n; ++i) {
+    T[i] = gs[i]->numT;
+    V[i] = gs[i]->numV;
+    E[i] = gs[i]->numE;
+    offT[i] = int32_t(sT);
+    offV[i] = int32_t(sV);
+    offE[i] = int32_t(sE);
+    sT += T[i];
+    sV += V[i];
+    sE += E[i];
+  }
+  if (sT > INT32_MAX / 4 || sV > INT32_MAX / 2) fail(FESTUNG_ERR_ARG, "concat: mesh too large for int32 ids");
+  Grid g;
+  g.numT = int32_t(sT);
+  g.numV = int32_t(sV);
+  g.numE = int32_t(sE);
+  auto shift = [](int32_t v, int32_t o) { return v < 0 ? v : v + o; };
+  g.coordV = cat_cols<double>(gs, V, 2, [&](size_t i, size_t j) { return gs[i]->coordV[j]; });
+  g.V0T = cat_cols<int32_t>(gs, T, 3, [&](size_t i, size_t j) { return shift(gs[i]->V0T[j], offV[i]); });
+  g.V0E = cat_cols<int32_t>(gs, E, 2, [&](size_t i, size_t j) { return shift(gs[i]->V0E[j], offV[i]); });
+  g.T0E = cat_cols<int32_t>(gs, E, 2, [&](size_t i, size_t j) { return shift(gs[i]->T0E[j], offT[i]); });
+  g.E0E = cat_cols<int32_t>(gs, E, 2, [&](size_t i, size_t j) { return gs[i]->E0E[j]; });
+  g.E0T = cat_cols<int32_t>(gs, T, 3, [&](size_t i, size_t j) { return shift(gs[i]->E0T[j], offE[i]); });
+  g.idE = cat_cols<int32_t>(gs, E, 1, [&](size_t i, size_t j) { return gs[i]->idE[j]; });
+  g.idE0T = cat_cols<int32_t>(gs, T, 3, [&](size_t i, size_t j) { return gs[i]->idE0T[j]; });
+  g.areaE = cat_cols<double>(gs, E, 1, [&](size_t i, size_t j) { return gs[i]->areaE[j]; });
+  g.nuE = cat_cols<double>(gs, E, 2, [&](size_t i, size_t j) { return gs[i]->nuE[j]; });
+  g.baryE = cat_cols<double>(gs, E, 2, [&](size_t i, size_t j) { return gs[i]->baryE[j]; });
+  g.areaT = cat_cols<double>(gs, T, 1, [&](size_t i, size_t j) { return gs[i]->areaT[j]; });
+  g.baryT = cat_cols<double>(gs, T, 2, [&](size_t i, size_t j) { return gs[i]->baryT[j]; });
+  g.B = cat_cols<double>(gs, T, 4, [&](size_t i, size_t j) { return gs[i]->B[j]; });
+  g.areaE0T = cat_cols<double>(gs, T, 3, [&](size_t i, size_t j) { return gs[i]->areaE0T[j]; });
+  g.sigE0T = cat_cols<double>(gs, T, 3, [&](size_t i, size_t j) { return gs[i]->sigE0T[j]; });
+  g.nuE0T = cat_cols<double>(gs, T, 6, [&](size_t i, size_t j) { return gs[i]->nuE0T[j]; });
+  g.coordV0T = cat_cols<double>(gs, T, 6, [&](size_t i, size_t j) { return gs[i]->coordV0T[j]; });
+  g.nbr = cat_cols<int32_t>(gs, T, 3, [&](size_t i, size_t j) { return shift(gs[i]->nbr[j], offT[i]); });
+  g.nbrEdge = cat_cols<int32_t>(gs, T, 3, [&](size_t i, size_t j) { return gs[i]->nbrEdge[j]; });
+  g.markE0Tint = cat_cols<uint8_t>(gs, T, 3, [&](size_t i, size_t j) { return gs[i]->markE0Tint[j]; });
+  g.markE0TbdrD = cat_cols<uint8_t>(gs, T, 3, [&](size_t i, size_t j) { return gs[i]->markE0TbdrD[j]; });
+  g.markE0TbdrN = cat_cols<uint8_t>(gs, T, 3, [&](size_t i, size_t j) { return gs[i]->markE0TbdrN[j]; });
+  g.markV0TbdrD = cat_cols<uint8_t>(gs, T, 3, [&](size_t i, size_t j) { return gs[i]->markV0TbdrD[j]; });
+  // vertex patches
+  g.v2tPtr.assign(size_t(sV) + 1, 0);
+  g.v2tIdx.clear();
+  g.v2tIdx.reserve(3 * size_t(sT));
+  for (size_t i = 0; i < n; ++i) {
+    const Grid& a = *gs[i];
+    for (int32_t v = 0; v < a.numV; ++v) {
+      for (int32_t t = a.v2tPtr[v]; t < a.v2tPtr[v + 1]; ++t) g.v2tIdx.push_back(a.v2tIdx[t] + offT[i]);
+      g.v2tPtr[offV[i] + v + 1] = int32_t(g.v2tIdx.size());
+    }
+  }
+  return g;
+}
+
+}  // namespace festung

@@ -36,6 +36,9 @@ Grid generate_grid_data(const double* coordV, int32_t numV, const int32_t* V0T, 
 Grid domain_square(double h, double a, double b);
 // Boundary marks for given boundary ids and Neumann marks.
 void mark_boundaries(Grid& g, const uint8_t* markE0TbdrN /* may be null */);
+// Disjoint union of several grids (ids offset in order); used to run many
+// independent problems of the same degree as one device problem.
+Grid concat_grids(const std::vector<const Grid*>& gs);
 // Physical coordinates of reference points: X{1,2} numT x nq (mapRef2Phy).
 void map_ref_to_phy(const Grid& g, int nq, const double* q1, const double* q2, double* X1, double* X2);
 

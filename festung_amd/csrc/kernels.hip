@@ -346,7 +346,8 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   } else {
     // RK combination (advection/solveSubStep.m:83 for the SSP weights)
     double nx[N];
-    rk_combine<N>(a.rk, a.tau, c, cdot, a.c0, a.acc, a.accOut, K, kk, active, nx);
+    const double tk = a.tauK ? a.tauK[kk] : a.tau;
+    rk_combine<N>(a.rk, tk, c, cdot, a.c0, a.acc, a.accOut, K, kk, active, nx);
     if constexpr (MODE == OUT_RK_DG) {
       if (active)
 #pragma unroll
@@ -532,7 +533,8 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
     double cs[N], nx[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) cs[i] = a.c[i * K + kk];
-    rk_combine<N>(a.rk, a.tau, cs, y, a.c0, a.acc, a.accOut, K, kk, active, nx);
+    const double tk = a.tauK ? a.tauK[kk] : a.tau;
+    rk_combine<N>(a.rk, tk, cs, y, a.c0, a.acc, a.accOut, K, kk, active, nx);
     if constexpr (MODE == LIM_DOT_RK_DG) {
       if (active)
 #pragma unroll

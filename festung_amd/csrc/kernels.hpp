@@ -73,6 +73,7 @@ struct StageArgs {
   const double* tab;
   TabOff off;
   double tau;
+  const double* tauK;    // per-element time step (batched problems) or null
   RKCoef rk;
   double* out;           // DG or Taylor coefficients
   MinMaxSink sink;       // for the Taylor outputs
@@ -105,6 +106,7 @@ struct LimitArgs {
   const double* acc;
   double* accOut;
   double tau;
+  const double* tauK;    // per-element time step or null
   RKCoef rk;
   double* out;           // DG (LIM_SOLUTION, LIM_DOT_RK_DG) or Taylor (LIM_DOT_RK_TAYLOR)
   MinMaxSink sink;

@@ -136,6 +136,8 @@ struct festung_problem {
   std::vector<StagePlan> plan;
   DBuf tab, B, area, nbr, flg, V0T, v2tPad, nuE0T;
   int maxDeg = 0;
+  DBuf tauK;
+  bool hasTauK = false;
   DBuf X, Y, Z, acc, T1, T2, cent, partials, gmm, vmin, vmax, vertexD, u1, u2, vn, rhs, alpha;
   bool hasRhs = false, hasVertexD = false, hasVelocity = false;
   int cur = 0;           // buffer holding the current state (0 = X)
@@ -254,6 +256,7 @@ struct festung_problem {
     a.tab = tab.d();
     a.off = off;
     a.tau = tau;
+    a.tauK = hasTauK ? tauK.d() : nullptr;
     a.rk = sp.rk;
     a.sink = sink();
     if (!lim) {
@@ -283,6 +286,7 @@ struct festung_problem {
       l.acc = acc.d();
       l.accOut = sp.accOut ? acc.d() : nullptr;
       l.tau = tau;
+      l.tauK = hasTauK ? tauK.d() : nullptr;
       l.rk = sp.rk;
       l.out = T2.d();
       l.alphaOut = nullptr;
@@ -790,6 +794,20 @@ int festung_l2_error(festung_problem* h, int ord, const double* cq, double* err)
   });
 }
 
+int festung_set_tau_field(festung_problem* h, const double* tauK) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    h->drop_graph();
+    if (!tauK) {
+      h->hasTauK = false;
+      return;
+    }
+    h->upload_elem(h->tauK, tauK, 1, !h->tauK.p);
+    h->hasTauK = true;
+  });
+}
+
 int festung_profile(festung_problem* h, int enable) {
   return guard([&] {
     check_handle(h);
@@ -823,6 +841,109 @@ int festung_profile_read(festung_problem* h, double* ms, int64_t* count) {
       h->pool.push_back(r.b);
     }
     h->recs.clear();
+  });
+}
+
+}  // extern "C"
+
+// Several problems advanced together: one captured hipGraph whose branches
+// are the problems' step sequences (each on its own stream), launched once.
+struct festung_batch {
+  std::vector<festung_problem*> hs;
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int nsteps = -1;
+  std::vector<double> taus;
+  std::vector<hipEvent_t> events;
+  void drop() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    nsteps = -1;
+  }
+  ~festung_batch() {
+    drop();
+    for (auto e : events) (void)hipEventDestroy(e);
+  }
+};
+
+extern "C" {
+
+int festung_batch_create(festung_problem* const* hs, int32_t n, festung_batch** out) {
+  return guard([&] {
+    if (!hs || !out || n < 1) fail(FESTUNG_ERR_ARG, "invalid argument");
+    *out = nullptr;
+    std::unique_ptr<festung_batch> b(new festung_batch);
+    for (int32_t i = 0; i < n; ++i) {
+      check_handle(hs[i]);
+      if (hs[i]->device != hs[0]->device) fail(FESTUNG_ERR_ARG, "batched problems must share one device");
+      for (int32_t j = 0; j < i; ++j)
+        if (hs[j] == hs[i]) fail(FESTUNG_ERR_ARG, "problem %d appears twice", i);
+      b->hs.push_back(hs[i]);
+    }
+    hs[0]->set_device();
+    b->device = hs[0]->device;
+    b->events.resize(size_t(n) + 1);
+    for (auto& e : b->events) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    *out = b.release();
+  });
+}
+
+void festung_batch_destroy(festung_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  (void)hipDeviceSynchronize();  // the graph may still run; problems may already be gone
+  delete b;
+}
+
+int festung_batch_step(festung_batch* b, int32_t nsteps, const double* tau) {
+  return guard([&] {
+    if (!b || !tau || nsteps < 0) fail(FESTUNG_ERR_ARG, "invalid argument");
+    if (nsteps == 0) return;
+    festung_problem* h0 = b->hs[0];
+    h0->set_device();
+    const size_t n = b->hs.size();
+    for (auto* h : b->hs) {
+      h->require_velocity();
+      h->finish_stage_sequence();
+      if (h->prof) fail(FESTUNG_ERR_STATE, "profiling is on for a batched problem");
+    }
+    std::vector<double> taus(tau, tau + n);
+    if (!b->exec || b->nsteps != nsteps || b->taus != taus) {
+      b->drop();
+      HIPCHK(hipStreamBeginCapture(h0->stream, hipStreamCaptureModeThreadLocal));
+      try {
+        HIPCHK(hipEventRecord(b->events[0], h0->stream));
+        for (size_t i = 1; i < n; ++i) HIPCHK(hipStreamWaitEvent(b->hs[i]->stream, b->events[0], 0));
+        for (size_t i = 0; i < n; ++i)
+          for (int s = 0; s < nsteps; ++s) b->hs[i]->enqueue_step(taus[i]);
+        for (size_t i = 1; i < n; ++i) {
+          HIPCHK(hipEventRecord(b->events[i], b->hs[i]->stream));
+          HIPCHK(hipStreamWaitEvent(h0->stream, b->events[i], 0));
+        }
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(h0->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(h0->stream, &b->graph));
+      HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->nsteps = nsteps;
+      b->taus = taus;
+    }
+    HIPCHK(hipGraphLaunch(b->exec, h0->stream));
+    for (auto* h : b->hs) h->cur = 0;
+  });
+}
+
+int festung_batch_sync(festung_batch* b) {
+  return guard([&] {
+    if (!b) fail(FESTUNG_ERR_ARG, "null batch");
+    b->hs[0]->set_device();
+    HIPCHK(hipStreamSynchronize(b->hs[0]->stream));
   });
 }
 

@@ -67,6 +67,12 @@ int festung_grid_create(const double* coordV, int32_t numV, const int32_t* V0T, 
  * advection/preprocessProblem.m:72-76. */
 int festung_grid_set_boundary(festung_grid* g, const int32_t* idE, const uint8_t* markE0TbdrN);
 
+/* Disjoint union of n grids (element, vertex and edge ids offset in order).
+ * Used to run many independent problems of one degree as one device
+ * problem (the batched form).  No counterpart in the reference, which
+ * solves one problem per run. */
+int festung_grid_concat(const festung_grid* const* grids, int32_t n, festung_grid** out);
+
 int festung_grid_sizes(const festung_grid* g, int32_t* numT, int32_t* numV, int32_t* numE);
 
 /* Copy a named grid field out.  int32 fields: "V0T" (numT x 3), "V0E",
@@ -173,6 +179,24 @@ int festung_stage(festung_problem* h, int nSubStep, double tau);
  * source and vertex Dirichlet data stay as set).  Replaces the main loop
  * main.m:151-166 over advection/solveStep.m:57-73 for time-independent data. */
 int festung_step(festung_problem* h, int nsteps, double tau);
+
+/* Per-element time step tauK (K values, reference element order) used by
+ * the RK combination instead of the scalar tau; NULL restores the scalar.
+ * Lets independent problems with different time steps share one device
+ * problem (see festung_grid_concat).  No counterpart in the reference. */
+int festung_set_tau_field(festung_problem* h, const double* tauK);
+
+/* Batched stepping: n problems advanced nsteps steps each (tau[i] for
+ * problem i; ignored where a tau field is set) as ONE hipGraph launch whose
+ * parallel branches are the problems' step sequences.  The graph is cached
+ * while nsteps and tau are unchanged.  festung_batch_step returns after
+ * enqueueing on the first problem's stream; festung_batch_sync waits.
+ * No counterpart in the reference (one problem per run, main.m:151-166). */
+typedef struct festung_batch festung_batch;
+int festung_batch_create(festung_problem* const* problems, int32_t n, festung_batch** out);
+int festung_batch_step(festung_batch* b, int32_t nsteps, const double* tau);
+int festung_batch_sync(festung_batch* b);
+void festung_batch_destroy(festung_batch* b);
 
 /* Apply the slope limiter to the current state.
  * Replaces core/applySlopeLimiterDisc.m:77-86 ('linear'). */
