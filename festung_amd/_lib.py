@@ -72,6 +72,8 @@ SIGNATURES = {
     'festung_batch_step': (C.c_int, [vp, C.c_int32, dp]),
     'festung_batch_sync': (C.c_int, [vp]),
     'festung_batch_destroy': (None, [vp]),
+    'festung_adjoint': (C.c_int, [vp, C.c_int32, C.c_double, dp, C.c_int32, dp, dp, dp, dp, dp, dp]),
+    'festung_adj_rhs': (C.c_int, [vp, dp, dp]),
     'festung_profile': (C.c_int, [vp, C.c_int]),
     'festung_profile_read': (C.c_int, [vp, dp, C.POINTER(C.c_int64)]),
     'festung_sync': (C.c_int, [vp]),

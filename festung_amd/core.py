@@ -252,6 +252,26 @@ class Problem:
         check(lib().festung_l2_error(self._h, int(ord_), ptr(q), C.byref(e)))
         return e.value
 
+    def adjoint(self, nsteps, tau, cObs, ckpt=0):
+        """Forward nsteps from the current state, loss and reverse sweep.
+        Returns dict(J, gradC0 (K x N), gradU1, gradU2 (K x N), gradVn
+        (K x 3 x Re), forward_ms, backward_ms)."""
+        K, N, Re = self.K, self.N, self.Re
+        o = fortran(cObs, (K, N))
+        J = np.zeros(1)
+        g0 = np.empty((K, N), order='F'); g1 = np.empty((K, N), order='F'); g2 = np.empty((K, N), order='F')
+        gv = np.empty((K, 3, Re), order='F')
+        tm = np.zeros(2)
+        check(lib().festung_adjoint(self._h, int(nsteps), float(tau), ptr(o), int(ckpt), ptr(J), ptr(g0), ptr(g1),
+                                    ptr(g2), ptr(gv), ptr(tm)))
+        return dict(J=float(J[0]), gradC0=g0, gradU1=g1, gradU2=g2, gradVn=gv, forward_ms=tm[0], backward_ms=tm[1])
+
+    def adj_rhs(self, mu):
+        m = fortran(mu, (self.K, self.N))
+        out = np.empty((self.K, self.N), order='F')
+        check(lib().festung_adj_rhs(self._h, ptr(m), ptr(out)))
+        return out
+
     KERNEL_KINDS = ('stage', 'vertex', 'limit_dot', 'limit', 'copy')
 
     def profile(self, enable):

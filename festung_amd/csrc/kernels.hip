@@ -709,6 +709,278 @@ __global__ __launch_bounds__(BLOCK) void l2_kernel(int K, const double* __restri
   }
 }
 
+
+// --------------------------------------------------------------- adjoint --
+// Transposed operator per element k (A = G1 + G2 - R):
+//   (G^T mu)_j  = sum_q w_q phi_j(q) [a(q) d1 mu(q) + b(q) d2 mu(q)]
+//   diagonal R blocks are symmetric (own outflow edges);
+//   (R^T mu)_k  += sum over neighbours k' of the block (k', k)^T mu_k',
+// gathered from the neighbour's side: its normal velocity, its upwind
+// selector and its trace, with the neighbour-trace table indexed by
+// (neighbour's local edge, own local edge).  No atomics.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void adj_stage_kernel(AdjArgs a) {
+  constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
+  __shared__ double sTheta[Re * N * 9];
+  __shared__ double sPhi1D[Re * N * 3];
+  for (int i = threadIdx.x; i < Re * N * 9; i += BLOCK) sTheta[i] = a.tab[a.off.theta + i];
+  for (int i = threadIdx.x; i < Re * N * 3; i += BLOCK) sPhi1D[i] = a.tab[a.off.phi1D + i];
+  __syncthreads();
+  const int K = a.m.K;
+  const int k = gtid();
+  if (k >= K) return;
+  const double* __restrict__ T = a.tab;
+  double mu[N], u1[N], u2[N], acc[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    mu[i] = a.mu[i * K + k];
+    u1[i] = a.u1[i * K + k];
+    u2[i] = a.u2[i * K + k];
+    acc[i] = 0.0;
+  }
+  Geo<P> g;
+  g.load(a.m.B, k, K);
+  if constexpr (P >= 1) {
+    constexpr int UQ = RE <= 16 ? RE : 4;
+#pragma unroll UQ
+    for (int q = 0; q < RE; ++q) {
+      double m1 = 0.0, m2 = 0.0, u1q = 0.0, u2q = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const double ph = T[a.off.phiE + q + j * RE];
+        m1 += mu[j] * T[a.off.gradE + q + j * RE];
+        m2 += mu[j] * T[a.off.gradE + q + j * RE + RE * N];
+        u1q += u1[j] * ph;
+        u2q += u2[j] * ph;
+      }
+      const double f = T[a.off.wE + q] *
+                       ((g.B22 * u1q - g.B12 * u2q) * m1 + (g.B11 * u2q - g.B21 * u1q) * m2);
+#pragma unroll
+      for (int j = 0; j < N; ++j) acc[j] += T[a.off.phiE + q + j * RE] * f;
+    }
+  }
+  const int fl = a.m.flags[k];
+#pragma unroll 1
+  for (int n = 0; n < 3; ++n) {
+    const double area = a.m.areaE[n * K + k];
+    // diagonal block (symmetric): own outflow
+    if (flag_out(fl, n)) {
+#pragma unroll
+      for (int r = 0; r < Re; ++r) {
+        const double v = a.vn[(r * 3 + n) * K + k];
+        if (v > 0.0) {
+          double t = 0.0;
+#pragma unroll
+          for (int i = 0; i < N; ++i) t += mu[i] * T[a.off.phi1D + r + i * Re + n * Re * N];
+          const double fx = area * v * T[a.off.wEdge + r] * t;
+#pragma unroll
+          for (int j = 0; j < N; ++j) acc[j] -= T[a.off.phi1D + r + j * Re + n * Re * N] * fx;
+        }
+      }
+    }
+    // transposed off-diagonal block: the neighbour's inflow through this edge
+    const int nb = a.m.nbr[n * K + k];
+    if (nb >= 0) {
+      const int np = flag_np(fl, n);  // neighbour's local edge
+      double mn[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) mn[i] = a.mu[i * K + nb];
+#pragma unroll
+      for (int r = 0; r < Re; ++r) {
+        const double v = a.vn[(r * 3 + np) * K + nb];
+        if (v < 0.0) {
+          double t = 0.0;
+#pragma unroll
+          for (int i = 0; i < N; ++i) t += mn[i] * sPhi1D[r + i * Re + np * Re * N];
+          const double fx = area * v * T[a.off.wEdge + r] * t;
+          const double* th = sTheta + r + np * Re * N + n * Re * N * 3;
+#pragma unroll
+          for (int j = 0; j < N; ++j) acc[j] -= th[j * Re] * fx;
+        }
+      }
+    }
+  }
+  double z[N];
+  const double idet = 1.0 / g.det;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) s += T[a.off.hatMinv + i + j * N] * acc[j];
+    z[i] = s * idet;
+  }
+  if (a.mode == 4) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) a.out[i * K + k] = z[i];
+    return;
+  }
+  const double tk = a.tauK ? a.tauK[k] : a.tau;
+  const double w1 = 1.0 - a.omega;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double y = w1 * (mu[i] + tk * z[i]);
+    const double om = a.omega * mu[i];
+    if (a.mode == 0) {
+      a.out[i * K + k] = y;
+      a.L0[i * K + k] = om;
+    } else if (a.mode == 1) {
+      a.out[i * K + k] = y;
+      a.L0[i * K + k] += om;
+    } else if (a.mode == 2) {
+      a.out[i * K + k] = a.L0[i * K + k] + om + y;
+    } else {
+      a.out[i * K + k] = om + y;
+    }
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(BLOCK) void adj_sens_kernel(SensArgs a) {
+  constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
+  __shared__ double sTheta[Re * N * 9];
+  for (int i = threadIdx.x; i < Re * N * 9; i += BLOCK) sTheta[i] = a.tab[a.off.theta + i];
+  __syncthreads();
+  const int K = a.m.K;
+  const int k = gtid();
+  if (k >= K) return;
+  const double* __restrict__ T = a.tab;
+  Geo<P> g;
+  g.load(a.m.B, k, K);
+  const double tk = a.tauK ? a.tauK[k] : a.tau;
+  const int fl = a.m.flags[k];
+  double gu1[N], gu2[N], gvn[3 * Re];
+#pragma unroll
+  for (int i = 0; i < N; ++i) gu1[i] = gu2[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3 * Re; ++i) gvn[i] = 0.0;
+  for (int s = 0; s < a.S; ++s) {
+    const double al = a.w[s] * tk;
+    double mu[N], c[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      mu[i] = a.mu[s][i * K + k];
+      c[i] = a.c[s][i * K + k];
+    }
+    if constexpr (P >= 1) {
+      constexpr int UQ = RE <= 16 ? RE : 4;
+#pragma unroll UQ
+      for (int q = 0; q < RE; ++q) {
+        double cq = 0.0, m1 = 0.0, m2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          cq += c[j] * T[a.off.phiE + q + j * RE];
+          m1 += mu[j] * T[a.off.gradE + q + j * RE];
+          m2 += mu[j] * T[a.off.gradE + q + j * RE + RE * N];
+        }
+        const double wc = al * T[a.off.wE + q] * cq;
+        const double f1 = wc * (g.B22 * m1 - g.B21 * m2);
+        const double f2 = wc * (g.B11 * m2 - g.B12 * m1);
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          gu1[l] += T[a.off.phiE + q + l * RE] * f1;
+          gu2[l] += T[a.off.phiE + q + l * RE] * f2;
+        }
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const double area = a.m.areaE[n * K + k];
+      const int nb = a.m.nbr[n * K + k];
+      const int np = flag_np(fl, n);
+      const bool outOn = flag_out(fl, n);
+#pragma unroll
+      for (int r = 0; r < Re; ++r) {
+        const double v = a.vn[(r * 3 + n) * K + k];
+        double cu = 0.0;
+        bool on = false;
+        if (v > 0.0 && outOn) {
+          on = true;
+#pragma unroll
+          for (int j = 0; j < N; ++j) cu += c[j] * T[a.off.phi1D + r + j * Re + n * Re * N];
+        } else if (v < 0.0 && nb >= 0) {
+          on = true;
+          const double* th = sTheta + r + n * Re * N + np * Re * N * 3;
+#pragma unroll
+          for (int j = 0; j < N; ++j) cu += a.c[s][j * K + nb] * th[j * Re];
+        }
+        if (on) {
+          double mr = 0.0;
+#pragma unroll
+          for (int i = 0; i < N; ++i) mr += mu[i] * T[a.off.phi1D + r + i * Re + n * Re * N];
+          gvn[r * 3 + n] -= al * area * T[a.off.wEdge + r] * mr * cu;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    a.Gu1[l * K + k] += gu1[l];
+    a.Gu2[l * K + k] += gu2[l];
+  }
+#pragma unroll
+  for (int i = 0; i < 3 * Re; ++i) a.Gvn[i * K + k] += gvn[i];
+}
+
+template <int P>
+__global__ __launch_bounds__(BLOCK) void loss_kernel(int K, const double* __restrict__ B, const double* __restrict__ c,
+                                                     const double* __restrict__ cObs, const double* __restrict__ tab,
+                                                     TabOff off, double* __restrict__ part, double* __restrict__ muT) {
+  constexpr int N = Dims<P>::N;
+  const int k = gtid();
+  double e = 0.0;
+  if (k < K) {
+    double d[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      d[i] = c[i * K + k] - cObs[i * K + k];
+      muT[i * K + k] = 2.0 * d[i];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s += tab[off.hatM + i + j * N] * d[j];
+      e += d[i] * s;
+    }
+    e *= B[k] * B[k + 3 * K] - B[k + 2 * K] * B[k + K];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o);
+  __shared__ double sm[BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < BLOCK / 64; ++i) t += sm[i];
+    part[blockIdx.x] = t;
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(BLOCK) void mass_kernel(int K, const double* __restrict__ B, const double* __restrict__ x,
+                                                     const double* __restrict__ tab, TabOff off,
+                                                     double* __restrict__ out) {
+  constexpr int N = Dims<P>::N;
+  const int k = gtid();
+  if (k >= K) return;
+  const double det = B[k] * B[k + 3 * K] - B[k + 2 * K] * B[k + K];
+  double v[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = x[i * K + k];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) s += tab[off.hatM + i + j * N] * v[j];
+    out[i * K + k] = det * s;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void fill_kernel(size_t n, double v, double* __restrict__ x) {
+  const size_t i = size_t(blockIdx.x) * BLOCK + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
 // ---------------------------------------------------------- launchers --
 #define FESTUNG_DISPATCH_P(P_, BODY)                           \
   switch (P_) {                                                \
@@ -824,6 +1096,38 @@ __global__ __launch_bounds__(BLOCK) void permute_kernel(int K, int cols, const d
 
 hipError_t launch_permute(int K, int cols, const double* src, const int* idx, double* dst, hipStream_t s) {
   hipLaunchKernelGGL(permute_kernel, grid_for(K), dim3(BLOCK), 0, s, K, cols, src, idx, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_stage(int p, const AdjArgs& a, hipStream_t s) {
+  dim3 gr = grid_for(a.m.K);
+  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_stage_kernel<PP>), gr, dim3(BLOCK), 0, s, a));
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_sens(int p, const SensArgs& a, hipStream_t s) {
+  dim3 gr = grid_for(a.m.K);
+  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_sens_kernel<PP>), gr, dim3(BLOCK), 0, s, a));
+  return hipGetLastError();
+}
+
+hipError_t launch_loss(int p, int K, const double* B, const double* c, const double* cObs, const double* tab,
+                       const TabOff& off, double* part, double* muT, hipStream_t s) {
+  dim3 gr = grid_for(K);
+  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((loss_kernel<PP>), gr, dim3(BLOCK), 0, s, K, B, c, cObs, tab, off, part, muT));
+  return hipGetLastError();
+}
+
+hipError_t launch_mass(int p, int K, const double* B, const double* x, const double* tab, const TabOff& off,
+                       double* out, hipStream_t s) {
+  dim3 gr = grid_for(K);
+  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((mass_kernel<PP>), gr, dim3(BLOCK), 0, s, K, B, x, tab, off, out));
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(size_t n, double v, double* x, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel, dim3(unsigned((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, n, v, x);
   return hipGetLastError();
 }
 

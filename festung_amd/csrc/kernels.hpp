@@ -123,7 +123,57 @@ struct ToTaylorArgs {
   MinMaxSink sink;
 };
 
+// Adjoint stage in mu = M^-1 lambda space:
+//   y = (1 - omega)(mu + tau_k M^-1 A^T mu)        (mode 0..3)
+//   mode 0: out = y, L0 = omega mu;  mode 1: out = y, L0 += omega mu;
+//   mode 2: out = L0 + omega mu + y; mode 3: out = omega mu + y;
+//   mode 4: out = M^-1 A^T mu (no RK weights, tau ignored)
+struct AdjArgs {
+  Mesh m;
+  const double* mu;
+  const double* u1;
+  const double* u2;
+  const double* vn;
+  const double* tab;
+  TabOff off;
+  double tau;
+  const double* tauK;
+  double omega;
+  int mode;
+  double* L0;
+  double* out;
+};
+
+// Velocity sensitivities of one time step, accumulated:
+//   G += sum_s alpha_s d/d(u1Disc, u2Disc, vn) [mu_{s+1}^T A c_s],
+// alpha_s = (1 - omega_s) tau_k.
+struct SensArgs {
+  Mesh m;
+  int S;
+  const double* mu[4];   // mu_{s+1}, s = 0 .. S-1
+  const double* c[4];    // c_s
+  double w[4];           // 1 - omega_s
+  double tau;
+  const double* tauK;
+  const double* vn;
+  const double* tab;
+  TabOff off;
+  double* Gu1;
+  double* Gu2;
+  double* Gvn;
+};
+
 hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s);
+hipError_t launch_adj_stage(int p, const AdjArgs& a, hipStream_t s);
+hipError_t launch_adj_sens(int p, const SensArgs& a, hipStream_t s);
+// J partials: part[b] = sum over the block of det_k d_k^T hatM d_k, d = c - cObs;
+// muT = 2 d
+hipError_t launch_loss(int p, int K, const double* B, const double* c, const double* cObs, const double* tab,
+                       const TabOff& off, double* part, double* muT, hipStream_t s);
+// out = det_k hatM x_k (the block mass matrix)
+hipError_t launch_mass(int p, int K, const double* B, const double* x, const double* tab, const TabOff& off,
+                       double* out, hipStream_t s);
+hipError_t launch_fill(size_t n, double v, double* x, hipStream_t s);
 hipError_t launch_vertex(const VertexArgs& a, hipStream_t s);
 hipError_t launch_limit(int p, int mode, const LimitArgs& a, hipStream_t s);
 hipError_t launch_to_taylor(int p, const ToTaylorArgs& a, hipStream_t s);

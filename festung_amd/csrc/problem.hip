@@ -808,6 +808,180 @@ int festung_set_tau_field(festung_problem* h, const double* tauK) {
   });
 }
 
+int festung_adj_rhs(festung_problem* h, const double* mu, double* out) {
+  return guard([&] {
+    check_handle(h);
+    if (!mu || !out) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    h->require_velocity();
+    DBuf dm, o;
+    h->upload_elem(dm, mu, h->N);
+    o.alloc(size_t(h->N) * h->K * sizeof(double));
+    AdjArgs a{};
+    a.m = h->mesh();
+    a.mu = dm.d();
+    a.u1 = h->u1.d();
+    a.u2 = h->u2.d();
+    a.vn = h->vn.d();
+    a.tab = h->tab.d();
+    a.off = h->off;
+    a.mode = 4;
+    a.out = o.d();
+    HIPCHK(launch_adj_stage(h->p, a, h->stream));
+    sync(h);
+    h->download_elem(o, o.d(), h->N, out);
+  });
+}
+
+int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double* cObs, int32_t ckpt, double* J,
+                    double* gradC0, double* gradU1, double* gradU2, double* gradVn, double* timing_ms) {
+  return guard([&] {
+    check_handle(h);
+    if (!cObs || nsteps < 1) fail(FESTUNG_ERR_ARG, "invalid argument");
+    h->set_device();
+    h->require_velocity();
+    h->finish_stage_sequence();
+    if (h->flags & FESTUNG_FLAG_LIMITER)
+      fail(FESTUNG_ERR_UNSUPPORTED, "the adjoint needs the limiter off (the limiter is not differentiable)");
+    if (h->ordRK > 3) fail(FESTUNG_ERR_UNSUPPORTED, "the adjoint supports the SSP orders 1..3");
+    if (h->hasRhs) fail(FESTUNG_ERR_UNSUPPORTED, "the adjoint assumes f = 0 and c_D = 0");
+    const int K = h->K, N = h->N, Re = h->tabs.Re, S = int(h->plan.size());
+    const size_t nk = size_t(N) * K * sizeof(double);
+    const int C = (ckpt <= 0 || ckpt > nsteps) ? nsteps : ckpt;
+    const int nseg = (nsteps + C - 1) / C;
+    std::vector<std::unique_ptr<DBuf>> ck(nseg + 1), sb(size_t(C) * S);
+    for (auto& b : ck) {
+      b.reset(new DBuf);
+      b->alloc(nk);
+    }
+    for (size_t j = 0; j < sb.size(); ++j)
+      if (j % S != 0) {  // stage 0 of local step 0 lives in the checkpoint
+        sb[j].reset(new DBuf);
+        sb[j]->alloc(nk);
+      } else if (j > 0) {
+        sb[j].reset(new DBuf);
+        sb[j]->alloc(nk);
+      }
+    DBuf dObs, part, muA, muB, mu1, mu2, L0, Gu1, Gu2, Gvn, tmp;
+    h->upload_elem(dObs, cObs, N);
+    part.alloc(size_t(h->nblocks) * sizeof(double));
+    for (DBuf* b : {&muA, &muB, &mu1, &mu2, &L0, &Gu1, &Gu2, &tmp}) b->alloc(nk);
+    Gvn.alloc(size_t(3) * Re * K * sizeof(double));
+    hipEvent_t ev[3];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    struct EvGuard {
+      hipEvent_t* e;
+      ~EvGuard() {
+        for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+      }
+    } evg{ev};
+
+    auto state = [&](int seg, int l, int len, int st) -> double* {
+      if (l == len) return ck[seg + 1]->d();
+      if (l == 0 && st == 0) return ck[seg]->d();
+      return sb[size_t(l) * S + st]->d();
+    };
+    auto run_segment = [&](int seg) {
+      const int len = std::min(C, nsteps - seg * C);
+      for (int l = 0; l < len; ++l)
+        for (int st = 0; st < S; ++st) {
+          StageArgs a{};
+          a.m = h->mesh();
+          a.c = state(seg, l, len, st);
+          a.c0 = state(seg, l, len, 0);
+          a.u1 = h->u1.d();
+          a.u2 = h->u2.d();
+          a.vn = h->vn.d();
+          a.tab = h->tab.d();
+          a.off = h->off;
+          a.tau = tau;
+          a.tauK = h->hasTauK ? h->tauK.d() : nullptr;
+          a.rk = h->plan[st].rk;
+          a.out = st + 1 < S ? state(seg, l, len, st + 1) : state(seg, l + 1, len, 0);
+          a.sink = h->sink();
+          HIPCHK(launch_stage(h->p, OUT_RK_DG, a, h->stream));
+        }
+    };
+
+    // forward, storing checkpoints (and, within the last segment, all stage states)
+    HIPCHK(hipEventRecord(ev[0], h->stream));
+    HIPCHK(hipMemcpyAsync(ck[0]->p, h->buf(h->cur), nk, hipMemcpyDeviceToDevice, h->stream));
+    for (int seg = 0; seg < nseg; ++seg) run_segment(seg);
+    HIPCHK(hipEventRecord(ev[1], h->stream));
+    // loss and terminal adjoint
+    HIPCHK(launch_loss(h->p, K, h->B.d(), ck[nseg]->d(), dObs.d(), h->tab.d(), h->off, part.d(), muA.d(), h->stream));
+    HIPCHK(launch_fill(size_t(N) * K, 0.0, Gu1.d(), h->stream));
+    HIPCHK(launch_fill(size_t(N) * K, 0.0, Gu2.d(), h->stream));
+    HIPCHK(launch_fill(size_t(3) * Re * K, 0.0, Gvn.d(), h->stream));
+    double* muCur = muA.d();
+    double* muNext = muB.d();
+    double* muS[4] = {nullptr, mu1.d(), mu2.d(), nullptr};
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+      const int len = std::min(C, nsteps - seg * C);
+      if (seg != nseg - 1) run_segment(seg);  // recompute the segment's stage states
+      for (int l = len - 1; l >= 0; --l) {
+        for (int st = S - 1; st >= 0; --st) {
+          AdjArgs a{};
+          a.m = h->mesh();
+          a.mu = st + 1 == S ? muCur : muS[st + 1];
+          a.u1 = h->u1.d();
+          a.u2 = h->u2.d();
+          a.vn = h->vn.d();
+          a.tab = h->tab.d();
+          a.off = h->off;
+          a.tau = tau;
+          a.tauK = h->hasTauK ? h->tauK.d() : nullptr;
+          a.omega = h->plan[st].rk.a0;
+          a.L0 = L0.d();
+          a.mode = S == 1 ? 3 : (st == S - 1 ? 0 : (st > 0 ? 1 : 2));
+          a.out = st > 0 ? muS[st] : muNext;
+          HIPCHK(launch_adj_stage(h->p, a, h->stream));
+        }
+        SensArgs sa{};
+        sa.m = h->mesh();
+        sa.S = S;
+        for (int st = 0; st < S; ++st) {
+          sa.mu[st] = st + 1 == S ? muCur : muS[st + 1];
+          sa.c[st] = state(seg, l, len, st);
+          sa.w[st] = 1.0 - h->plan[st].rk.a0;
+        }
+        sa.tau = tau;
+        sa.tauK = h->hasTauK ? h->tauK.d() : nullptr;
+        sa.vn = h->vn.d();
+        sa.tab = h->tab.d();
+        sa.off = h->off;
+        sa.Gu1 = Gu1.d();
+        sa.Gu2 = Gu2.d();
+        sa.Gvn = Gvn.d();
+        HIPCHK(launch_adj_sens(h->p, sa, h->stream));
+        std::swap(muCur, muNext);
+      }
+    }
+    HIPCHK(launch_mass(h->p, K, h->B.d(), muCur, h->tab.d(), h->off, tmp.d(), h->stream));
+    HIPCHK(hipEventRecord(ev[2], h->stream));
+    // final state becomes the problem's state
+    HIPCHK(hipMemcpyAsync(h->X.p, ck[nseg]->p, nk, hipMemcpyDeviceToDevice, h->stream));
+    h->cur = 0;
+    sync(h);
+    std::vector<double> hp(h->nblocks);
+    HIPCHK(hipMemcpy(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double j = 0;
+    for (double v : hp) j += v;
+    if (J) *J = j;
+    if (gradC0) h->download_elem(tmp, tmp.d(), N, gradC0);
+    if (gradU1) h->download_elem(Gu1, Gu1.d(), N, gradU1);
+    if (gradU2) h->download_elem(Gu2, Gu2.d(), N, gradU2);
+    if (gradVn) h->download_elem(Gvn, Gvn.d(), 3 * Re, gradVn);
+    if (timing_ms) {
+      float f = 0, b = 0;
+      HIPCHK(hipEventElapsedTime(&f, ev[0], ev[1]));
+      HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+      timing_ms[0] = f;
+      timing_ms[1] = b;
+    }
+  });
+}
+
 int festung_profile(festung_problem* h, int enable) {
   return guard([&] {
     check_handle(h);

@@ -198,6 +198,25 @@ int festung_batch_step(festung_batch* b, int32_t nsteps, const double* tau);
 int festung_batch_sync(festung_batch* b);
 void festung_batch_destroy(festung_batch* b);
 
+/* Discrete adjoint (limiter off, f = 0, c_D = 0, SSP orders 1..3).
+ * Runs nsteps steps of size tau from the current state (which becomes the
+ * final state), the loss J = (c(T) - cObs)^T globM (c(T) - cObs) (M-weighted
+ * squared L2 misfit; cObs K x N), and the reverse sweep through the
+ * transposed element and edge operators and the reverse RK stages.  Outputs
+ * (any may be NULL): J, dJ/dc0 (K x N), dJ/du1Disc, dJ/du2Disc (K x N) and
+ * dJ/dvNormalOnQuadEdge (K x 3 x Re) -- the velocity enters only through
+ * these fields, so a caller's parameter gradient is their contraction with
+ * the fields' derivatives.  Stage states are kept in HBM: checkpoints every
+ * ckptSteps steps (<= 0: all steps stored, no recomputation), each segment
+ * recomputed once in the reverse sweep.  timing_ms (NULL or 2 values):
+ * forward and reverse device time.  No counterpart in the reference. */
+int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double* cObs, int32_t ckptSteps,
+                    double* J, double* gradC0, double* gradU1, double* gradU2, double* gradVn,
+                    double* timing_ms);
+/* out = M^-1 A^T mu for the current velocity (K x N): the transposed
+ * right-hand-side operator, for dot-product tests of the adjoint. */
+int festung_adj_rhs(festung_problem* h, const double* mu, double* out);
+
 /* Apply the slope limiter to the current state.
  * Replaces core/applySlopeLimiterDisc.m:77-86 ('linear'). */
 int festung_limit(festung_problem* h);

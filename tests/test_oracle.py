@@ -491,3 +491,43 @@ def test_convergence_order_transient():
             errs.append(d.error)
         eoc = np.log(np.array(errs[:-1]) / np.array(errs[1:])) / np.log(2)
         assert np.all(eoc > lo), eoc
+
+
+# ---------------------------------------------------------------- adjoint --
+def test_oracle_adjoint_vs_central_differences():
+    """The oracle's reverse sweep (transposed sparse operators, frozen upwind
+    selection) against central differences of its own forward map."""
+    from oracle import adjoint as OAd
+    pd = _setup(2, h=1 / 4)
+    modes_kl = [(1, 1), (2, 1)]
+
+    def mode(k, l):
+        s = 1.0 / (np.pi * np.sqrt(k * k + l * l))
+        return (lambda t, x1, x2: s * np.sin(k * np.pi * x1) * l * np.pi * np.cos(l * np.pi * x2),
+                lambda t, x1, x2: -s * k * np.pi * np.cos(k * np.pi * x1) * np.sin(l * np.pi * x2))
+
+    def fields(u1, u2):
+        d1 = A.projectFuncCont2DataDisc(pd.g, lambda x1, x2: u1(0, x1, x2), 4, pd.hatM, pd.basesOnQuad)
+        d2 = A.projectFuncCont2DataDisc(pd.g, lambda x1, x2: u2(0, x1, x2), 4, pd.hatM, pd.basesOnQuad)
+        vn = A.computeFuncContNuOnQuadEdge(pd.g, lambda x1, x2: u1(0, x1, x2), lambda x1, x2: u2(0, x1, x2), 5)
+        return d1, d2, vn
+    base = fields(lambda t, x1, x2: 0.5 - x2, lambda t, x1, x2: x1 - 0.5)
+    mf = [fields(*mode(k, l)) for (k, l) in modes_kl]
+
+    def at(a):
+        return tuple(base[i] + sum(a[m] * mf[m][i] for m in range(len(a))) for i in range(3))
+    rng = np.random.default_rng(0)
+    c0 = rng.standard_normal((pd.K, pd.N))
+    cObs = rng.standard_normal((pd.K, pd.N))
+    a = np.array([0.03, -0.02])
+    pd.ordRK = 3
+    tau, nsteps = 0.01, 4
+    J, g0, ga = OAd.gradient(pd, *at(a), mf, c0, nsteps, tau, cObs)
+    h = 1e-6
+    d0 = rng.standard_normal(c0.shape)
+    da = rng.standard_normal(2)
+    Jp = OAd.gradient(pd, *at(a + h * da), [], c0 + h * d0, nsteps, tau, cObs)[0]
+    Jm = OAd.gradient(pd, *at(a - h * da), [], c0 - h * d0, nsteps, tau, cObs)[0]
+    fd = (Jp - Jm) / (2 * h)
+    an = np.sum(g0 * d0) + np.sum(ga * da)
+    assert abs(fd - an) <= 1e-7 * abs(an)
