@@ -116,9 +116,69 @@ def run_batched(nprob, nsteps, seed0=4000):
     return t_batched, t_single, upd, setup, K_tot
 
 
+def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6):
+    """Gradient of J = ||c(T) - c_obs||^2_M w.r.t. c0 and nmodes stream-function
+    coefficients: cost against the plain forward (graph) run, and central
+    differences (step h) along ndir random directions."""
+    import numpy as np
+    import torch
+    from festung_amd import adjoint as AD
+    from festung_amd.advection import testcases as TC
+    modes = AD.stream_modes()[:nmodes]
+    m = AD.StreamModel(hmax, p, modes=modes, c0=TC.gaussian_humps(51))
+    a_true = 0.05 * np.random.default_rng(52).standard_normal(nmodes)
+    a = a_true + 0.005 * np.random.default_rng(53).standard_normal(nmodes)
+    dim = int(np.ceil(1 / hmax))
+    tau = 0.1 / (dim * 0.8 * (2 * p + 1))
+    cObs = m.run(m.c0, a_true, nsteps, tau)
+    # plain forward (captured graph), timed with events on the problem's stream
+    stream = torch.cuda.ExternalStream(m.prob.stream)
+    m.set_velocity(a)
+    m.prob.set_state(m.c0)
+    m.prob.step(nsteps, tau)
+    m.prob.sync()
+    m.prob.set_state(m.c0)
+    e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    m.prob.step(nsteps, tau)
+    e1.record(stream)
+    m.prob.sync()
+    fwd_ms = e0.elapsed_time(e1)
+    out = {'forward_ms': fwd_ms}
+    for c in (ckpt, 0):
+        t0 = time.perf_counter()
+        J, g0, ga, r = m.gradient(m.c0, a, nsteps, tau, cObs, ckpt=c)
+        wall = time.perf_counter() - t0
+        key = 'ckpt%d' % c if c else 'all_states'
+        out[key] = {'adjoint_forward_ms': r['forward_ms'], 'adjoint_reverse_ms': r['backward_ms'],
+                    'cost_vs_forward': (r['forward_ms'] + r['backward_ms']) / fwd_ms, 'wall_s': wall}
+    # the other sweep variants, same checkpoint interval
+    for serial, kern in ((True, 'fused'), (False, 'fused'), (True, 'split'), (False, 'split')):
+        mf = AD.StreamModel(hmax, p, modes=modes, c0=TC.gaussian_humps(51), adj_serial=serial, adj_kernel=kern)
+        mf.gradient(mf.c0, a, 2, tau, cObs, ckpt=1)  # warm-up
+        r = mf.gradient(mf.c0, a, nsteps, tau, cObs, ckpt=ckpt)[3]
+        out['ckpt%d_%s_%s' % (ckpt, kern, 'serial' if serial else 'overlap')] = {
+            'adjoint_forward_ms': r['forward_ms'], 'adjoint_reverse_ms': r['backward_ms'],
+            'cost_vs_forward': (r['forward_ms'] + r['backward_ms']) / fwd_ms}
+        del mf
+    rng = np.random.default_rng(54)
+    errs = []
+    for _ in range(ndir):
+        d0 = rng.standard_normal(g0.shape) * np.abs(m.c0).max()
+        da = rng.standard_normal(ga.shape) * 0.05
+        Jp = m.misfit(m.c0 + h * d0, a + h * da, nsteps, tau, cObs)
+        Jm = m.misfit(m.c0 - h * d0, a - h * da, nsteps, tau, cObs)
+        fd = (Jp - Jm) / (2 * h)
+        an = float(np.sum(g0 * d0) + np.sum(ga * da))
+        errs.append(abs(fd - an) / abs(an))
+    out.update(J=J, K=m.prob.K, fd_max_rel_err=max(errs), fd_directions=ndir, fd_h=h, tau=tau)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument('--workload', choices=('step', 'batched'), default='step')
+    ap.add_argument('--workload', choices=('step', 'batched', 'adjoint'), default='step')
+    ap.add_argument('--ckpt', type=int, default=20)
     ap.add_argument('--nprob', type=int, default=512)
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=100)
@@ -134,7 +194,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print('bench.py measures one GPU only: the path runs on a single MI355X; '
-              'multi-GPU sharding is described in DESIGN.md but not run.', file=sys.stderr)
+              'multi-GPU sharding is not run.', file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -142,6 +202,21 @@ def main():
     if not torch.cuda.is_available():
         print('no GPU visible', file=sys.stderr)
         sys.exit(1)
+    if a.workload == 'adjoint':
+        nsteps = a.steps if a.steps != 100 else 400
+        hm = a.hmax if a.hmax != 2.0 ** -9 else 2.0 ** -7
+        r = run_adjoint(hm, a.p, nsteps, a.ckpt)
+        print(json.dumps({
+            'metric': 'adjoint gradient cost / forward cost (and FD agreement)',
+            'value': r['ckpt%d' % a.ckpt]['cost_vs_forward'] if a.ckpt else r['all_states']['cost_vs_forward'],
+            'unit': 'x forward', 'n_gpus': 1, 'steps': nsteps, 'warmup': 1,
+            'ms_per_step': r['forward_ms'] / nsteps, 'higher_is_better': False, 'scaling': 'weak',
+            'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
+            'config': {'workload': 'adjoint: domainSquare hmax=2^%g K=%d p=%d SSP-RK3 %d steps, limiter off, '
+                                   'J=||c(T)-c_obs||^2_M, grad wrt c0 and 32 stream-function coefficients, '
+                                   'checkpoint every %d steps' % (np.log2(hm), r['K'], a.p, nsteps, a.ckpt)},
+            'adjoint': r}))
+        return
     if a.workload == 'batched':
         nsteps = a.steps if a.steps != 100 else 50
         tb, ts, upd, setup, Ktot = run_batched(a.nprob, nsteps)

@@ -15,6 +15,9 @@ LIB_PATH = os.path.join(HERE, 'libfestung_hip.so')
 OK, ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_STATE, ERR_UNSUPPORTED, ERR_INTERNAL = range(7)
 FLAG_LIMITER = 1
 FLAG_GRAPH = 2
+FLAG_ADJ_SERIAL = 4
+FLAG_ADJ_FUSED = 8
+FLAG_ADJ_SPLIT = 16
 
 
 class FestungError(RuntimeError):
@@ -68,6 +71,9 @@ SIGNATURES = {
     'festung_limiter_info': (C.c_int, [vp, dp, dp, dp]),
     'festung_l2_error': (C.c_int, [vp, C.c_int, dp, dp]),
     'festung_set_tau_field': (C.c_int, [vp, dp]),
+    'festung_set_elem_block': (C.c_int, [vp, C.c_int32]),
+    'festung_get_elem_block': (C.c_int, [vp, C.POINTER(C.c_int32)]),
+    'festung_autotune': (C.c_int, [vp, C.c_double, C.c_int32, C.POINTER(C.c_int32), dp]),
     'festung_batch_create': (C.c_int, [C.POINTER(vp), C.c_int32, C.POINTER(vp)]),
     'festung_batch_step': (C.c_int, [vp, C.c_int32, dp]),
     'festung_batch_sync': (C.c_int, [vp]),

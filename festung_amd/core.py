@@ -144,8 +144,15 @@ def quadRule2D(qOrd):
 class Problem:
     """Device-resident advection problem (festung_problem_* of the C-ABI)."""
 
-    def __init__(self, grid, p, ordRK, limiter=False, graph=False):
-        flags = (L.FLAG_LIMITER if limiter else 0) | (L.FLAG_GRAPH if graph else 0)
+    def __init__(self, grid, p, ordRK, limiter=False, graph=False, adj_serial=False, adj_kernel=None):
+        """adj_serial: recompute checkpointed segments on the problem's own
+        stream; adj_kernel: None (by mesh size), 'fused' or 'split' -- the
+        transposed stage kernel of the adjoint."""
+        if adj_kernel not in (None, 'fused', 'split'):
+            raise ValueError('adj_kernel must be None, "fused" or "split"')
+        flags = ((L.FLAG_LIMITER if limiter else 0) | (L.FLAG_GRAPH if graph else 0)
+                 | (L.FLAG_ADJ_SERIAL if adj_serial else 0)
+                 | {None: 0, 'fused': L.FLAG_ADJ_FUSED, 'split': L.FLAG_ADJ_SPLIT}[adj_kernel])
         h = C.c_void_p()
         check(lib().festung_problem_create(grid._h, int(p), int(ordRK), flags, C.byref(h)))
         self._h = h
@@ -192,6 +199,26 @@ class Problem:
     def set_tau_field(self, tauK):
         t = None if tauK is None else fortran(tauK, (self.K,))
         check(lib().festung_set_tau_field(self._h, ptr(t)))
+
+    @property
+    def elem_block(self):
+        """Elements per workgroup of the element kernels (64, 128 or 256)."""
+        v = C.c_int32()
+        check(lib().festung_get_elem_block(self._h, C.byref(v)))
+        return v.value
+
+    @elem_block.setter
+    def elem_block(self, bs):
+        check(lib().festung_set_elem_block(self._h, int(bs)))
+
+    def autotune(self, tau, nsteps=10):
+        """Time nsteps steps with 64, 128 and 256 elements per workgroup,
+        keep the fastest; the state is left as it was.  Returns (best,
+        {elements per workgroup: ms per step})."""
+        best = C.c_int32()
+        ms = np.zeros(3)
+        check(lib().festung_autotune(self._h, float(tau), int(nsteps), C.byref(best), ptr(ms)))
+        return best.value, dict(zip((64, 128, 256), ms.tolist()))
 
     def set_vertex_dirichlet(self, vertexD):
         v = None if vertexD is None else fortran(vertexD, (self.grid.numV,))

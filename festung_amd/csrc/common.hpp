@@ -48,6 +48,13 @@ int guard(F&& f) {
   }
 }
 
+const char* last_error();
+
+// Re-raise the status of a nested C-ABI call made inside guard().
+inline void check_call(int rc) {
+  if (rc != FESTUNG_OK) throw Error(rc, last_error());
+}
+
 inline int nbasis(int p) { return (p + 1) * (p + 2) / 2; }
 
 }  // namespace festung

@@ -42,7 +42,7 @@ __device__ __forceinline__ int logical_block() {
   const int full = nb - nb % kXCDs;
   return b < full ? (b % kXCDs) * (full / kXCDs) + b / kXCDs : b;
 }
-__device__ __forceinline__ int gtid() { return logical_block() * BLOCK + threadIdx.x; }
+__device__ __forceinline__ int gtid() { return logical_block() * int(blockDim.x) + threadIdx.x; }
 
 // ---------------------------------------------------------------- Taylor --
 // E_D(A): the degree-D symmetric power of a 2x2 matrix A acting on
@@ -199,8 +199,7 @@ __device__ __forceinline__ void minmax_block_partial(double mn, double mx, const
   __syncthreads();
   if (threadIdx.x == 0) {
     double a = smn[0], b = smx[0];
-#pragma unroll
-    for (int i = 1; i < BLOCK / 64; ++i) {
+    for (int i = 1; i < int(blockDim.x) / 64; ++i) {
       a = fmin(a, smn[i]);
       b = fmax(b, smx[i]);
     }
@@ -242,7 +241,7 @@ template <int P, int MODE>
 __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
   __shared__ double sTheta[Re * N * 9];
-  for (int i = threadIdx.x; i < Re * N * 9; i += BLOCK) sTheta[i] = a.tab[a.off.theta + i];
+  for (int i = threadIdx.x; i < Re * N * 9; i += blockDim.x) sTheta[i] = a.tab[a.off.theta + i];
   __syncthreads();
 
   const int K = a.m.K;
@@ -718,17 +717,14 @@ __global__ __launch_bounds__(BLOCK) void l2_kernel(int K, const double* __restri
 // gathered from the neighbour's side: its normal velocity, its upwind
 // selector and its trace, with the neighbour-trace table indexed by
 // (neighbour's local edge, own local edge).  No atomics.
-template <int P>
-__global__ __launch_bounds__(BLOCK) void adj_stage_kernel(AdjArgs a) {
+// W = ADJ_STAGE: transposed stage only; ADJ_FUSED: with the stage's velocity
+// sensitivities; ADJ_SENS: the sensitivities only.
+// UNR: unroll the edge loop (more loads in flight per thread, more registers).
+template <int P, int W, bool UNR = false>
+__device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* sTheta, const double* sPhi1D) {
+  constexpr bool SENS = W != ADJ_STAGE, ADJ = W != ADJ_SENS;
   constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
-  __shared__ double sTheta[Re * N * 9];
-  __shared__ double sPhi1D[Re * N * 3];
-  for (int i = threadIdx.x; i < Re * N * 9; i += BLOCK) sTheta[i] = a.tab[a.off.theta + i];
-  for (int i = threadIdx.x; i < Re * N * 3; i += BLOCK) sPhi1D[i] = a.tab[a.off.phi1D + i];
-  __syncthreads();
   const int K = a.m.K;
-  const int k = gtid();
-  if (k >= K) return;
   const double* __restrict__ T = a.tab;
   double mu[N], u1[N], u2[N], acc[N];
 #pragma unroll
@@ -740,11 +736,23 @@ __global__ __launch_bounds__(BLOCK) void adj_stage_kernel(AdjArgs a) {
   }
   Geo<P> g;
   g.load(a.m.B, k, K);
+  const double tk = a.tauK ? a.tauK[k] : a.tau;
+  const double al = (1.0 - a.omega) * tk;   // weight of this stage's sensitivity
+  double c[SENS ? N : 1], gu1[SENS ? N : 1], gu2[SENS ? N : 1], gvn[SENS ? 3 * Re : 1];
+  if constexpr (SENS) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      c[i] = a.c[i * K + k];
+      gu1[i] = gu2[i] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 3 * Re; ++i) gvn[i] = 0.0;
+  }
   if constexpr (P >= 1) {
     constexpr int UQ = RE <= 16 ? RE : 4;
 #pragma unroll UQ
     for (int q = 0; q < RE; ++q) {
-      double m1 = 0.0, m2 = 0.0, u1q = 0.0, u2q = 0.0;
+      double m1 = 0.0, m2 = 0.0, u1q = 0.0, u2q = 0.0, cq = 0.0;
 #pragma unroll
       for (int j = 0; j < N; ++j) {
         const double ph = T[a.off.phiE + q + j * RE];
@@ -752,36 +760,65 @@ __global__ __launch_bounds__(BLOCK) void adj_stage_kernel(AdjArgs a) {
         m2 += mu[j] * T[a.off.gradE + q + j * RE + RE * N];
         u1q += u1[j] * ph;
         u2q += u2[j] * ph;
+        if constexpr (SENS) cq += c[j] * ph;
       }
-      const double f = T[a.off.wE + q] *
-                       ((g.B22 * u1q - g.B12 * u2q) * m1 + (g.B11 * u2q - g.B21 * u1q) * m2);
+      const double wq = T[a.off.wE + q];
+      if constexpr (ADJ) {
+        const double f = wq * ((g.B22 * u1q - g.B12 * u2q) * m1 + (g.B11 * u2q - g.B21 * u1q) * m2);
 #pragma unroll
-      for (int j = 0; j < N; ++j) acc[j] += T[a.off.phiE + q + j * RE] * f;
+        for (int j = 0; j < N; ++j) acc[j] += T[a.off.phiE + q + j * RE] * f;
+      }
+      if constexpr (SENS) {
+        const double wc = al * wq * cq;
+        const double f1 = wc * (g.B22 * m1 - g.B21 * m2);
+        const double f2 = wc * (g.B11 * m2 - g.B12 * m1);
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          gu1[l] += T[a.off.phiE + q + l * RE] * f1;
+          gu2[l] += T[a.off.phiE + q + l * RE] * f2;
+        }
+      }
     }
   }
   const int fl = a.m.flags[k];
-#pragma unroll 1
+#pragma unroll(UNR ? 3 : 1)
   for (int n = 0; n < 3; ++n) {
     const double area = a.m.areaE[n * K + k];
-    // diagonal block (symmetric): own outflow
-    if (flag_out(fl, n)) {
+    const int nb = a.m.nbr[n * K + k];
+    const int np = flag_np(fl, n);  // neighbour's local edge
+    const bool outOn = flag_out(fl, n);
+    // own edge points: symmetric outflow block, and the sensitivities
 #pragma unroll
-      for (int r = 0; r < Re; ++r) {
-        const double v = a.vn[(r * 3 + n) * K + k];
-        if (v > 0.0) {
-          double t = 0.0;
+    for (int r = 0; r < Re; ++r) {
+      const double v = a.vn[(r * 3 + n) * K + k];
+      const bool up = v > 0.0 && outOn;
+      const bool in = v < 0.0 && nb >= 0;
+      if ((ADJ && up) || (SENS && (up || in))) {
+        double t = 0.0;
 #pragma unroll
-          for (int i = 0; i < N; ++i) t += mu[i] * T[a.off.phi1D + r + i * Re + n * Re * N];
-          const double fx = area * v * T[a.off.wEdge + r] * t;
+        for (int i = 0; i < N; ++i) t += mu[i] * T[a.off.phi1D + r + i * Re + n * Re * N];
+        const double wr = T[a.off.wEdge + r];
+        if (ADJ && up) {
+          const double fx = area * v * wr * t;
 #pragma unroll
           for (int j = 0; j < N; ++j) acc[j] -= T[a.off.phi1D + r + j * Re + n * Re * N] * fx;
+        }
+        if constexpr (SENS) {
+          double cu = 0.0;
+          if (up) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) cu += c[j] * T[a.off.phi1D + r + j * Re + n * Re * N];
+          } else {
+            const double* th = sTheta + r + n * Re * N + np * Re * N * 3;
+#pragma unroll
+            for (int j = 0; j < N; ++j) cu += a.c[j * K + nb] * th[j * Re];
+          }
+          gvn[r * 3 + n] -= al * area * wr * t * cu;
         }
       }
     }
     // transposed off-diagonal block: the neighbour's inflow through this edge
-    const int nb = a.m.nbr[n * K + k];
-    if (nb >= 0) {
-      const int np = flag_np(fl, n);  // neighbour's local edge
+    if (ADJ && nb >= 0) {
       double mn[N];
 #pragma unroll
       for (int i = 0; i < N; ++i) mn[i] = a.mu[i * K + nb];
@@ -800,6 +837,16 @@ __global__ __launch_bounds__(BLOCK) void adj_stage_kernel(AdjArgs a) {
       }
     }
   }
+  if constexpr (SENS) {
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      a.Gu1[l * K + k] += gu1[l];
+      a.Gu2[l * K + k] += gu2[l];
+    }
+#pragma unroll
+    for (int i = 0; i < 3 * Re; ++i) a.Gvn[i * K + k] += gvn[i];
+  }
+  if constexpr (!ADJ) return;
   double z[N];
   const double idet = 1.0 / g.det;
 #pragma unroll
@@ -814,111 +861,65 @@ __global__ __launch_bounds__(BLOCK) void adj_stage_kernel(AdjArgs a) {
     for (int i = 0; i < N; ++i) a.out[i * K + k] = z[i];
     return;
   }
-  const double tk = a.tauK ? a.tauK[k] : a.tau;
   const double w1 = 1.0 - a.omega;
+  if (a.mode == 0) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const double y = w1 * (mu[i] + tk * z[i]);
-    const double om = a.omega * mu[i];
-    if (a.mode == 0) {
-      a.out[i * K + k] = y;
-      a.L0[i * K + k] = om;
-    } else if (a.mode == 1) {
-      a.out[i * K + k] = y;
-      a.L0[i * K + k] += om;
-    } else if (a.mode == 2) {
-      a.out[i * K + k] = a.L0[i * K + k] + om + y;
-    } else {
-      a.out[i * K + k] = om + y;
+    for (int i = 0; i < N; ++i) {
+      a.out[i * K + k] = w1 * (mu[i] + tk * z[i]);
+      a.L0[i * K + k] = a.omega * mu[i];
     }
+  } else if (a.mode == 1) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      a.out[i * K + k] = w1 * (mu[i] + tk * z[i]);
+      a.L0[i * K + k] += a.omega * mu[i];
+    }
+  } else if (a.mode == 2) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) a.out[i * K + k] = a.L0[i * K + k] + a.omega * mu[i] + w1 * (mu[i] + tk * z[i]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) a.out[i * K + k] = a.omega * mu[i] + w1 * (mu[i] + tk * z[i]);
   }
 }
 
 template <int P>
-__global__ __launch_bounds__(BLOCK) void adj_sens_kernel(SensArgs a) {
-  constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
-  __shared__ double sTheta[Re * N * 9];
-  for (int i = threadIdx.x; i < Re * N * 9; i += BLOCK) sTheta[i] = a.tab[a.off.theta + i];
+__device__ __forceinline__ void adj_tables(const AdjArgs& a, double* sTheta, double* sPhi1D) {
+  constexpr int N = Dims<P>::N, Re = Dims<P>::Re;
+  for (int i = threadIdx.x; i < Re * N * 9; i += blockDim.x) sTheta[i] = a.tab[a.off.theta + i];
+  for (int i = threadIdx.x; i < Re * N * 3; i += blockDim.x) sPhi1D[i] = a.tab[a.off.phi1D + i];
   __syncthreads();
-  const int K = a.m.K;
+}
+
+// One thread per element.
+template <int P, int W>
+__global__ __launch_bounds__(BLOCK) void adj_stage_kernel(AdjArgs a) {
+  constexpr int N = Dims<P>::N, Re = Dims<P>::Re;
+  __shared__ double sTheta[Re * N * 9];
+  __shared__ double sPhi1D[Re * N * 3];
+  adj_tables<P>(a, sTheta, sPhi1D);
   const int k = gtid();
-  if (k >= K) return;
-  const double* __restrict__ T = a.tab;
-  Geo<P> g;
-  g.load(a.m.B, k, K);
-  const double tk = a.tauK ? a.tauK[k] : a.tau;
-  const int fl = a.m.flags[k];
-  double gu1[N], gu2[N], gvn[3 * Re];
-#pragma unroll
-  for (int i = 0; i < N; ++i) gu1[i] = gu2[i] = 0.0;
-#pragma unroll
-  for (int i = 0; i < 3 * Re; ++i) gvn[i] = 0.0;
-  for (int s = 0; s < a.S; ++s) {
-    const double al = a.w[s] * tk;
-    double mu[N], c[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      mu[i] = a.mu[s][i * K + k];
-      c[i] = a.c[s][i * K + k];
-    }
-    if constexpr (P >= 1) {
-      constexpr int UQ = RE <= 16 ? RE : 4;
-#pragma unroll UQ
-      for (int q = 0; q < RE; ++q) {
-        double cq = 0.0, m1 = 0.0, m2 = 0.0;
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-          cq += c[j] * T[a.off.phiE + q + j * RE];
-          m1 += mu[j] * T[a.off.gradE + q + j * RE];
-          m2 += mu[j] * T[a.off.gradE + q + j * RE + RE * N];
-        }
-        const double wc = al * T[a.off.wE + q] * cq;
-        const double f1 = wc * (g.B22 * m1 - g.B21 * m2);
-        const double f2 = wc * (g.B11 * m2 - g.B12 * m1);
-#pragma unroll
-        for (int l = 0; l < N; ++l) {
-          gu1[l] += T[a.off.phiE + q + l * RE] * f1;
-          gu2[l] += T[a.off.phiE + q + l * RE] * f2;
-        }
-      }
-    }
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {
-      const double area = a.m.areaE[n * K + k];
-      const int nb = a.m.nbr[n * K + k];
-      const int np = flag_np(fl, n);
-      const bool outOn = flag_out(fl, n);
-#pragma unroll
-      for (int r = 0; r < Re; ++r) {
-        const double v = a.vn[(r * 3 + n) * K + k];
-        double cu = 0.0;
-        bool on = false;
-        if (v > 0.0 && outOn) {
-          on = true;
-#pragma unroll
-          for (int j = 0; j < N; ++j) cu += c[j] * T[a.off.phi1D + r + j * Re + n * Re * N];
-        } else if (v < 0.0 && nb >= 0) {
-          on = true;
-          const double* th = sTheta + r + n * Re * N + np * Re * N * 3;
-#pragma unroll
-          for (int j = 0; j < N; ++j) cu += a.c[s][j * K + nb] * th[j * Re];
-        }
-        if (on) {
-          double mr = 0.0;
-#pragma unroll
-          for (int i = 0; i < N; ++i) mr += mu[i] * T[a.off.phi1D + r + i * Re + n * Re * N];
-          gvn[r * 3 + n] -= al * area * T[a.off.wEdge + r] * mr * cu;
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int l = 0; l < N; ++l) {
-    a.Gu1[l * K + k] += gu1[l];
-    a.Gu2[l * K + k] += gu2[l];
-  }
-#pragma unroll
-  for (int i = 0; i < 3 * Re; ++i) a.Gvn[i * K + k] += gvn[i];
+  if (k >= a.m.K) return;
+  adj_body<P, W>(a, k, sTheta, sPhi1D);
+}
+
+// Two threads per element: the first half of the workgroup applies the
+// transposed stage to E elements, the second half accumulates the same
+// elements' velocity sensitivities.  The halves share only read-only inputs,
+// so they need no synchronisation; on small meshes this doubles the waves in
+// flight.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void adj_split_kernel(AdjArgs a) {
+  constexpr int N = Dims<P>::N, Re = Dims<P>::Re;
+  __shared__ double sTheta[Re * N * 9];
+  __shared__ double sPhi1D[Re * N * 3];
+  adj_tables<P>(a, sTheta, sPhi1D);
+  const int E = int(blockDim.x) / 2;
+  const int t = threadIdx.x;
+  const int k = logical_block() * E + (t < E ? t : t - E);
+  if (k >= a.m.K) return;
+  if (t < E) adj_body<P, ADJ_STAGE, true>(a, k, sTheta, sPhi1D);
+  else adj_body<P, ADJ_SENS, true>(a, k, sTheta, sPhi1D);
 }
 
 template <int P>
@@ -1002,13 +1003,17 @@ __global__ __launch_bounds__(BLOCK) void fill_kernel(size_t n, double v, double*
   }
 
 static inline dim3 grid_for(int n) { return dim3((n + BLOCK - 1) / BLOCK); }
+static inline dim3 grid_for(int n, int bs) { return dim3((n + bs - 1) / bs); }
+bool valid_elem_block(int bs) { return bs == 64 || bs == 128 || bs == 256; }
 
 hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s) {
-  dim3 gr = grid_for(a.m.K);
+  const int BS = a.m.bs;
+  if (!valid_elem_block(BS)) return hipErrorInvalidValue;
+  dim3 gr = grid_for(a.m.K, BS);
 #define L_STAGE(MODE_)                                                                  \
-  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((stage_kernel<PP, MODE_>), gr, dim3(BLOCK), 0, s, a))
+  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((stage_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
 #define L_STAGE1(MODE_)                                                                 \
-  FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((stage_kernel<PP, MODE_>), gr, dim3(BLOCK), 0, s, a))
+  FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((stage_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
   switch (mode) {
     case OUT_RK_DG: L_STAGE(OUT_RK_DG); break;
     case OUT_RK_TAYLOR: L_STAGE1(OUT_RK_TAYLOR); break;
@@ -1027,9 +1032,10 @@ hipError_t launch_vertex(const VertexArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_limit(int p, int mode, const LimitArgs& a, hipStream_t s) {
-  if (p < 1) return hipErrorInvalidValue;
-  dim3 gr = grid_for(a.m.K);
-#define L_LIM(MODE_) FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((limit_kernel<PP, MODE_>), gr, dim3(BLOCK), 0, s, a))
+  const int BS = a.m.bs;
+  if (p < 1 || !valid_elem_block(BS)) return hipErrorInvalidValue;
+  dim3 gr = grid_for(a.m.K, BS);
+#define L_LIM(MODE_) FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((limit_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
   switch (mode) {
     case LIM_SOLUTION: L_LIM(LIM_SOLUTION); break;
     case LIM_DOT_RK_DG: L_LIM(LIM_DOT_RK_DG); break;
@@ -1041,8 +1047,10 @@ hipError_t launch_limit(int p, int mode, const LimitArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_to_taylor(int p, const ToTaylorArgs& a, hipStream_t s) {
-  dim3 gr = grid_for(a.K);
-  FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((to_taylor_kernel<PP>), gr, dim3(BLOCK), 0, s, a));
+  const int BS = a.bs;
+  if (!valid_elem_block(BS)) return hipErrorInvalidValue;
+  dim3 gr = grid_for(a.K, BS);
+  FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((to_taylor_kernel<PP>), gr, dim3(BS), 0, s, a));
   return hipGetLastError();
 }
 
@@ -1067,8 +1075,10 @@ hipError_t launch_vnormal(int K, int Re, const double* nuE0T, const double* u1e,
 
 hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* cDe, const double* vn,
                          const double* tab, const TabOff& off, double* rhs, hipStream_t s) {
-  dim3 gr = grid_for(m.K);
-  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((source_kernel<PP>), gr, dim3(BLOCK), 0, s, m, fq, cDe, vn, tab, off, rhs));
+  const int BS = m.bs;
+  if (!valid_elem_block(BS)) return hipErrorInvalidValue;
+  dim3 gr = grid_for(m.K, BS);
+  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((source_kernel<PP>), gr, dim3(BS), 0, s, m, fq, cDe, vn, tab, off, rhs));
   return hipGetLastError();
 }
 
@@ -1100,14 +1110,20 @@ hipError_t launch_permute(int K, int cols, const double* src, const int* idx, do
 }
 
 hipError_t launch_adj_stage(int p, const AdjArgs& a, hipStream_t s) {
-  dim3 gr = grid_for(a.m.K);
-  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_stage_kernel<PP>), gr, dim3(BLOCK), 0, s, a));
-  return hipGetLastError();
-}
-
-hipError_t launch_adj_sens(int p, const SensArgs& a, hipStream_t s) {
-  dim3 gr = grid_for(a.m.K);
-  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_sens_kernel<PP>), gr, dim3(BLOCK), 0, s, a));
+  const int BS = a.m.bs;
+  if (!valid_elem_block(BS)) return hipErrorInvalidValue;
+  dim3 gr = grid_for(a.m.K, BS);
+  switch (a.what) {
+    case ADJ_STAGE: FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_stage_kernel<PP, ADJ_STAGE>), gr, dim3(BS), 0, s, a)); break;
+    case ADJ_FUSED: FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_stage_kernel<PP, ADJ_FUSED>), gr, dim3(BS), 0, s, a)); break;
+    case ADJ_SENS: FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_stage_kernel<PP, ADJ_SENS>), gr, dim3(BS), 0, s, a)); break;
+    case ADJ_SPLIT: {
+      const int E = BS < 128 ? BS : 128;
+      dim3 g2 = grid_for(a.m.K, E);
+      FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_split_kernel<PP>), g2, dim3(2 * E), 0, s, a));
+    } break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

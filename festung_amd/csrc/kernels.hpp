@@ -55,9 +55,11 @@ struct Mesh {
   const int* nbr;        // 3 x K
   const int* flags;      // K
   const int* V0T;        // 3 x K (device vertex ids)
+  int bs;                // elements per workgroup of the element kernels (64, 128 or 256)
 };
 
 enum StageOut { OUT_RK_DG = 0, OUT_RK_TAYLOR = 1, OUT_DOT_TAYLOR = 2, OUT_DOT_DG = 3 };
+enum AdjWhat { ADJ_STAGE = 0, ADJ_FUSED = 1, ADJ_SENS = 2, ADJ_SPLIT = 3 };
 enum LimitMode { LIM_SOLUTION = 0, LIM_DOT_RK_DG = 1, LIM_DOT_RK_TAYLOR = 2 };
 
 struct StageArgs {
@@ -121,6 +123,7 @@ struct ToTaylorArgs {
   TabOff off;
   double* out;
   MinMaxSink sink;
+  int bs;
 };
 
 // Adjoint stage in mu = M^-1 lambda space:
@@ -142,22 +145,10 @@ struct AdjArgs {
   int mode;
   double* L0;
   double* out;
-};
-
-// Velocity sensitivities of one time step, accumulated:
-//   G += sum_s alpha_s d/d(u1Disc, u2Disc, vn) [mu_{s+1}^T A c_s],
-// alpha_s = (1 - omega_s) tau_k.
-struct SensArgs {
-  Mesh m;
-  int S;
-  const double* mu[4];   // mu_{s+1}, s = 0 .. S-1
-  const double* c[4];    // c_s
-  double w[4];           // 1 - omega_s
-  double tau;
-  const double* tauK;
-  const double* vn;
-  const double* tab;
-  TabOff off;
+  int what;              // ADJ_STAGE, ADJ_FUSED, ADJ_SENS or ADJ_SPLIT (stage + sensitivities, two threads per element)
+  // velocity sensitivities of this stage (ADJ_FUSED, ADJ_SENS):
+  //   G += (1 - omega) tau_k d/d(u1Disc, u2Disc, vn) [mu^T A c]
+  const double* c;       // stage state c_s
   double* Gu1;
   double* Gu2;
   double* Gvn;
@@ -165,7 +156,6 @@ struct SensArgs {
 
 hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s);
 hipError_t launch_adj_stage(int p, const AdjArgs& a, hipStream_t s);
-hipError_t launch_adj_sens(int p, const SensArgs& a, hipStream_t s);
 // J partials: part[b] = sum over the block of det_k d_k^T hatM d_k, d = c - cObs;
 // muT = 2 d
 hipError_t launch_loss(int p, int K, const double* B, const double* c, const double* cObs, const double* tab,
@@ -186,6 +176,7 @@ hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* c
 hipError_t launch_l2(int p, bool ordF, int K, const double* B, const double* c, const double* cq, const double* tab,
                      const TabOff& off, double* partial, hipStream_t s);
 hipError_t launch_permute(int K, int cols, const double* src, const int* idx, double* dst, hipStream_t s);
-int block_size();
+int block_size();          // workgroup size of the reduction / set-up kernels
+bool valid_elem_block(int bs);  // supported elements per workgroup
 
 }  // namespace festung

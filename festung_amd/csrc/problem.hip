@@ -145,7 +145,10 @@ struct festung_problem {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   double graphTau = std::numeric_limits<double>::quiet_NaN();
-  int nblocks = 0;
+  int bs = 256;      // elements per workgroup of the element kernels
+  int nblocks = 0;   // workgroups of the element kernels = min/max partials
+  int nred = 0;      // workgroups of the reduction kernels (L2 error, misfit)
+  int version = 0;   // bumped when a change invalidates captured step sequences
   // per-launch timing (HIP events on the problem's stream)
   bool prof = false;
   struct Rec {
@@ -180,7 +183,7 @@ struct festung_problem {
 
   double* buf(int id) const { return id == 0 ? X.d() : id == 1 ? Y.d() : Z.d(); }
   Mesh mesh() const {
-    return Mesh{K, numV, B.d(), area.d(), nbr.i(), flg.i(), V0T.i()};
+    return Mesh{K, numV, B.d(), area.d(), nbr.i(), flg.i(), V0T.i(), bs};
   }
   MinMaxSink sink() const {
     return MinMaxSink{cent.d(), partials.d()};
@@ -193,8 +196,15 @@ struct festung_problem {
     graph = nullptr;
     graphTau = std::numeric_limits<double>::quiet_NaN();
   }
+  hipStream_t aux[2] = {nullptr, nullptr};  // side streams of the adjoint sweep
+  void ensure_aux_streams() {
+    for (auto& a : aux)
+      if (!a) HIPCHK(hipStreamCreateWithFlags(&a, hipStreamNonBlocking));
+  }
   ~festung_problem() {
     drop_graph();
+    for (auto a : aux)
+      if (a) (void)hipStreamDestroy(a);
     for (auto& r : recs) {
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);
@@ -443,7 +453,11 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     h->u2.alloc(nk);
     h->vn.alloc(size_t(3) * t.Re * K * sizeof(double));
     HIPCHK(hipMemset(h->X.p, 0, nk));
-    h->nblocks = (K + block_size() - 1) / block_size();
+    // default elements per workgroup: the largest size that still gives at
+    // least 1024 workgroups (4 per compute unit), 64 below that
+    h->bs = K >= 1024 * 256 ? 256 : K >= 1024 * 128 ? 128 : 64;
+    h->nblocks = (K + h->bs - 1) / h->bs;
+    h->nred = (K + block_size() - 1) / block_size();
     if (flags & FESTUNG_FLAG_LIMITER) {
       h->T1.alloc(nk);
       h->T2.alloc(nk);
@@ -455,7 +469,7 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     } else {
       h->cent.alloc(size_t(K) * sizeof(double));
     }
-    h->partials.alloc(size_t(2) * (h->nblocks + 1) * sizeof(double));
+    h->partials.alloc(size_t(2) * ((K + 63) / 64 + 1) * sizeof(double));  // room for 64 elements per workgroup
     h->gmm.alloc(2 * sizeof(double));
     HIPCHK(hipDeviceSynchronize());
     *out = h.release();
@@ -692,7 +706,7 @@ int festung_limit(festung_problem* h) {
     h->set_device();
     if (!(h->flags & FESTUNG_FLAG_LIMITER)) fail(FESTUNG_ERR_STATE, "problem created without the limiter");
     h->finish_stage_sequence();
-    ToTaylorArgs ta{h->K, h->B.d(), h->X.d(), h->tab.d(), h->off, h->T2.d(), h->sink()};
+    ToTaylorArgs ta{h->K, h->B.d(), h->X.d(), h->tab.d(), h->off, h->T2.d(), h->sink(), h->bs};
     HIPCHK(launch_to_taylor(h->p, ta, h->stream));
     h->enqueue_vertex();
     LimitArgs l{};
@@ -783,10 +797,10 @@ int festung_l2_error(festung_problem* h, int ord, const double* cq, double* err)
     const int R = ordF ? h->tabs.RF : h->tabs.RE;
     DBuf q, part;
     h->upload_elem(q, cq, R);
-    part.alloc(size_t(h->nblocks) * sizeof(double));
+    part.alloc(size_t(h->nred) * sizeof(double));
     HIPCHK(launch_l2(h->p, ordF, h->K, h->B.d(), h->buf(h->cur), q.d(), h->tab.d(), h->off, part.d(), h->stream));
     sync(h);
-    std::vector<double> hp(h->nblocks);
+    std::vector<double> hp(h->nred);
     HIPCHK(hipMemcpy(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost));
     double s = 0;
     for (double v : hp) s += v;
@@ -799,12 +813,87 @@ int festung_set_tau_field(festung_problem* h, const double* tauK) {
     check_handle(h);
     h->set_device();
     h->drop_graph();
+    ++h->version;
     if (!tauK) {
       h->hasTauK = false;
       return;
     }
     h->upload_elem(h->tauK, tauK, 1, !h->tauK.p);
     h->hasTauK = true;
+  });
+}
+
+int festung_set_elem_block(festung_problem* h, int32_t bs) {
+  return guard([&] {
+    check_handle(h);
+    if (!valid_elem_block(bs)) fail(FESTUNG_ERR_ARG, "elements per workgroup must be 64, 128 or 256 (got %d)", bs);
+    h->set_device();
+    h->finish_stage_sequence();
+    if (bs == h->bs) return;
+    sync(h);
+    h->drop_graph();
+    ++h->version;
+    h->bs = bs;
+    h->nblocks = (h->K + bs - 1) / bs;
+  });
+}
+
+int festung_get_elem_block(const festung_problem* h, int32_t* bs) {
+  return guard([&] {
+    check_handle(h);
+    if (!bs) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    *bs = h->bs;
+  });
+}
+
+int festung_autotune(festung_problem* h, double tau, int32_t nsteps, int32_t* best, double* msPerStep) {
+  return guard([&] {
+    check_handle(h);
+    if (nsteps < 1) fail(FESTUNG_ERR_ARG, "autotune needs at least one timed step");
+    h->set_device();
+    h->require_velocity();
+    h->finish_stage_sequence();
+    if (h->prof) fail(FESTUNG_ERR_STATE, "profiling is on");
+    // the trial steps advance the state: keep it and put it back afterwards
+    DBuf save;
+    const size_t nk = size_t(h->N) * h->K * sizeof(double);
+    save.alloc(nk);
+    HIPCHK(hipMemcpyAsync(save.p, h->buf(h->cur), nk, hipMemcpyDeviceToDevice, h->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int cand[3] = {64, 128, 256};
+    double bestMs = 0.0;
+    int bestBs = h->bs;
+    try {
+      for (int c = 0; c < 3; ++c) {
+        check_call(festung_set_elem_block(h, cand[c]));
+        check_call(festung_step(h, 2, tau));  // capture + warm-up
+        HIPCHK(hipEventRecord(e0, h->stream));
+        check_call(festung_step(h, nsteps, tau));
+        HIPCHK(hipEventRecord(e1, h->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        const double per = double(ms) / nsteps;
+        if (msPerStep) msPerStep[c] = per;
+        if (c == 0 || per < bestMs) {
+          bestMs = per;
+          bestBs = cand[c];
+        }
+        HIPCHK(hipMemcpyAsync(h->X.p, save.p, nk, hipMemcpyDeviceToDevice, h->stream));
+        h->cur = 0;
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    check_call(festung_set_elem_block(h, bestBs));
+    sync(h);
+    if (best) *best = bestBs;
   });
 }
 
@@ -849,46 +938,87 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
     const size_t nk = size_t(N) * K * sizeof(double);
     const int C = (ckpt <= 0 || ckpt > nsteps) ? nsteps : ckpt;
     const int nseg = (nsteps + C - 1) / C;
-    std::vector<std::unique_ptr<DBuf>> ck(nseg + 1), sb(size_t(C) * S);
+    // Transposed stage kernel: the velocity sensitivities fused into it (one
+    // thread per element), or split over two threads per element, which
+    // doubles the waves in flight where the mesh is too small to fill the GPU.
+    int what = h->nblocks < 2048 ? ADJ_SPLIT : ADJ_FUSED;
+    if (h->flags & FESTUNG_FLAG_ADJ_FUSED) what = ADJ_FUSED;
+    if (h->flags & FESTUNG_FLAG_ADJ_SPLIT) what = ADJ_SPLIT;
+    // Streams: A = the problem's stream (the reverse sweep); Cs = the
+    // recomputation of the next segment's stage states, into the other of
+    // two buffer sets, overlapping the sweep (unless FESTUNG_FLAG_ADJ_SERIAL).
+    const bool overlap = !(h->flags & FESTUNG_FLAG_ADJ_SERIAL) && nseg > 2;
+    hipStream_t A = h->stream, Cst = A;
+    if (overlap) {
+      h->ensure_aux_streams();
+      Cst = h->aux[0];
+    }
+    const int nsets = overlap ? 2 : 1;
+    std::vector<std::unique_ptr<DBuf>> ck(nseg + 1), sb(size_t(nsets) * C * S);
     for (auto& b : ck) {
       b.reset(new DBuf);
       b->alloc(nk);
     }
     for (size_t j = 0; j < sb.size(); ++j)
-      if (j % S != 0) {  // stage 0 of local step 0 lives in the checkpoint
-        sb[j].reset(new DBuf);
-        sb[j]->alloc(nk);
-      } else if (j > 0) {
+      if (j % (size_t(C) * S) != 0) {  // stage 0 of local step 0 lives in the checkpoint
         sb[j].reset(new DBuf);
         sb[j]->alloc(nk);
       }
-    DBuf dObs, part, muA, muB, mu1, mu2, L0, Gu1, Gu2, Gvn, tmp;
+    // ring of adjoint stage states: stage i reads ring[(i-1) mod R], writes ring[i mod R]
+    constexpr int R = 4;
+    DBuf ring[R];
+    for (auto& b : ring) b.alloc(nk);
+    DBuf dObs, part, L0, Gu1, Gu2, Gvn, tmp;
     h->upload_elem(dObs, cObs, N);
-    part.alloc(size_t(h->nblocks) * sizeof(double));
-    for (DBuf* b : {&muA, &muB, &mu1, &mu2, &L0, &Gu1, &Gu2, &tmp}) b->alloc(nk);
+    part.alloc(size_t(h->nred) * sizeof(double));
+    for (DBuf* b : {&L0, &Gu1, &Gu2, &tmp}) b->alloc(nk);
     Gvn.alloc(size_t(3) * Re * K * sizeof(double));
-    hipEvent_t ev[3];
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    // events: 0..2 timing; segment hand-offs between A and Cs
+    std::vector<hipEvent_t> ev(3, nullptr), evSeg(2, nullptr), evDone(2, nullptr), evFwd(1, nullptr);
     struct EvGuard {
-      hipEvent_t* e;
+      std::vector<hipEvent_t>* v[4];
       ~EvGuard() {
-        for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+        for (auto* x : v)
+          for (auto e : *x)
+            if (e) (void)hipEventDestroy(e);
       }
-    } evg{ev};
+    } evg{{&ev, &evSeg, &evDone, &evFwd}};
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    for (auto* x : {&evSeg, &evDone, &evFwd})
+      for (auto& e : *x) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     auto state = [&](int seg, int l, int len, int st) -> double* {
       if (l == len) return ck[seg + 1]->d();
       if (l == 0 && st == 0) return ck[seg]->d();
-      return sb[size_t(l) * S + st]->d();
+      return sb[(size_t(seg % nsets) * C + l) * S + st]->d();
     };
-    auto run_segment = [&](int seg) {
+    // A recomputation does not rewrite the segment's end checkpoint (the
+    // sweep may read it meanwhile): its last stage output goes to junk.
+    // Segments whose stage states are not kept run through a small rotation
+    // of buffers instead (only their end checkpoint is written).
+    DBuf junk, rot[4];
+    if (nseg > 1) junk.alloc(nk);
+    if (nseg > nsets)
+      for (auto& b : rot) b.alloc(nk);
+    enum { SEG_STORE, SEG_RECOMPUTE, SEG_PASS };
+    auto run_segment = [&](int seg, hipStream_t sm, int how) {
       const int len = std::min(C, nsteps - seg * C);
-      for (int l = 0; l < len; ++l)
+      double* cur = ck[seg]->d();
+      for (int l = 0; l < len; ++l) {
+        double* next = l + 1 == len ? ck[seg + 1]->d() : rot[l % 2].d();
         for (int st = 0; st < S; ++st) {
           StageArgs a{};
           a.m = h->mesh();
-          a.c = state(seg, l, len, st);
-          a.c0 = state(seg, l, len, 0);
+          if (how == SEG_PASS) {
+            a.c = st == 0 ? cur : rot[1 + st].d();
+            a.c0 = cur;
+            a.out = st + 1 < S ? rot[2 + st].d() : next;
+          } else {
+            a.c = state(seg, l, len, st);
+            a.c0 = state(seg, l, len, 0);
+            a.out = st + 1 < S ? state(seg, l, len, st + 1) : state(seg, l + 1, len, 0);
+            if (how == SEG_RECOMPUTE && l + 1 == len && st + 1 == S) a.out = junk.d();
+          }
           a.u1 = h->u1.d();
           a.u2 = h->u2.d();
           a.vn = h->vn.d();
@@ -897,33 +1027,51 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
           a.tau = tau;
           a.tauK = h->hasTauK ? h->tauK.d() : nullptr;
           a.rk = h->plan[st].rk;
-          a.out = st + 1 < S ? state(seg, l, len, st + 1) : state(seg, l + 1, len, 0);
           a.sink = h->sink();
-          HIPCHK(launch_stage(h->p, OUT_RK_DG, a, h->stream));
+          HIPCHK(launch_stage(h->p, OUT_RK_DG, a, sm));
         }
+        cur = next;
+      }
     };
 
-    // forward, storing checkpoints (and, within the last segment, all stage states)
-    HIPCHK(hipEventRecord(ev[0], h->stream));
-    HIPCHK(hipMemcpyAsync(ck[0]->p, h->buf(h->cur), nk, hipMemcpyDeviceToDevice, h->stream));
-    for (int seg = 0; seg < nseg; ++seg) run_segment(seg);
-    HIPCHK(hipEventRecord(ev[1], h->stream));
-    // loss and terminal adjoint
-    HIPCHK(launch_loss(h->p, K, h->B.d(), ck[nseg]->d(), dObs.d(), h->tab.d(), h->off, part.d(), muA.d(), h->stream));
-    HIPCHK(launch_fill(size_t(N) * K, 0.0, Gu1.d(), h->stream));
-    HIPCHK(launch_fill(size_t(N) * K, 0.0, Gu2.d(), h->stream));
-    HIPCHK(launch_fill(size_t(3) * Re * K, 0.0, Gvn.d(), h->stream));
-    double* muCur = muA.d();
-    double* muNext = muB.d();
-    double* muS[4] = {nullptr, mu1.d(), mu2.d(), nullptr};
+    // forward, storing checkpoints (and the stage states of the last segments)
+    HIPCHK(hipEventRecord(ev[0], A));
+    HIPCHK(hipMemcpyAsync(ck[0]->p, h->buf(h->cur), nk, hipMemcpyDeviceToDevice, A));
+    for (int seg = 0; seg < nseg; ++seg) run_segment(seg, A, seg < nseg - nsets ? SEG_PASS : SEG_STORE);
+    HIPCHK(hipEventRecord(ev[1], A));
+    // loss and terminal adjoint (into ring[R-1], read by stage 0)
+    HIPCHK(launch_loss(h->p, K, h->B.d(), ck[nseg]->d(), dObs.d(), h->tab.d(), h->off, part.d(), ring[R - 1].d(), A));
+    HIPCHK(launch_fill(size_t(N) * K, 0.0, Gu1.d(), A));
+    HIPCHK(launch_fill(size_t(N) * K, 0.0, Gu2.d(), A));
+    HIPCHK(launch_fill(size_t(3) * Re * K, 0.0, Gvn.d(), A));
+    if (overlap) {
+      HIPCHK(hipEventRecord(evFwd[0], A));
+      // enqueue the second stream's work only once the forward pass is done:
+      // a pending cross-stream wait slows the forward's dispatch
+      HIPCHK(hipEventSynchronize(ev[1]));
+      HIPCHK(hipStreamWaitEvent(Cst, evFwd[0], 0));
+    }
+    long i = 0;  // adjoint stage counter
     for (int seg = nseg - 1; seg >= 0; --seg) {
       const int len = std::min(C, nsteps - seg * C);
-      if (seg != nseg - 1) run_segment(seg);  // recompute the segment's stage states
+      // The forward pass leaves the stage states of the last nsets segments
+      // in place; earlier ones are recomputed from their checkpoints.
+      if (seg < nseg - nsets) {
+        if (overlap) HIPCHK(hipStreamWaitEvent(A, evSeg[seg % 2], 0));
+        else run_segment(seg, A, SEG_RECOMPUTE);
+      }
+      // recompute the previous segment on Cs, into the other set, once the
+      // sweep over the segment that used that set (seg + 1) is done
+      if (overlap && seg >= 1 && seg - 1 < nseg - nsets) {
+        HIPCHK(hipStreamWaitEvent(Cst, evDone[(seg + 1) % 2], 0));
+        run_segment(seg - 1, Cst, SEG_RECOMPUTE);
+        HIPCHK(hipEventRecord(evSeg[(seg - 1) % 2], Cst));
+      }
       for (int l = len - 1; l >= 0; --l) {
-        for (int st = S - 1; st >= 0; --st) {
+        for (int st = S - 1; st >= 0; --st, ++i) {
           AdjArgs a{};
           a.m = h->mesh();
-          a.mu = st + 1 == S ? muCur : muS[st + 1];
+          a.mu = ring[(i + R - 1) % R].d();
           a.u1 = h->u1.d();
           a.u2 = h->u2.d();
           a.vn = h->vn.d();
@@ -934,36 +1082,25 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
           a.omega = h->plan[st].rk.a0;
           a.L0 = L0.d();
           a.mode = S == 1 ? 3 : (st == S - 1 ? 0 : (st > 0 ? 1 : 2));
-          a.out = st > 0 ? muS[st] : muNext;
-          HIPCHK(launch_adj_stage(h->p, a, h->stream));
+          a.out = ring[i % R].d();
+          a.c = state(seg, l, len, st);
+          a.Gu1 = Gu1.d();
+          a.Gu2 = Gu2.d();
+          a.Gvn = Gvn.d();
+          a.what = what;
+          HIPCHK(launch_adj_stage(h->p, a, A));
         }
-        SensArgs sa{};
-        sa.m = h->mesh();
-        sa.S = S;
-        for (int st = 0; st < S; ++st) {
-          sa.mu[st] = st + 1 == S ? muCur : muS[st + 1];
-          sa.c[st] = state(seg, l, len, st);
-          sa.w[st] = 1.0 - h->plan[st].rk.a0;
-        }
-        sa.tau = tau;
-        sa.tauK = h->hasTauK ? h->tauK.d() : nullptr;
-        sa.vn = h->vn.d();
-        sa.tab = h->tab.d();
-        sa.off = h->off;
-        sa.Gu1 = Gu1.d();
-        sa.Gu2 = Gu2.d();
-        sa.Gvn = Gvn.d();
-        HIPCHK(launch_adj_sens(h->p, sa, h->stream));
-        std::swap(muCur, muNext);
       }
+      if (overlap) HIPCHK(hipEventRecord(evDone[seg % 2], A));
     }
-    HIPCHK(launch_mass(h->p, K, h->B.d(), muCur, h->tab.d(), h->off, tmp.d(), h->stream));
-    HIPCHK(hipEventRecord(ev[2], h->stream));
+    const double* muFinal = ring[(i + R - 1) % R].d();
+    HIPCHK(launch_mass(h->p, K, h->B.d(), muFinal, h->tab.d(), h->off, tmp.d(), A));
+    HIPCHK(hipEventRecord(ev[2], A));
     // final state becomes the problem's state
-    HIPCHK(hipMemcpyAsync(h->X.p, ck[nseg]->p, nk, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->X.p, ck[nseg]->p, nk, hipMemcpyDeviceToDevice, A));
     h->cur = 0;
     sync(h);
-    std::vector<double> hp(h->nblocks);
+    std::vector<double> hp(h->nred);
     HIPCHK(hipMemcpy(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost));
     double j = 0;
     for (double v : hp) j += v;
@@ -1029,6 +1166,7 @@ struct festung_batch {
   hipGraphExec_t exec = nullptr;
   int nsteps = -1;
   std::vector<double> taus;
+  std::vector<int> versions;
   std::vector<hipEvent_t> events;
   void drop() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -1085,7 +1223,9 @@ int festung_batch_step(festung_batch* b, int32_t nsteps, const double* tau) {
       if (h->prof) fail(FESTUNG_ERR_STATE, "profiling is on for a batched problem");
     }
     std::vector<double> taus(tau, tau + n);
-    if (!b->exec || b->nsteps != nsteps || b->taus != taus) {
+    std::vector<int> versions;
+    for (auto* h : b->hs) versions.push_back(h->version);
+    if (!b->exec || b->nsteps != nsteps || b->taus != taus || b->versions != versions) {
       b->drop();
       HIPCHK(hipStreamBeginCapture(h0->stream, hipStreamCaptureModeThreadLocal));
       try {
@@ -1107,6 +1247,7 @@ int festung_batch_step(festung_batch* b, int32_t nsteps, const double* tau) {
       HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
       b->nsteps = nsteps;
       b->taus = taus;
+      b->versions = versions;
     }
     HIPCHK(hipGraphLaunch(b->exec, h0->stream));
     for (auto* h : b->hs) h->cur = 0;

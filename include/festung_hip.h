@@ -115,7 +115,13 @@ int festung_ref_get(int p, const char* name, int ord, double* out);
 
 enum {
   FESTUNG_FLAG_LIMITER = 1,      /* Taylor-linear vertex-based slope limiter (typeSlopeLim 'linear') */
-  FESTUNG_FLAG_GRAPH = 2         /* capture whole steps into a hipGraph in festung_step */
+  FESTUNG_FLAG_GRAPH = 2,        /* capture whole steps into a hipGraph in festung_step */
+  FESTUNG_FLAG_ADJ_SERIAL = 4,   /* festung_adjoint: recompute checkpointed segments on the problem's
+                                    stream (default: on a second stream, overlapping the sweep) */
+  FESTUNG_FLAG_ADJ_FUSED = 8,    /* festung_adjoint: velocity sensitivities in the transposed stage
+                                    kernel's thread (default on large meshes) */
+  FESTUNG_FLAG_ADJ_SPLIT = 16    /* festung_adjoint: stage and sensitivities on two threads per
+                                    element (default on small meshes) */
 };
 
 /* Upload the grid (its connectivity and marks as they are now) and build the
@@ -185,6 +191,20 @@ int festung_step(festung_problem* h, int nsteps, double tau);
  * Lets independent problems with different time steps share one device
  * problem (see festung_grid_concat).  No counterpart in the reference. */
 int festung_set_tau_field(festung_problem* h, const double* tauK);
+
+/* Elements per workgroup of the element kernels (stage, limiter, adjoint):
+ * 64, 128 or 256.  Small meshes fill more compute units with
+ * smaller workgroups.  The default is the largest size giving at least 1024
+ * workgroups, else 64.  Drops a captured step graph.  No counterpart in the
+ * reference (its operators are sparse matrix products). */
+int festung_set_elem_block(festung_problem* h, int32_t elemsPerBlock);
+int festung_get_elem_block(const festung_problem* h, int32_t* elemsPerBlock);
+
+/* Autotuner: times nsteps steps of size tau with each elements-per-workgroup
+ * choice (64, 128, 256; msPerStep[0..2] if not NULL), keeps the fastest
+ * (*best) and restores the state the call started from.  No counterpart in
+ * the reference. */
+int festung_autotune(festung_problem* h, double tau, int32_t nsteps, int32_t* best, double* msPerStep);
 
 /* Batched stepping: n problems advanced nsteps steps each (tau[i] for
  * problem i; ignored where a tau field is set) as ONE hipGraph launch whose

@@ -125,3 +125,38 @@ def test_autograd_wrapper(gpu):
     _, g0, ga, _ = m.gradient(m.c0, a, 5, tau, cObs)
     assert np.allclose(c0.grad.numpy(), g0, rtol=0, atol=0)
     assert np.allclose(at.grad.numpy(), ga, rtol=0, atol=0)
+
+
+def test_adjoint_elem_block_invariance(gpu):
+    """The transposed stage kernels, with their fused velocity sensitivities,
+    give the same gradient bit for bit for every workgroup size."""
+    m, a_true, a, tau = setup(2, 1 / 24, 4)
+    cObs = m.run(m.c0, a_true, 6, tau)
+    res = []
+    for bs in (64, 128, 256):
+        m.prob.elem_block = bs
+        res.append(m.gradient(m.c0, a, 6, tau, cObs, ckpt=2))
+    for r in res[1:]:
+        assert r[0] == res[0][0]
+        assert np.array_equal(r[1], res[0][1]) and np.array_equal(r[2], res[0][2])
+
+
+@pytest.mark.parametrize('nsteps,ckpt', [(12, 2), (12, 5), (7, 0), (9, 4), (10, 3)])
+def test_adjoint_variants_agree(gpu, nsteps, ckpt):
+    """Every way of running the sweep gives the same results bit for bit:
+    the velocity sensitivities fused into the transposed stage kernel or on
+    a second thread per element, checkpointed segments recomputed on the
+    problem's stream or on a second stream overlapping the sweep."""
+    res = []
+    for serial, kern in ((True, 'fused'), (False, 'fused'), (False, 'split'), (True, 'split')):
+        modes = AD.stream_modes()[:4]
+        m = AD.StreamModel(1 / 16, 2, modes=modes, c0=TC.gaussian_humps(51), adj_serial=serial, adj_kernel=kern)
+        a = 0.05 * np.random.default_rng(52).standard_normal(len(modes))
+        tau = 0.1 / (16 * 0.8 * 5)
+        cObs = m.run(m.c0, 1.1 * a, nsteps, tau)
+        res.append(m.gradient(m.c0, a, nsteps, tau, cObs, ckpt=ckpt))
+    for r in res[1:]:
+        assert r[0] == res[0][0]
+        assert np.array_equal(r[1], res[0][1]) and np.array_equal(r[2], res[0][2])
+        for key in ('gradU1', 'gradU2', 'gradVn'):
+            assert np.array_equal(r[3][key], res[0][3][key])

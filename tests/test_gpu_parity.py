@@ -164,3 +164,45 @@ def test_device_pointer_state(gpu):
     d.problem.get_state_dev(out)
     d.problem.sync()
     assert np.array_equal(out.cpu().numpy().T, c)
+
+
+@pytest.mark.parametrize('p,ordRK,lim', [(1, 2, True), (2, 3, True), (3, 3, True), (4, 3, True), (2, 4, True),
+                                         (0, 1, False)])
+def test_elem_block_invariance(gpu, p, ordRK, lim):
+    """Every element's arithmetic, and the limiter's global min / max, are
+    independent of the number of elements per workgroup: the states after a
+    few steps agree bit for bit (K = 1152 leaves a partial workgroup)."""
+    tc = TC.rotating_humps()
+    outs = []
+    for bs in (64, 128, 256):
+        d = product_problem(tc, p, 1 / 24, ordRK, lim, graph=True)
+        d.problem.elem_block = bs
+        assert d.problem.elem_block == bs
+        product_step(d, 1)
+        d.problem.step(3, d.tau)
+        outs.append(d.problem.get_state())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_elem_block_rejects_invalid(gpu):
+    from festung_amd._lib import FestungError
+    d = product_problem(TC.rotating_humps(), 1, 1 / 8, 2, False)
+    for bs in (0, 32, 96, 512):
+        with pytest.raises(FestungError, match='elements per workgroup'):
+            d.problem.elem_block = bs
+
+
+def test_autotune_keeps_state(gpu):
+    tc = TC.rotating_humps()
+    d = product_problem(tc, 2, 1 / 32, 3, True, graph=True)
+    product_step(d, 1)
+    c0 = d.problem.get_state()
+    best, ms = d.problem.autotune(d.tau, 3)
+    assert best in (64, 128, 256) and d.problem.elem_block == best
+    assert set(ms) == {64, 128, 256} and all(v > 0 for v in ms.values())
+    assert np.array_equal(d.problem.get_state(), c0)
+    d.problem.step(2, d.tau)
+    e = product_problem(tc, 2, 1 / 32, 3, True, graph=True)
+    product_step(e, 1)
+    e.problem.step(2, e.tau)
+    assert np.array_equal(d.problem.get_state(), e.problem.get_state())
