@@ -253,6 +253,8 @@ def main():
     K, N, numV = prob.K, prob.N, pd.g.numV
     maxDeg = int(np.max(np.diff(pd.g.v2tPtr)))
     nstage = {1: 1, 2: 2, 3: 3, 4: 4}[ordRK]
+    # elements per workgroup: the autotuner times 64 / 128 / 256 and keeps the fastest
+    best_bs, tune_ms = prob.autotune(tau, 10)
     prob.step(a.warmup, tau)
     prob.sync()
     torch.cuda.synchronize()
@@ -344,6 +346,7 @@ def main():
                                % (np.log2(a.hmax), K, p, N, ordRK, 'on' if lim else 'off'),
                    'K': K, 'p': p, 'N': N, 'ordRK': ordRK, 'limiter': lim, 'tau': tau,
                    'stages_per_step': nstage, 'event_ms_per_step': ev_ms / a.steps,
+                   'elem_block': best_bs, 'autotune_ms_per_step': tune_ms,
                    'setup_s': t_setup},
         'roofline': roof,
         'kernels': kernels,

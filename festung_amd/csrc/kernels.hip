@@ -477,8 +477,6 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
     if (d > dMax - tol) al = fmax(0.0, fmin(1.0, dMax / (d + tol)));
     alpha = fmin(alpha, al);
   }
-  if (a.alphaOut && active) a.alphaOut[k] = alpha;
-
   double lim[N];
   lim[0] = x[0];
   lim[1] = alpha * x[1];
@@ -524,6 +522,7 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
   double y[N];
   taylor_to_dg<P>(lim, g, T + a.off.Finv, y);
 
+  if (a.alphaOut && active) a.alphaOut[k] = alpha;
   if constexpr (MODE == LIM_SOLUTION) {
     if (active)
 #pragma unroll

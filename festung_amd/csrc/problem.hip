@@ -249,7 +249,9 @@ struct festung_problem {
   }
 
   // One RK stage: state buffer `in` -> `out`, step start in X.
-  void enqueue_stage(const StagePlan& sp, double tau) {
+  // keepAlpha: write the solution limiter's alpha (read by festung_limiter_info;
+  // only the last stage of a whole step needs to)
+  void enqueue_stage(const StagePlan& sp, double tau, bool keepAlpha = true) {
     const bool lim = (flags & FESTUNG_FLAG_LIMITER) != 0;
     const double* cin = buf(sp.in);
     double* cout = buf(sp.out);
@@ -309,12 +311,12 @@ struct festung_problem {
     l.taylor = T2.d();
     l.out = cout;
     l.vertexD = hasVertexD ? vertexD.d() : nullptr;
-    l.alphaOut = alpha.d();
+    l.alphaOut = keepAlpha ? alpha.d() : nullptr;
     timed(FESTUNG_KERNEL_LIMIT, [&] { HIPCHK(launch_limit(p, LIM_SOLUTION, l, stream)); });
   }
 
   void enqueue_step(double tau) {
-    for (const auto& sp : plan) enqueue_stage(sp, tau);
+    for (size_t i = 0; i < plan.size(); ++i) enqueue_stage(plan[i], tau, i + 1 == plan.size());
     if (plan.back().out != 0)
       timed(FESTUNG_KERNEL_COPY, [&] {
         HIPCHK(hipMemcpyAsync(X.p, buf(plan.back().out), size_t(N) * K * sizeof(double), hipMemcpyDeviceToDevice,
