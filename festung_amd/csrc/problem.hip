@@ -215,6 +215,7 @@ struct festung_problem {
 
   template <class T>
   void upload(DBuf& b, const T* h, size_t n) {
+    HIPCHK(hipStreamSynchronize(stream));  // kernels in flight may still use b
     b.alloc(n * sizeof(T));
     HIPCHK(hipMemcpy(b.p, h, n * sizeof(T), hipMemcpyHostToDevice));
   }
@@ -231,12 +232,16 @@ struct festung_problem {
   }
   void upload_elem(DBuf& b, const double* src, int cols, bool realloc = true) {
     std::vector<double> v = to_dev(src, cols);
+    // the copy goes through the null stream, which does not wait for the
+    // problem's (non-blocking) stream: let kernels in flight finish first
+    HIPCHK(hipStreamSynchronize(stream));
     if (realloc || !b.p) b.alloc(v.size() * sizeof(double));
     HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   void download_elem(const DBuf& b, const double* devp, int cols, double* dst) {
     (void)b;
     std::vector<double> v(size_t(K) * cols);
+    HIPCHK(hipStreamSynchronize(stream));
     HIPCHK(hipMemcpy(v.data(), devp, v.size() * sizeof(double), hipMemcpyDeviceToHost));
     from_dev(v.data(), cols, dst);
   }
@@ -568,6 +573,7 @@ int festung_set_vertex_dirichlet(festung_problem* h, const double* vertexD) {
     }
     std::vector<double> v(h->numV);
     for (int32_t d = 0; d < h->numV; ++d) v[d] = vertexD[h->vperm[d]];
+    sync(h);  // kernels in flight may still read the old values
     HIPCHK(hipMemcpy(h->vertexD.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
     h->hasVertexD = true;
   });

@@ -206,3 +206,34 @@ def test_autotune_keeps_state(gpu):
     product_step(e, 1)
     e.problem.step(2, e.tau)
     assert np.array_equal(d.problem.get_state(), e.problem.get_state())
+
+
+def test_stagewise_then_steps(gpu):
+    """Stage-by-stage steps, a state reset and then captured whole steps give
+    the result of the same number of steps on a fresh problem."""
+    tc = TC.rotating_humps()
+    a = product_problem(tc, 2, 1 / 16, 3, True, graph=True)
+    product_step(a, 1)
+    c1 = a.problem.get_state()
+    for _ in range(2):
+        a.problem.set_state(c1)
+        a.problem.begin_step()
+        for n in range(1, 4):
+            a.problem.stage(n, a.tau)
+    a.problem.step(2, a.tau)
+    b = product_problem(tc, 2, 1 / 16, 3, True, graph=True)
+    product_step(b, 1)
+    b.problem.step(3, b.tau)
+    assert np.array_equal(a.problem.get_state(), b.problem.get_state())
+
+
+def test_set_state_waits_for_running_steps(gpu):
+    """A state upload right after asynchronously enqueued steps lands after
+    them (the problem's stream is drained first), and so does a download."""
+    tc = TC.rotating_humps()
+    d = product_problem(tc, 2, 1 / 32, 3, True, graph=True)
+    product_step(d, 1)
+    c1 = d.problem.get_state()
+    d.problem.step(20, d.tau)       # returns before the steps have run
+    d.problem.set_state(c1)
+    assert np.array_equal(d.problem.get_state(), c1)
