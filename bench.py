@@ -38,23 +38,24 @@ def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6):
     once per element / vertex)."""
     N = (p + 1) * (p + 2) // 2
     Re = p + 1
-    omega = {1: [0], 2: [0, 0.5], 3: [0, 0.75, 1 / 3], 4: [1, 1, 1, 0]}[ordRK]
-    S = len(omega)
+    # per stage: (reads c0, reads the RK4 accumulator, writes it)
+    rk = {1: [(0, 0, 0)], 2: [(0, 0, 0), (1, 0, 0)], 3: [(0, 0, 0), (1, 0, 0), (1, 0, 0)],
+          4: [(1, 0, 1), (1, 1, 1), (1, 1, 1), (0, 1, 0)]}[ordRK]
     tot = 0.0
-    for s in range(S):
-        w = omega[s] != 0
+    for (c0, ai, ao) in rk:
+        comb = N * (c0 + ai + ao)    # RK-combination operands besides c and the output
         if kind == 'stage':
             if lim and p >= 2:       # cdot -> Taylor: c, u1, u2, vn, B, |E|, nbr, flags -> cdotT, centroid
                 tot += K * (8 * (N + 2 * N + 3 * Re + 4 + 3 + N + 1) + 4 * 4)
             elif lim:                # RK -> Taylor
-                tot += K * (8 * (N + w * N + 2 * N + 3 * Re + 7 + N + 1) + 16)
+                tot += K * (8 * (N + comb + 2 * N + 3 * Re + 7 + N + 1) + 16)
             else:                    # RK -> DG
-                tot += K * (8 * (N + w * N + 2 * N + 3 * Re + 7 + N) + 16)
+                tot += K * (8 * (N + comb + 2 * N + 3 * Re + 7 + N) + 16)
         elif kind == 'vertex' and lim:
             nv = 2 if p >= 2 else 1
             tot += nv * (4 * maxDeg * numV + 8 * K + 16 * numV)   # padded lists, centroids, bounds
         elif kind == 'limit_dot' and lim and p >= 2:
-            tot += K * (8 * (N + 4 + N + w * N + N + 1) + 12) + 16 * numV
+            tot += K * (8 * (N + 4 + N + comb + N + 1) + 12) + 16 * numV
         elif kind == 'limit' and lim:
             tot += K * (8 * (N + 4 + N) + 12) + 16 * numV
     return tot
@@ -187,11 +188,25 @@ def main():
     ap.add_argument('--p', type=int, default=2)
     ap.add_argument('--ordRK', type=int, default=3)
     ap.add_argument('--no-limiter', action='store_true')
+    ap.add_argument('--testcase', choices=('rotating_humps', 'random_discontinuous'), default='rotating_humps',
+                    help='rotating_humps: rotation + 16-mode perturbation, Gaussian humps (p=2 drop-in); '
+                         'random_discontinuous: 32-mode unit-speed velocity, disc/square indicators (p=4)')
+    ap.add_argument('--config', choices=('c1', 'c2', 'c3'), default=None,
+                    help='preset: c1 = hmax 2^-5 p=1 SSP-RK2 200 steps no limiter; c2 = the default; '
+                         'c3 = hmax 2^-8 p=4 RK4 200 steps limiter, random_discontinuous')
     ap.add_argument('--kernel-steps', type=int, default=5, help='eager steps timed per kernel')
     ap.add_argument('--cpu-hmax', type=float, default=2.0 ** -8, help='mesh of the CPU sample')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--copy-mb', type=int, default=1024, help='size of the device copy probe (MiB per buffer)')
     a = ap.parse_args()
+    if a.config == 'c1':
+        a.hmax, a.p, a.ordRK, a.no_limiter, a.testcase = 2.0 ** -5, 1, 2, True, 'rotating_humps'
+        a.steps = 200 if a.steps == 100 else a.steps
+        a.cpu_hmax = 2.0 ** -5
+    elif a.config == 'c3':
+        a.hmax, a.p, a.ordRK, a.no_limiter, a.testcase = 2.0 ** -8, 4, 4, False, 'random_discontinuous'
+        a.steps = 200 if a.steps == 100 else a.steps
+        a.cpu_hmax = 2.0 ** -6
     if a.gpus != 1:
         print('bench.py measures one GPU only: the path runs on a single MI355X; '
               'multi-GPU sharding is not run.', file=sys.stderr)
@@ -235,7 +250,7 @@ def main():
     lim = not a.no_limiter
     p, ordRK = a.p, a.ordRK
     t_setup = time.perf_counter()
-    pd = S.Struct(testcase='rotating_humps', p=p, hmax=a.hmax, ordRK=ordRK, isSlopeLim=lim,
+    pd = S.Struct(testcase=a.testcase, p=p, hmax=a.hmax, ordRK=ordRK, isSlopeLim=lim,
                   numSteps=a.steps, useGraph=True)
     pd = S.configureProblem(pd)
     # tau = CFL 0.1 h / (max|u| (2p+1)), max|u| over the vertices
@@ -329,7 +344,7 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        v, Kc, tl = cpu_baseline(a.cpu_hmax, p, ordRK, lim, TC.rotating_humps)
+        v, Kc, tl = cpu_baseline(a.cpu_hmax, p, ordRK, lim, TC.TESTCASES[a.testcase])
         cpu = dict(value=v, unit='DoF-stage updates/s', cores=1, kind='port',
                    sample='oracle (numpy/scipy reference-structured restatement: per-stage sparse '
                           'kron/kronVec assembly, SpMV, block solves, limiter), 1 thread, 1 time step '
@@ -341,9 +356,13 @@ def main():
         'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': 1e3 * secs / a.steps,
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
         'data': 'synthetic',
-        'config': {'workload': 'advection drop-in step: domainSquare hmax=2^%g K=%d p=%d N=%d SSP-RK%d '
-                               'limiter=%s, rotation+16-mode stream function, 8 Gaussian humps'
-                               % (np.log2(a.hmax), K, p, N, ordRK, 'on' if lim else 'off'),
+        'config': {'workload': 'advection drop-in step: domainSquare hmax=2^%g K=%d p=%d N=%d %s '
+                               'limiter=%s, %s'
+                               % (np.log2(a.hmax), K, p, N, 'RK4' if ordRK == 4 else 'SSP-RK%d' % ordRK,
+                                  'on' if lim else 'off',
+                                  'rotation+16-mode stream function, 8 Gaussian humps'
+                                  if a.testcase == 'rotating_humps' else
+                                  '32-mode stream function (unit max speed), disc/square indicators'),
                    'K': K, 'p': p, 'N': N, 'ordRK': ordRK, 'limiter': lim, 'tau': tau,
                    'stages_per_step': nstage, 'event_ms_per_step': ev_ms / a.steps,
                    'elem_block': best_bs, 'autotune_ms_per_step': tune_ms,

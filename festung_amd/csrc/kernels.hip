@@ -86,6 +86,18 @@ struct Geo {
   }
 };
 
+// Indexing of the loop-ordered tables (TabOff::elemQ, edgeQ).
+template <int P>
+struct QIdx {
+  static constexpr int N = Dims<P>::N, QB = 3 * N + 1, EB = N + 1;
+  static __device__ __forceinline__ int w(int q) { return q * QB; }
+  static __device__ __forceinline__ int phi(int q, int j) { return q * QB + 1 + j; }
+  static __device__ __forceinline__ int d1(int q, int j) { return q * QB + 1 + N + j; }
+  static __device__ __forceinline__ int d2(int q, int j) { return q * QB + 1 + 2 * N + j; }
+  static __device__ __forceinline__ int ew(int n, int r) { return (n * Dims<P>::Re + r) * EB; }
+  static __device__ __forceinline__ int ephi(int n, int r, int j) { return (n * Dims<P>::Re + r) * EB + 1 + j; }
+};
+
 // DG -> Taylor coefficients (exact change of basis between the orthonormal
 // basis and the element's centroid-scaled, mean-free Taylor basis of
 // core/phiTaylorPhy.m: T_0 = mean, T_(a,b) = d_xi^a d_eta^b u at the centroid,
@@ -100,7 +112,7 @@ __device__ __forceinline__ void dg_to_taylor(const double (&x)[Dims<P>::N], cons
   for (int r = 0; r < N; ++r) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) s += F[r + j * N] * x[j];
+    for (int j = 0; j < N; ++j) s += F[r * N + j] * x[j];
     y[r] = s;
   }
   const double s = 1.0 / (g.det * g.dX);
@@ -127,7 +139,7 @@ __device__ __forceinline__ void taylor_to_dg(const double (&T)[Dims<P>::N], cons
   for (int i = 0; i < N; ++i) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) s += Finv[i + j * N] * z[j];
+    for (int j = 0; j < N; ++j) s += Finv[i * N + j] * z[j];
     x[i] = s;
   }
 }
@@ -269,17 +281,17 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
       double cq = 0.0, u1q = 0.0, u2q = 0.0;
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        const double ph = T[a.off.phiE + q + j * RE];
+        const double ph = T[a.off.elemQ + QIdx<P>::phi(q, j)];
         cq += c[j] * ph;
         u1q += u1[j] * ph;
         u2q += u2[j] * ph;
       }
-      const double wc = T[a.off.wE + q] * cq;
+      const double wc = T[a.off.elemQ + QIdx<P>::w(q)] * cq;
       const double f1 = wc * (g.B22 * u1q - g.B12 * u2q);
       const double f2 = wc * (g.B11 * u2q - g.B21 * u1q);
 #pragma unroll
       for (int i = 1; i < N; ++i)
-        acc[i] += T[a.off.gradE + q + i * RE] * f1 + T[a.off.gradE + q + i * RE + RE * N] * f2;
+        acc[i] += T[a.off.elemQ + QIdx<P>::d1(q, i)] * f1 + T[a.off.elemQ + QIdx<P>::d2(q, i)] * f2;
     }
   }
 
@@ -302,15 +314,15 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
       double tr = 0.0;
       if (v > 0.0 && outOn) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) tr += c[j] * T[a.off.phi1D + r + j * Re + n * Re * N];
+        for (int j = 0; j < N; ++j) tr += c[j] * T[a.off.edgeQ + QIdx<P>::ephi(n, r, j)];
       } else if (v < 0.0 && nb >= 0) {
         const double* th = sTheta + r + n * Re * N + np * Re * N * 3;
 #pragma unroll
         for (int j = 0; j < N; ++j) tr += cn[j] * th[j * Re];
       }
-      const double fx = area * v * T[a.off.wEdge + r] * tr;
+      const double fx = area * v * T[a.off.edgeQ + QIdx<P>::ew(n, r)] * tr;
 #pragma unroll
-      for (int i = 0; i < N; ++i) acc[i] -= T[a.off.phi1D + r + i * Re + n * Re * N] * fx;
+      for (int i = 0; i < N; ++i) acc[i] -= T[a.off.edgeQ + QIdx<P>::ephi(n, r, i)] * fx;
     }
   }
   if (a.rhs) {
@@ -324,7 +336,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   for (int i = 0; i < N; ++i) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) s += T[a.off.hatMinv + i + j * N] * acc[j];
+    for (int j = 0; j < N; ++j) s += T[a.off.hatMinvR + i * N + j] * acc[j];
     cdot[i] = s * idet;
   }
 
@@ -335,7 +347,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
     return;
   } else if constexpr (MODE == OUT_DOT_TAYLOR) {
     double Tc[N];
-    dg_to_taylor<P>(cdot, g, T + a.off.F, Tc);
+    dg_to_taylor<P>(cdot, g, T + a.off.FR, Tc);
     if (active) {
 #pragma unroll
       for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
@@ -353,7 +365,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
         for (int i = 0; i < N; ++i) a.out[i * K + k] = nx[i];
     } else {
       double Tc[N];
-      dg_to_taylor<P>(nx, g, T + a.off.F, Tc);
+      dg_to_taylor<P>(nx, g, T + a.off.FR, Tc);
       if (active) {
 #pragma unroll
         for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
@@ -520,7 +532,7 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
     }
   }
   double y[N];
-  taylor_to_dg<P>(lim, g, T + a.off.Finv, y);
+  taylor_to_dg<P>(lim, g, T + a.off.FinvR, y);
 
   if (a.alphaOut && active) a.alphaOut[k] = alpha;
   if constexpr (MODE == LIM_SOLUTION) {
@@ -539,7 +551,7 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
         for (int i = 0; i < N; ++i) a.out[i * K + k] = nx[i];
     } else {
       double Tc[N];
-      dg_to_taylor<P>(nx, g, T + a.off.F, Tc);
+      dg_to_taylor<P>(nx, g, T + a.off.FR, Tc);
       if (active) {
 #pragma unroll
         for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
@@ -564,7 +576,7 @@ __global__ __launch_bounds__(BLOCK) void to_taylor_kernel(ToTaylorArgs a) {
   for (int i = 0; i < N; ++i) x[i] = a.c[i * K + kk];
   Geo<P> g;
   g.load(a.B, kk, K);
-  dg_to_taylor<P>(x, g, a.tab + a.off.F, Tc);
+  dg_to_taylor<P>(x, g, a.tab + a.off.FR, Tc);
   if (active) {
 #pragma unroll
     for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
@@ -754,18 +766,18 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
       double m1 = 0.0, m2 = 0.0, u1q = 0.0, u2q = 0.0, cq = 0.0;
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        const double ph = T[a.off.phiE + q + j * RE];
-        m1 += mu[j] * T[a.off.gradE + q + j * RE];
-        m2 += mu[j] * T[a.off.gradE + q + j * RE + RE * N];
+        const double ph = T[a.off.elemQ + QIdx<P>::phi(q, j)];
+        m1 += mu[j] * T[a.off.elemQ + QIdx<P>::d1(q, j)];
+        m2 += mu[j] * T[a.off.elemQ + QIdx<P>::d2(q, j)];
         u1q += u1[j] * ph;
         u2q += u2[j] * ph;
         if constexpr (SENS) cq += c[j] * ph;
       }
-      const double wq = T[a.off.wE + q];
+      const double wq = T[a.off.elemQ + QIdx<P>::w(q)];
       if constexpr (ADJ) {
         const double f = wq * ((g.B22 * u1q - g.B12 * u2q) * m1 + (g.B11 * u2q - g.B21 * u1q) * m2);
 #pragma unroll
-        for (int j = 0; j < N; ++j) acc[j] += T[a.off.phiE + q + j * RE] * f;
+        for (int j = 0; j < N; ++j) acc[j] += T[a.off.elemQ + QIdx<P>::phi(q, j)] * f;
       }
       if constexpr (SENS) {
         const double wc = al * wq * cq;
@@ -773,8 +785,8 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
         const double f2 = wc * (g.B11 * m2 - g.B12 * m1);
 #pragma unroll
         for (int l = 0; l < N; ++l) {
-          gu1[l] += T[a.off.phiE + q + l * RE] * f1;
-          gu2[l] += T[a.off.phiE + q + l * RE] * f2;
+          gu1[l] += T[a.off.elemQ + QIdx<P>::phi(q, l)] * f1;
+          gu2[l] += T[a.off.elemQ + QIdx<P>::phi(q, l)] * f2;
         }
       }
     }
@@ -795,18 +807,18 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
       if ((ADJ && up) || (SENS && (up || in))) {
         double t = 0.0;
 #pragma unroll
-        for (int i = 0; i < N; ++i) t += mu[i] * T[a.off.phi1D + r + i * Re + n * Re * N];
-        const double wr = T[a.off.wEdge + r];
+        for (int i = 0; i < N; ++i) t += mu[i] * T[a.off.edgeQ + QIdx<P>::ephi(n, r, i)];
+        const double wr = T[a.off.edgeQ + QIdx<P>::ew(n, r)];
         if (ADJ && up) {
           const double fx = area * v * wr * t;
 #pragma unroll
-          for (int j = 0; j < N; ++j) acc[j] -= T[a.off.phi1D + r + j * Re + n * Re * N] * fx;
+          for (int j = 0; j < N; ++j) acc[j] -= T[a.off.edgeQ + QIdx<P>::ephi(n, r, j)] * fx;
         }
         if constexpr (SENS) {
           double cu = 0.0;
           if (up) {
 #pragma unroll
-            for (int j = 0; j < N; ++j) cu += c[j] * T[a.off.phi1D + r + j * Re + n * Re * N];
+            for (int j = 0; j < N; ++j) cu += c[j] * T[a.off.edgeQ + QIdx<P>::ephi(n, r, j)];
           } else {
             const double* th = sTheta + r + n * Re * N + np * Re * N * 3;
 #pragma unroll
@@ -828,7 +840,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
           double t = 0.0;
 #pragma unroll
           for (int i = 0; i < N; ++i) t += mn[i] * sPhi1D[r + i * Re + np * Re * N];
-          const double fx = area * v * T[a.off.wEdge + r] * t;
+          const double fx = area * v * T[a.off.edgeQ + QIdx<P>::ew(n, r)] * t;
           const double* th = sTheta + r + np * Re * N + n * Re * N * 3;
 #pragma unroll
           for (int j = 0; j < N; ++j) acc[j] -= th[j * Re] * fx;
@@ -852,7 +864,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
   for (int i = 0; i < N; ++i) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) s += T[a.off.hatMinv + i + j * N] * acc[j];
+    for (int j = 0; j < N; ++j) s += T[a.off.hatMinvR + i * N + j] * acc[j];
     z[i] = s * idet;
   }
   if (a.mode == 4) {

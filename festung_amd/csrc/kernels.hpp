@@ -23,6 +23,11 @@ struct TabOff {
   int hatM, hatMinv;        // N x N
   int F, Finv;              // Taylor change of basis, N x N
   int nu;                   // 9 x 9 centred reference moments
+  // the same data laid out for the element kernels' loop order, so that the
+  // values one loop iteration needs are contiguous (wide uniform loads):
+  int elemQ;                // per point q of quadRule2D(max(2p,1)): w_q, phi_j(q), d1phi_j(q), d2phi_j(q)
+  int edgeQ;                // per edge n, point r: w_r, phi_j(gamma_n(s_r))
+  int hatMinvR, FR, FinvR;  // N x N row-major
   int total;
 };
 

@@ -397,6 +397,33 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     o.F = put(t.F);
     o.Finv = put(t.Finv);
     o.nu = put(t.nu);
+    {
+      // loop-ordered copies (see TabOff)
+      const int RE = t.RE, Re = t.Re;
+      std::vector<double> eq, ed, mi(size_t(N) * N), fr(size_t(N) * N), fir(size_t(N) * N);
+      for (int q = 0; q < RE; ++q) {
+        eq.push_back(t.ruleE.w[q]);
+        for (int j = 0; j < N; ++j) eq.push_back(t.phiE[q + j * RE]);
+        for (int m = 0; m < 2; ++m)
+          for (int j = 0; j < N; ++j) eq.push_back(t.gradE[q + j * RE + m * RE * N]);
+      }
+      for (int n = 0; n < 3; ++n)
+        for (int r = 0; r < Re; ++r) {
+          ed.push_back(t.ruleEdge.w[r]);
+          for (int j = 0; j < N; ++j) ed.push_back(t.phi1D[r + j * Re + n * Re * N]);
+        }
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+          mi[size_t(i) * N + j] = t.hatMinv[i + size_t(j) * N];
+          fr[size_t(i) * N + j] = t.F[i + size_t(j) * N];
+          fir[size_t(i) * N + j] = t.Finv[i + size_t(j) * N];
+        }
+      o.elemQ = put(eq);
+      o.edgeQ = put(ed);
+      o.hatMinvR = put(mi);
+      o.FR = put(fr);
+      o.FinvR = put(fir);
+    }
     o.total = int(tb.size());
     h->upload(h->tab, tb.data(), tb.size());
 
