@@ -145,9 +145,10 @@ __device__ __forceinline__ void taylor_to_dg(const double (&T)[Dims<P>::N], cons
 }
 
 // Element moments mu_(a,b) = mean over the element of xi^a eta^b for
-// a + b <= 2P, stored at d(d+1)/2 + b (d = a + b).
+// a + b <= 2P, stored at d(d+1)/2 + b (d = a + b); moments_deg writes the
+// D + 1 moments of degree D to out[b].
 template <int P, int D>
-__device__ __forceinline__ void moments_deg(const Geo<P>& g, const double* __restrict__ nu, double* mu) {
+__device__ __forceinline__ void moments_deg(const Geo<P>& g, const double* __restrict__ nu, double* out) {
   // xi^a eta^b = (dX B11 s + dX B12 t)^a (dX B21 s + dX B22 t)^b, s, t centred
   // reference coordinates; contract the coefficients with the reference moments
   const double A11 = g.dX * g.B11, A21 = g.dX * g.B12, A12 = g.dX * g.B21, A22 = g.dX * g.B22;
@@ -168,23 +169,10 @@ __device__ __forceinline__ void moments_deg(const Geo<P>& g, const double* __res
     double s = 0.0;
 #pragma unroll
     for (int bp = 0; bp <= D; ++bp) s += pw[D - bp] * nu[(D - bp) * 9 + bp];
-    mu[D * (D + 1) / 2 + b] = s;
+    out[b] = s;
   }
 }
 
-template <int P>
-__device__ __forceinline__ void element_moments(const Geo<P>& g, const double* __restrict__ nu,
-                                                double (&mu)[(2 * P + 1) * (2 * P + 2) / 2]) {
-  mu[0] = 1.0;
-  if constexpr (2 * P >= 1) moments_deg<P, 1>(g, nu, mu);
-  if constexpr (2 * P >= 2) moments_deg<P, 2>(g, nu, mu);
-  if constexpr (2 * P >= 3) moments_deg<P, 3>(g, nu, mu);
-  if constexpr (2 * P >= 4) moments_deg<P, 4>(g, nu, mu);
-  if constexpr (2 * P >= 5) moments_deg<P, 5>(g, nu, mu);
-  if constexpr (2 * P >= 6) moments_deg<P, 6>(g, nu, mu);
-  if constexpr (2 * P >= 7) moments_deg<P, 7>(g, nu, mu);
-  if constexpr (2 * P >= 8) moments_deg<P, 8>(g, nu, mu);
-}
 
 // Taylor index i -> total degree d, exponent b of the second variable, and
 // a! b! (a = d - b), for i = 0 .. 14 in the order (0,0),(1,0),(0,1),(2,0),...
@@ -192,6 +180,33 @@ __host__ __device__ constexpr int tdeg(int i) { return i == 0 ? 0 : i < 3 ? 1 : 
 __host__ __device__ constexpr int tb(int i) { return i - tdeg(i) * (tdeg(i) + 1) / 2; }
 __host__ __device__ constexpr double fact4(int n) { return n == 4 ? 24.0 : n == 3 ? 6.0 : n == 2 ? 2.0 : 1.0; }
 __host__ __device__ constexpr double tfab(int i) { return fact4(tdeg(i) - tb(i)) * fact4(tb(i)); }
+
+// Rows of the element's Taylor mass matrix applied to d, degree by degree:
+// s_i += sum over j with deg(i) + deg(j) = D of mu_(i+j) d_j, and the
+// diagonal den_i = mu_(2i) - mu_i^2 once D = 2 deg(i).  Only the moments of
+// one degree above P are live at a time (mlo holds degrees 0..P); for each i
+// the j are visited in increasing order.
+template <int P, int D>
+__device__ __forceinline__ void lump_rows(const Geo<P>& g, const double* __restrict__ nu,
+                                          const double (&mlo)[Dims<P>::N], const double (&d)[Dims<P>::N],
+                                          double (&s)[Dims<P>::N], double (&den)[Dims<P>::N]) {
+  constexpr int N = Dims<P>::N;
+  double m[D + 1];
+  if constexpr (D <= P) {
+#pragma unroll
+    for (int b = 0; b <= D; ++b) m[b] = mlo[D * (D + 1) / 2 + b];
+  } else {
+    moments_deg<P, D>(g, nu, m);
+  }
+#pragma unroll
+  for (int i = 1; i < N; ++i) {
+#pragma unroll
+    for (int j = 1; j < N; ++j)
+      if (tdeg(i) + tdeg(j) == D) s[i] += m[tb(i) + tb(j)] * d[j];
+    if (2 * tdeg(i) == D) den[i] = m[2 * tb(i)] - mlo[i] * mlo[i];
+  }
+  if constexpr (D < 2 * P) lump_rows<P, D + 1>(g, nu, mlo, d, s, den);
+}
 
 // ------------------------------------------------------ min / max reduce --
 // Block min / max of the centroid values -> partials[2 blockIdx .. +1].
@@ -249,12 +264,24 @@ __device__ __forceinline__ void rk_combine(const RKCoef& rk, double tau, const d
 
 // ----------------------------------------------------------------- stage --
 
+// Large element tables (p = 4) are staged in LDS: the scalar cache cannot
+// hold them, and every wave streams the whole table.
+template <int P>
+struct ElemTabInLDS {
+  static constexpr bool value = P >= 4;
+  static constexpr int size = value ? Dims<P>::RE * (3 * Dims<P>::N + 1) : 1;
+};
+
 template <int P, int MODE>
 __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
   __shared__ double sTheta[Re * N * 9];
+  __shared__ double sQ[ElemTabInLDS<P>::size];
   for (int i = threadIdx.x; i < Re * N * 9; i += blockDim.x) sTheta[i] = a.tab[a.off.theta + i];
+  if constexpr (ElemTabInLDS<P>::value)
+    for (int i = threadIdx.x; i < ElemTabInLDS<P>::size; i += blockDim.x) sQ[i] = a.tab[a.off.elemQ + i];
   __syncthreads();
+  const double* __restrict__ EQ = ElemTabInLDS<P>::value ? sQ : a.tab + a.off.elemQ;
 
   const int K = a.m.K;
   const int k = gtid();
@@ -281,17 +308,16 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
       double cq = 0.0, u1q = 0.0, u2q = 0.0;
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        const double ph = T[a.off.elemQ + QIdx<P>::phi(q, j)];
+        const double ph = EQ[QIdx<P>::phi(q, j)];
         cq += c[j] * ph;
         u1q += u1[j] * ph;
         u2q += u2[j] * ph;
       }
-      const double wc = T[a.off.elemQ + QIdx<P>::w(q)] * cq;
+      const double wc = EQ[QIdx<P>::w(q)] * cq;
       const double f1 = wc * (g.B22 * u1q - g.B12 * u2q);
       const double f2 = wc * (g.B11 * u2q - g.B21 * u1q);
 #pragma unroll
-      for (int i = 1; i < N; ++i)
-        acc[i] += T[a.off.elemQ + QIdx<P>::d1(q, i)] * f1 + T[a.off.elemQ + QIdx<P>::d2(q, i)] * f2;
+      for (int i = 1; i < N; ++i) acc[i] += EQ[QIdx<P>::d1(q, i)] * f1 + EQ[QIdx<P>::d2(q, i)] * f2;
     }
   }
 
@@ -500,35 +526,28 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
   if constexpr (MODE != LIM_SOLUTION && P >= 2) {
     // selective mass lumping: lim + diag(MT)^-1 MT (orig - lim)
     if (alpha != 1.0) {
-      constexpr int NM = (2 * P + 1) * (2 * P + 2) / 2;
-      double mu[NM];
-      element_moments<P>(g, T + a.off.nu, mu);
       // corr_i = f_i / (mu_2i - mu_i^2) * (sum_j mu_(i+j) d_j - mu_i sum_j mu_j d_j),
       // d_j = (orig_j - lim_j) / f_j, f = a! b!: row i of diag(MT)^-1 MT applied
       // to (orig - lim), with MT the element's Taylor mass matrix
+      double mlo[N];
+      mlo[0] = 1.0;
+      if constexpr (P >= 1) moments_deg<P, 1>(g, T + a.off.nu, mlo + 1);
+      if constexpr (P >= 2) moments_deg<P, 2>(g, T + a.off.nu, mlo + 3);
+      if constexpr (P >= 3) moments_deg<P, 3>(g, T + a.off.nu, mlo + 6);
+      if constexpr (P >= 4) moments_deg<P, 4>(g, T + a.off.nu, mlo + 10);
       double dj[N];
+      dj[0] = 0.0;
 #pragma unroll
       for (int j = 1; j < N; ++j) dj[j] = (x[j] - lim[j]) * (1.0 / tfab(j));
       double S0 = 0.0;
 #pragma unroll
-      for (int j = 1; j < N; ++j) S0 += mu[j] * dj[j];
-      double corr[N];
+      for (int j = 1; j < N; ++j) S0 += mlo[j] * dj[j];
+      double s[N], den[N];
 #pragma unroll
-      for (int i = 1; i < N; ++i) {
-        const int di = tdeg(i), bi = tb(i);
-        double s = 0.0;
+      for (int i = 0; i < N; ++i) s[i] = den[i] = 0.0;
+      lump_rows<P, 2>(g, T + a.off.nu, mlo, dj, s, den);
 #pragma unroll
-        for (int j = 1; j < N; ++j) {
-          const int dj_ = tdeg(j), bj = tb(j);
-          s += mu[(di + dj_) * (di + dj_ + 1) / 2 + (bi + bj)] * dj[j];
-        }
-        s -= mu[i] * S0;
-        const double fi = tfab(i);
-        const double den = mu[(2 * di) * (2 * di + 1) / 2 + 2 * bi] - mu[i] * mu[i];
-        corr[i] = s * fi / den;
-      }
-#pragma unroll
-      for (int i = 1; i < N; ++i) lim[i] += corr[i];
+      for (int i = 1; i < N; ++i) lim[i] += (s[i] - mlo[i] * S0) * tfab(i) / den[i];
     }
   }
   double y[N];
