@@ -21,6 +21,7 @@
 //                      change back to the DG basis, and for dc/dt the RK combination
 //   to_taylor_kernel   DG -> Taylor for a stand-alone limiter call
 //   project / source / vnormal / l2 kernels for setup and checks.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -1013,6 +1014,22 @@ __global__ __launch_bounds__(BLOCK) void fill_kernel(size_t n, double v, double*
 }
 
 // ---------------------------------------------------------- launchers --
+// Profiling events of the calling thread: when set, launches record them at
+// the kernel's own start and end (hipExtLaunchKernel), so the elapsed time
+// is the kernel's duration rather than that of the launch.
+static thread_local hipEvent_t t_evStart = nullptr, t_evStop = nullptr;
+void set_kernel_events(hipEvent_t start, hipEvent_t stop) {
+  t_evStart = start;
+  t_evStop = stop;
+}
+#define FESTUNG_LAUNCH(K_, G_, B_, S_, STREAM_, ...)                                        \
+  do {                                                                                      \
+    if (t_evStart)                                                                          \
+      hipExtLaunchKernelGGL(K_, G_, B_, S_, STREAM_, t_evStart, t_evStop, 0, __VA_ARGS__); \
+    else                                                                                    \
+      hipLaunchKernelGGL(K_, G_, B_, S_, STREAM_, __VA_ARGS__);                             \
+  } while (0)
+
 #define FESTUNG_DISPATCH_P(P_, BODY)                           \
   switch (P_) {                                                \
     case 0: { constexpr int PP = 0; BODY; } break;             \
@@ -1041,9 +1058,9 @@ hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s) {
   if (!valid_elem_block(BS)) return hipErrorInvalidValue;
   dim3 gr = grid_for(a.m.K, BS);
 #define L_STAGE(MODE_)                                                                  \
-  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((stage_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
+  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((stage_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
 #define L_STAGE1(MODE_)                                                                 \
-  FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((stage_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
+  FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((stage_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
   switch (mode) {
     case OUT_RK_DG: L_STAGE(OUT_RK_DG); break;
     case OUT_RK_TAYLOR: L_STAGE1(OUT_RK_TAYLOR); break;
@@ -1057,7 +1074,7 @@ hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_vertex(const VertexArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(vertex_minmax_kernel, grid_for(a.numV), dim3(BLOCK), 0, s, a);
+  FESTUNG_LAUNCH(vertex_minmax_kernel, grid_for(a.numV), dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1065,7 +1082,7 @@ hipError_t launch_limit(int p, int mode, const LimitArgs& a, hipStream_t s) {
   const int BS = a.m.bs;
   if (p < 1 || !valid_elem_block(BS)) return hipErrorInvalidValue;
   dim3 gr = grid_for(a.m.K, BS);
-#define L_LIM(MODE_) FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((limit_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
+#define L_LIM(MODE_) FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((limit_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
   switch (mode) {
     case LIM_SOLUTION: L_LIM(LIM_SOLUTION); break;
     case LIM_DOT_RK_DG: L_LIM(LIM_DOT_RK_DG); break;
@@ -1080,7 +1097,7 @@ hipError_t launch_to_taylor(int p, const ToTaylorArgs& a, hipStream_t s) {
   const int BS = a.bs;
   if (!valid_elem_block(BS)) return hipErrorInvalidValue;
   dim3 gr = grid_for(a.K, BS);
-  FESTUNG_DISPATCH_P1(p, hipLaunchKernelGGL((to_taylor_kernel<PP>), gr, dim3(BS), 0, s, a));
+  FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((to_taylor_kernel<PP>), gr, dim3(BS), 0, s, a));
   return hipGetLastError();
 }
 
@@ -1088,10 +1105,10 @@ hipError_t launch_project(int p, bool ordF, int K, const double* fq, const doubl
                           double* out, hipStream_t s) {
   dim3 gr = grid_for(K);
   if (ordF) {
-    FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((project_kernel<PP, Dims<PP>::RF>), gr, dim3(BLOCK), 0, s, K, fq, tab,
+    FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((project_kernel<PP, Dims<PP>::RF>), gr, dim3(BLOCK), 0, s, K, fq, tab,
                                              off.wF, off.phiF, off.hatMinv, out));
   } else {
-    FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((project_kernel<PP, Dims<PP>::RE>), gr, dim3(BLOCK), 0, s, K, fq, tab,
+    FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((project_kernel<PP, Dims<PP>::RE>), gr, dim3(BLOCK), 0, s, K, fq, tab,
                                              off.wE, off.phiE, off.hatMinv, out));
   }
   return hipGetLastError();
@@ -1099,7 +1116,7 @@ hipError_t launch_project(int p, bool ordF, int K, const double* fq, const doubl
 
 hipError_t launch_vnormal(int K, int Re, const double* nuE0T, const double* u1e, const double* u2e, double* vn,
                           hipStream_t s) {
-  hipLaunchKernelGGL(vnormal_kernel, grid_for(K), dim3(BLOCK), 0, s, K, Re, nuE0T, u1e, u2e, vn);
+  FESTUNG_LAUNCH(vnormal_kernel, grid_for(K), dim3(BLOCK), 0, s, K, Re, nuE0T, u1e, u2e, vn);
   return hipGetLastError();
 }
 
@@ -1108,7 +1125,7 @@ hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* c
   const int BS = m.bs;
   if (!valid_elem_block(BS)) return hipErrorInvalidValue;
   dim3 gr = grid_for(m.K, BS);
-  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((source_kernel<PP>), gr, dim3(BS), 0, s, m, fq, cDe, vn, tab, off, rhs));
+  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((source_kernel<PP>), gr, dim3(BS), 0, s, m, fq, cDe, vn, tab, off, rhs));
   return hipGetLastError();
 }
 
@@ -1116,10 +1133,10 @@ hipError_t launch_l2(int p, bool ordF, int K, const double* B, const double* c, 
                      const TabOff& off, double* partial, hipStream_t s) {
   dim3 gr = grid_for(K);
   if (ordF) {
-    FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((l2_kernel<PP, Dims<PP>::RF>), gr, dim3(BLOCK), 0, s, K, B, c, cq, tab,
+    FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((l2_kernel<PP, Dims<PP>::RF>), gr, dim3(BLOCK), 0, s, K, B, c, cq, tab,
                                              off.wF, off.phiF, partial));
   } else {
-    FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((l2_kernel<PP, Dims<PP>::RE>), gr, dim3(BLOCK), 0, s, K, B, c, cq, tab,
+    FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((l2_kernel<PP, Dims<PP>::RE>), gr, dim3(BLOCK), 0, s, K, B, c, cq, tab,
                                              off.wE, off.phiE, partial));
   }
   return hipGetLastError();
@@ -1135,7 +1152,7 @@ __global__ __launch_bounds__(BLOCK) void permute_kernel(int K, int cols, const d
 }
 
 hipError_t launch_permute(int K, int cols, const double* src, const int* idx, double* dst, hipStream_t s) {
-  hipLaunchKernelGGL(permute_kernel, grid_for(K), dim3(BLOCK), 0, s, K, cols, src, idx, dst);
+  FESTUNG_LAUNCH(permute_kernel, grid_for(K), dim3(BLOCK), 0, s, K, cols, src, idx, dst);
   return hipGetLastError();
 }
 
@@ -1144,13 +1161,13 @@ hipError_t launch_adj_stage(int p, const AdjArgs& a, hipStream_t s) {
   if (!valid_elem_block(BS)) return hipErrorInvalidValue;
   dim3 gr = grid_for(a.m.K, BS);
   switch (a.what) {
-    case ADJ_STAGE: FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_stage_kernel<PP, ADJ_STAGE>), gr, dim3(BS), 0, s, a)); break;
-    case ADJ_FUSED: FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_stage_kernel<PP, ADJ_FUSED>), gr, dim3(BS), 0, s, a)); break;
-    case ADJ_SENS: FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_stage_kernel<PP, ADJ_SENS>), gr, dim3(BS), 0, s, a)); break;
+    case ADJ_STAGE: FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((adj_stage_kernel<PP, ADJ_STAGE>), gr, dim3(BS), 0, s, a)); break;
+    case ADJ_FUSED: FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((adj_stage_kernel<PP, ADJ_FUSED>), gr, dim3(BS), 0, s, a)); break;
+    case ADJ_SENS: FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((adj_stage_kernel<PP, ADJ_SENS>), gr, dim3(BS), 0, s, a)); break;
     case ADJ_SPLIT: {
       const int E = BS < 128 ? BS : 128;
       dim3 g2 = grid_for(a.m.K, E);
-      FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((adj_split_kernel<PP>), g2, dim3(2 * E), 0, s, a));
+      FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((adj_split_kernel<PP>), g2, dim3(2 * E), 0, s, a));
     } break;
     default: return hipErrorInvalidValue;
   }
@@ -1160,20 +1177,20 @@ hipError_t launch_adj_stage(int p, const AdjArgs& a, hipStream_t s) {
 hipError_t launch_loss(int p, int K, const double* B, const double* c, const double* cObs, const double* tab,
                        const TabOff& off, double* part, double* muT, hipStream_t s) {
   dim3 gr = grid_for(K);
-  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((loss_kernel<PP>), gr, dim3(BLOCK), 0, s, K, B, c, cObs, tab, off, part, muT));
+  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((loss_kernel<PP>), gr, dim3(BLOCK), 0, s, K, B, c, cObs, tab, off, part, muT));
   return hipGetLastError();
 }
 
 hipError_t launch_mass(int p, int K, const double* B, const double* x, const double* tab, const TabOff& off,
                        double* out, hipStream_t s) {
   dim3 gr = grid_for(K);
-  FESTUNG_DISPATCH_P(p, hipLaunchKernelGGL((mass_kernel<PP>), gr, dim3(BLOCK), 0, s, K, B, x, tab, off, out));
+  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((mass_kernel<PP>), gr, dim3(BLOCK), 0, s, K, B, x, tab, off, out));
   return hipGetLastError();
 }
 
 hipError_t launch_fill(size_t n, double v, double* x, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(fill_kernel, dim3(unsigned((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, n, v, x);
+  FESTUNG_LAUNCH(fill_kernel, dim3(unsigned((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, n, v, x);
   return hipGetLastError();
 }
 

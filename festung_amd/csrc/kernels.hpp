@@ -181,7 +181,10 @@ hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* c
 hipError_t launch_l2(int p, bool ordF, int K, const double* B, const double* c, const double* cq, const double* tab,
                      const TabOff& off, double* partial, hipStream_t s);
 hipError_t launch_permute(int K, int cols, const double* src, const int* idx, double* dst, hipStream_t s);
-int block_size();          // workgroup size of the reduction / set-up kernels
+int block_size();
+// profiling: events recorded at the start / end of the calling thread's next
+// kernels (null: plain launches)
+void set_kernel_events(hipEvent_t start, hipEvent_t stop);          // workgroup size of the reduction / set-up kernels
 bool valid_elem_block(int bs);  // supported elements per workgroup
 
 }  // namespace festung

@@ -175,9 +175,18 @@ struct festung_problem {
       return;
     }
     Rec r{kind, get_event(), get_event()};
-    HIPCHK(hipEventRecord(r.a, stream));
-    f();
-    HIPCHK(hipEventRecord(r.b, stream));
+    if (kind == FESTUNG_KERNEL_COPY) {
+      HIPCHK(hipEventRecord(r.a, stream));
+      f();
+      HIPCHK(hipEventRecord(r.b, stream));
+    } else {
+      // the kernel's own start / end (not the launch's)
+      struct Reset {
+        ~Reset() { set_kernel_events(nullptr, nullptr); }
+      } reset;
+      set_kernel_events(r.a, r.b);
+      f();
+    }
     recs.push_back(r);
   }
 
