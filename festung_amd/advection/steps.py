@@ -68,8 +68,8 @@ def configureProblem(pd):
     assert pd.hmax > 0, 'Maximum edge length must be positive.'
     assert pd.numSteps > 0, 'Number of time steps must be positive.'
     assert (not pd.isSlopeLim) or pd.p > 0, 'Slope limiting only available for p > 0.'
-    if pd.isSlopeLim and pd.typeSlopeLim != 'linear':
-        raise NotImplementedError('typeSlopeLim %r' % pd.typeSlopeLim)
+    if pd.isSlopeLim and pd.typeSlopeLim not in ('linear', 'hierarch_vert', 'strict'):
+        raise ValueError('Unknown limiter type')
     if pd.testcase == 'custom':
         for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):
             assert k in pd, 'custom testcase needs %s' % k
@@ -92,7 +92,8 @@ def preprocessProblem(pd):
     pd.K = pd.g.numT
     pd.N = (pd.p + 1) * (pd.p + 2) // 2
     pd.tau = pd.tEnd / pd.numSteps
-    pd.problem = Problem(pd.g, pd.p, pd.ordRK, limiter=pd.isSlopeLim, graph=pd.useGraph)
+    pd.problem = Problem(pd.g, pd.p, pd.ordRK, limiter=pd.isSlopeLim, graph=pd.useGraph,
+                         limiter_type=pd.typeSlopeLim if pd.isSlopeLim else 'linear')
     prob = pd.problem
     # physical evaluation points (the closures are evaluated on the host)
     Q1, Q2, _ = quadRule2D(prob.ordE)

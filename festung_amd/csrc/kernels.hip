@@ -210,29 +210,41 @@ __device__ __forceinline__ void lump_rows(const Geo<P>& g, const double* __restr
 }
 
 // ------------------------------------------------------ min / max reduce --
-// Block min / max of the centroid values -> partials[2 blockIdx .. +1].
-// Plain stores: the consumer is a later kernel on the same stream.
-__device__ __forceinline__ void minmax_block_partial(double mn, double mx, const MinMaxSink& s) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    mn = fmin(mn, __shfl_xor(mn, off));
-    mx = fmax(mx, __shfl_xor(mx, off));
-  }
-  __shared__ double smn[BLOCK / 64], smx[BLOCK / 64];
+// Workgroup min / max of the first s.F of the N Taylor coefficients Tc of
+// each active thread -> partials (see MinMaxSink).  Plain stores: the
+// consumer is a later kernel on the same stream.  Every thread of the
+// workgroup must call this (it synchronises the workgroup).
+template <int P>
+__device__ __forceinline__ void sink_taylor(const double (&Tc)[Dims<P>::N], bool active, const MinMaxSink& s) {
+  constexpr int FM = P * (P + 1) / 2 > 1 ? P * (P + 1) / 2 : 1;  // most fields a limiter type needs
+  __shared__ double smn[FM][BLOCK / 64], smx[FM][BLOCK / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) {
-    smn[w] = mn;
-    smx[w] = mx;
+#pragma unroll
+  for (int f = 0; f < FM; ++f) {
+    if (f < s.F) {
+      double mn = active ? Tc[f] : INFINITY, mx = active ? Tc[f] : -INFINITY;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, off));
+        mx = fmax(mx, __shfl_xor(mx, off));
+      }
+      if (lane == 0) {
+        smn[f][w] = mn;
+        smx[f][w] = mx;
+      }
+    }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = smn[0], b = smx[0];
+  const int f = threadIdx.x;
+  if (f < s.F) {
+    double a = smn[f][0], b = smx[f][0];
     for (int i = 1; i < int(blockDim.x) / 64; ++i) {
-      a = fmin(a, smn[i]);
-      b = fmax(b, smx[i]);
+      a = fmin(a, smn[f][i]);
+      b = fmax(b, smx[f][i]);
     }
-    s.partials[2 * blockIdx.x] = a;
-    s.partials[2 * blockIdx.x + 1] = b;
+    const size_t o = 2 * (size_t(f) * gridDim.x + blockIdx.x);
+    s.partials[o] = a;
+    s.partials[o + 1] = b;
   }
 }
 
@@ -378,9 +390,8 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
     if (active) {
 #pragma unroll
       for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
-      a.sink.centroid[k] = Tc[0];
     }
-    minmax_block_partial(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+    sink_taylor<P>(Tc, active, a.sink);
   } else {
     // RK combination (advection/solveSubStep.m:83 for the SSP weights)
     double nx[N];
@@ -396,9 +407,8 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
       if (active) {
 #pragma unroll
         for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
-        a.sink.centroid[k] = Tc[0];
       }
-      minmax_block_partial(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+      sink_taylor<P>(Tc, active, a.sink);
     }
   }
 }
@@ -407,69 +417,77 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 
 __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
   if (blockIdx.x == 0) {
-    // global min / max of the centroid values from the producer's partials;
+    // global min / max of each coefficient from the producer's partials;
     // loads are issued in batches of 8 before they are reduced
-    double mn = INFINITY, mx = -INFINITY;
-    for (int i0 = 0; i0 < a.nPartials; i0 += 8 * BLOCK) {
-      double lo[8], hi[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int i = i0 + u * BLOCK + threadIdx.x;
-        const bool ok = i < a.nPartials;
-        lo[u] = ok ? a.partials[2 * i] : INFINITY;
-        hi[u] = ok ? a.partials[2 * i + 1] : -INFINITY;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        mn = fmin(mn, lo[u]);
-        mx = fmax(mx, hi[u]);
-      }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      mn = fmin(mn, __shfl_xor(mn, off));
-      mx = fmax(mx, __shfl_xor(mx, off));
-    }
     __shared__ double smn[BLOCK / 64], smx[BLOCK / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) {
-      smn[w] = mn;
-      smx[w] = mx;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
+    for (int f = 0; f < a.F; ++f) {
+      const double* part = a.partials + 2 * size_t(f) * a.nPartials;
+      double mn = INFINITY, mx = -INFINITY;
+      for (int i0 = 0; i0 < a.nPartials; i0 += 8 * BLOCK) {
+        double lo[8], hi[8];
 #pragma unroll
-      for (int i = 1; i < BLOCK / 64; ++i) {
-        mn = fmin(mn, smn[i]);
-        mx = fmax(mx, smx[i]);
+        for (int u = 0; u < 8; ++u) {
+          const int i = i0 + u * BLOCK + threadIdx.x;
+          const bool ok = i < a.nPartials;
+          lo[u] = ok ? part[2 * i] : INFINITY;
+          hi[u] = ok ? part[2 * i + 1] : -INFINITY;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          mn = fmin(mn, lo[u]);
+          mx = fmax(mx, hi[u]);
+        }
       }
-      a.gmm[0] = mn;
-      a.gmm[1] = mx;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, off));
+        mx = fmax(mx, __shfl_xor(mx, off));
+      }
+      if (lane == 0) {
+        smn[w] = mn;
+        smx[w] = mx;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 1; i < BLOCK / 64; ++i) {
+          mn = fmin(mn, smn[i]);
+          mx = fmax(mx, smx[i]);
+        }
+        a.gmm[2 * f] = mn;
+        a.gmm[2 * f + 1] = mx;
+      }
+      __syncthreads();  // smn / smx are reused for the next coefficient
     }
   }
   const int v = gtid();
   if (v >= a.numV) return;
-  double m = INFINITY, M = -INFINITY;
   for (int j0 = 0; j0 < a.maxDeg; j0 += 8) {
     int id[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) id[u] = j0 + u < a.maxDeg ? a.v2tPad[(j0 + u) * a.numV + v] : -1;
+    for (int f = 0; f < a.F; ++f) {
+      const double* fld = a.fields + size_t(f) * a.K;
+      double m = j0 == 0 ? INFINITY : a.vmin[size_t(f) * a.numV + v];
+      double M = j0 == 0 ? -INFINITY : a.vmax[size_t(f) * a.numV + v];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (id[u] >= 0) {
-        const double c = a.centroid[id[u]];
-        m = fmin(m, c);
-        M = fmax(M, c);
+      for (int u = 0; u < 8; ++u) {
+        if (id[u] >= 0) {
+          const double c = fld[id[u]];
+          m = fmin(m, c);
+          M = fmax(M, c);
+        }
       }
+      a.vmin[size_t(f) * a.numV + v] = m;
+      a.vmax[size_t(f) * a.numV + v] = M;
     }
   }
-  a.vmin[v] = m;
-  a.vmax[v] = M;
 }
 
 // ----------------------------------------------------------------- limit --
 
-template <int P, int MODE>
+template <int P, int MODE, int LT>
 __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
   constexpr int N = Dims<P>::N;
   const int K = a.m.K;
@@ -494,39 +512,160 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
   // patch bounds with the reference's global shift: min over the patch of
   // (c - sNeg), then + sNeg (and max with sPos); the Dirichlet vertex value
   // joins NaN-ignoring afterwards (core/computeMinMaxV0TElementPatch.m:75-82,107-110)
-  const double sNeg = fabs(a.gmm[1]) + 1.0;
-  const double sPos = fabs(a.gmm[0]) + 1.0;
   double alpha = 1.0;
-#pragma unroll
-  for (int n = 0; n < 3; ++n) {
-    const double val = x[0] + x[1] * (px[n] * g.dX) + x[2] * (py[n] * g.dX);
-    const int v = a.m.V0T[n * K + kk];
-    double lo = (a.vmin[v] - sNeg) + sNeg;
-    double hi = (a.vmax[v] + sPos) - sPos;
-    if (a.vertexD) {
-      const double dv = a.vertexD[v];
-      lo = fmin(lo, dv);
-      hi = fmax(hi, dv);
-    }
-    const double d = val - xc;
-    const double dMin = lo - xc;
-    const double dMax = hi - xc;
-    double al = 1.0;
-    if (d < dMin + tol) al = fmax(0.0, fmin(1.0, dMin / (d - tol)));
-    if (d > dMax - tol) al = fmax(0.0, fmin(1.0, dMax / (d + tol)));
-    alpha = fmin(alpha, al);
-  }
   double lim[N];
-  lim[0] = x[0];
-  lim[1] = alpha * x[1];
-  lim[2] = alpha * x[2];
-  const double keep = alpha == 1.0 ? 1.0 : 0.0;
+  bool changed;
+  if constexpr (LT == LT_LINEAR) {
+    // core/applySlopeLimiterTaylorLinear.m:108-122
+    const double sNeg = fabs(a.gmm[1]) + 1.0;
+    const double sPos = fabs(a.gmm[0]) + 1.0;
 #pragma unroll
-  for (int i = 3; i < N; ++i) lim[i] = keep * x[i];
+    for (int n = 0; n < 3; ++n) {
+      const double val = x[0] + x[1] * (px[n] * g.dX) + x[2] * (py[n] * g.dX);
+      const int v = a.m.V0T[n * K + kk];
+      double lo = (a.vmin[v] - sNeg) + sNeg;
+      double hi = (a.vmax[v] + sPos) - sPos;
+      if (a.vertexD) {
+        const double dv = a.vertexD[v];
+        lo = fmin(lo, dv);
+        hi = fmax(hi, dv);
+      }
+      const double d = val - xc;
+      const double dMin = lo - xc;
+      const double dMax = hi - xc;
+      double al = 1.0;
+      if (d < dMin + tol) al = fmax(0.0, fmin(1.0, dMin / (d - tol)));
+      if (d > dMax - tol) al = fmax(0.0, fmin(1.0, dMax / (d + tol)));
+      alpha = fmin(alpha, al);
+    }
+    lim[0] = x[0];
+    lim[1] = alpha * x[1];
+    lim[2] = alpha * x[2];
+    const double keep = alpha == 1.0 ? 1.0 : 0.0;
+#pragma unroll
+    for (int i = 3; i < N; ++i) lim[i] = keep * x[i];
+    changed = alpha != 1.0;
+  } else {
+    // core/applySlopeLimiterTaylorHierarchicalVertex.m:96-131 and
+    // core/applySlopeLimiterTaylorStrict.m:93-127: for ord = p .. 1 and
+    // i = 1 .. ord, the reconstruction of the derivative coefficient
+    // (ord - i, i - 1) (linear for 'hierarch_vert', of degree p - ord + 1 for
+    // 'strict') is limited against the vertex-patch bounds of that
+    // coefficient; for ord > 1 the element's own vertex value stands in for
+    // the Dirichlet data at Dirichlet vertices.
+    const int fl = a.m.flags[kk];
+    int vtx[3];
+    double xi[3], eta[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      vtx[n] = a.m.V0T[n * K + kk];
+      xi[n] = px[n] * g.dX;
+      eta[n] = py[n] * g.dX;
+    }
+    // Taylor basis at the vertices, core/computeTaylorBasesV0T.m:58-63
+    // (degree >= 2: (xi^a eta^b - mean) / (a! b!); strict only)
+    constexpr int NV = LT == LT_STRICT ? N : 3;
+    double phiV[3][NV];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      phiV[n][0] = 1.0;
+      phiV[n][1] = xi[n];
+      phiV[n][2] = eta[n];
+    }
+    if constexpr (LT == LT_STRICT && P >= 2) {
+      double mlo[N];
+      mlo[0] = 1.0;
+      moments_deg<P, 1>(g, T + a.off.nu, mlo + 1);
+      moments_deg<P, 2>(g, T + a.off.nu, mlo + 3);
+      if constexpr (P >= 3) moments_deg<P, 3>(g, T + a.off.nu, mlo + 6);
+      if constexpr (P >= 4) moments_deg<P, 4>(g, T + a.off.nu, mlo + 10);
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+#pragma unroll
+        for (int m = 3; m < NV; ++m) {
+          const int ea = tdeg(m) - tb(m), eb = tb(m);
+          double pw = 1.0;
+#pragma unroll
+          for (int t = 0; t < ea; ++t) pw *= xi[n];
+#pragma unroll
+          for (int t = 0; t < eb; ++t) pw *= eta[n];
+          phiV[n][m] = (pw - mlo[m]) / tfab(m);
+        }
+      }
+    }
+    if constexpr (LT == LT_HIERARCH) {
+      lim[0] = x[0];
+    } else {
+#pragma unroll
+      for (int m = 0; m < N; ++m) lim[m] = x[m];
+    }
+    double alphaH = 0.0;
+#pragma unroll
+    for (int ord = P; ord >= 1; --ord) {
+      double alphaOrd = 1.0;
+#pragma unroll
+      for (int i = 1; i <= ord; ++i) {
+        const int f = (ord - 1) * ord / 2 + (i - 1);  // coefficient (ord - i, i - 1)
+        const double sNeg = fabs(a.gmm[2 * f + 1]) + 1.0;
+        const double sPos = fabs(a.gmm[2 * f]) + 1.0;
+        const double cen = LT == LT_STRICT ? lim[f] : x[f];
+        double alphaTmp = 1.0;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+          double val;
+          if constexpr (LT == LT_HIERARCH) {
+            val = x[f] + x[ord * (ord + 1) / 2 + (i - 1)] * xi[n] + x[ord * (ord + 1) / 2 + i] * eta[n];
+          } else {
+            const int pR = P - ord + 1, nI = (pR + 1) * (pR + 2) / 2;
+            val = lim[f];
+#pragma unroll
+            for (int m = 1; m < nI; ++m) {
+              const int dm = tdeg(m) + ord - 1, bm = tb(m) + i - 1;
+              val += lim[dm * (dm + 1) / 2 + bm] * phiV[n][m];
+            }
+          }
+          const int v = vtx[n];
+          double lo = (a.vmin[size_t(f) * a.m.numV + v] - sNeg) + sNeg;
+          double hi = (a.vmax[size_t(f) * a.m.numV + v] + sPos) - sPos;
+          if (ord > 1) {
+            if (flag_vdir(fl, n)) {
+              lo = fmin(lo, val);
+              hi = fmax(hi, val);
+            }
+          } else if (a.vertexD) {
+            const double dv = a.vertexD[v];
+            lo = fmin(lo, dv);
+            hi = fmax(hi, dv);
+          }
+          const double d = val - cen;
+          const double dMin = lo - cen;
+          const double dMax = hi - cen;
+          double al = 1.0;
+          if (d < dMin + tol) al = fmax(0.0, fmin(1.0, dMin / (d - tol)));
+          if (d > dMax - tol) al = fmax(0.0, fmin(1.0, dMax / (d + tol)));
+          alphaTmp = fmin(alphaTmp, al);
+        }
+        alphaOrd = fmin(alphaOrd, alphaTmp);
+      }
+      if constexpr (LT == LT_HIERARCH) {
+        alphaH = fmax(alphaH, alphaOrd);
+#pragma unroll
+        for (int m = ord * (ord + 1) / 2; m < (ord + 1) * (ord + 2) / 2; ++m) lim[m] = alphaH * x[m];
+        alpha = alphaH;
+      } else {
+#pragma unroll
+        for (int m = ord * (ord + 1) / 2; m < N; ++m) lim[m] = alphaOrd * lim[m];
+        alpha = alphaOrd;
+      }
+    }
+    changed = false;
+#pragma unroll
+    for (int m = 1; m < N; ++m) changed |= lim[m] != x[m];
+  }
 
   if constexpr (MODE != LIM_SOLUTION && P >= 2) {
     // selective mass lumping: lim + diag(MT)^-1 MT (orig - lim)
-    if (alpha != 1.0) {
+    if (changed) {
       // corr_i = f_i / (mu_2i - mu_i^2) * (sum_j mu_(i+j) d_j - mu_i sum_j mu_j d_j),
       // d_j = (orig_j - lim_j) / f_j, f = a! b!: row i of diag(MT)^-1 MT applied
       // to (orig - lim), with MT the element's Taylor mass matrix
@@ -575,9 +714,8 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
       if (active) {
 #pragma unroll
         for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
-        a.sink.centroid[k] = Tc[0];
       }
-      minmax_block_partial(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+      sink_taylor<P>(Tc, active, a.sink);
     }
   }
 }
@@ -600,9 +738,8 @@ __global__ __launch_bounds__(BLOCK) void to_taylor_kernel(ToTaylorArgs a) {
   if (active) {
 #pragma unroll
     for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
-    a.sink.centroid[k] = Tc[0];
   }
-  minmax_block_partial(active ? Tc[0] : INFINITY, active ? Tc[0] : -INFINITY, a.sink);
+  sink_taylor<P>(Tc, active, a.sink);
 }
 
 // ------------------------------------------------------ setup kernels --
@@ -1078,18 +1215,27 @@ hipError_t launch_vertex(const VertexArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_limit(int p, int mode, const LimitArgs& a, hipStream_t s) {
+hipError_t launch_limit(int p, int mode, int type, const LimitArgs& a, hipStream_t s) {
   const int BS = a.m.bs;
   if (p < 1 || !valid_elem_block(BS)) return hipErrorInvalidValue;
   dim3 gr = grid_for(a.m.K, BS);
-#define L_LIM(MODE_) FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((limit_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
+#define L_LIM(MODE_, LT_) \
+  FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((limit_kernel<PP, MODE_, LT_>), gr, dim3(BS), 0, s, a))
+#define L_LIM_T(MODE_)                                 \
+  switch (type) {                                      \
+    case LT_LINEAR: L_LIM(MODE_, LT_LINEAR); break;     \
+    case LT_HIERARCH: L_LIM(MODE_, LT_HIERARCH); break; \
+    case LT_STRICT: L_LIM(MODE_, LT_STRICT); break;     \
+    default: return hipErrorInvalidValue;              \
+  }
   switch (mode) {
-    case LIM_SOLUTION: L_LIM(LIM_SOLUTION); break;
-    case LIM_DOT_RK_DG: L_LIM(LIM_DOT_RK_DG); break;
-    case LIM_DOT_RK_TAYLOR: L_LIM(LIM_DOT_RK_TAYLOR); break;
+    case LIM_SOLUTION: L_LIM_T(LIM_SOLUTION); break;
+    case LIM_DOT_RK_DG: L_LIM_T(LIM_DOT_RK_DG); break;
+    case LIM_DOT_RK_TAYLOR: L_LIM_T(LIM_DOT_RK_TAYLOR); break;
     default: return hipErrorInvalidValue;
   }
 #undef L_LIM
+#undef L_LIM_T
   return hipGetLastError();
 }
 

@@ -33,10 +33,12 @@ struct TabOff {
 
 // Per-element flag word: bits 2n..2n+1 = neighbour's local edge across
 // local edge n; bit 6+n = outflow (diagonal) term active on edge n; bit 9+n =
-// Dirichlet inflow on edge n.
+// Dirichlet inflow on edge n; bit 12+n = local vertex n lies on a Dirichlet
+// edge (markV0TbdrD).
 __host__ __device__ inline int flag_np(int f, int n) { return (f >> (2 * n)) & 3; }
 __host__ __device__ inline bool flag_out(int f, int n) { return (f >> (6 + n)) & 1; }
 __host__ __device__ inline bool flag_dir(int f, int n) { return (f >> (9 + n)) & 1; }
+__host__ __device__ inline bool flag_vdir(int f, int n) { return (f >> (12 + n)) & 1; }
 
 // Coefficients of one explicit RK stage:
 //   out = a0*c0 + a1*c + b*tau*cdot + g*acc
@@ -45,12 +47,16 @@ struct RKCoef {
   double a0, a1, b, g, d0, d1, e;
 };
 
-// Centroid values of one limiter application and their per-block min / max
-// (partials[2b], partials[2b+1]); the vertex kernel, launched after the
-// producer, folds the partials into the global min / max.
+// Per-workgroup min / max of the first F Taylor coefficients a producer
+// kernel writes (F = 1: the centroid values; more for the hierarchical and
+// strict limiters, whose vertex bounds are taken over derivative
+// coefficients too): partials[2 (f nblk + b)], partials[2 (f nblk + b) + 1].
+// The vertex kernel, launched after the producer, folds them into the global
+// min / max per coefficient and reads the coefficients themselves from the
+// producer's Taylor output.
 struct MinMaxSink {
-  double* centroid;   // K
-  double* partials;   // 2 x gridDim.x of the producer
+  double* partials;
+  int F;
 };
 
 struct Mesh {
@@ -66,6 +72,10 @@ struct Mesh {
 enum StageOut { OUT_RK_DG = 0, OUT_RK_TAYLOR = 1, OUT_DOT_TAYLOR = 2, OUT_DOT_DG = 3 };
 enum AdjWhat { ADJ_STAGE = 0, ADJ_FUSED = 1, ADJ_SENS = 2, ADJ_SPLIT = 3 };
 enum LimitMode { LIM_SOLUTION = 0, LIM_DOT_RK_DG = 1, LIM_DOT_RK_TAYLOR = 2 };
+// Limiter types of core/applySlopeLimiterTaylor.m:69-85
+enum LimiterType { LT_LINEAR = 0, LT_HIERARCH = 1, LT_STRICT = 2 };
+// Taylor coefficients whose vertex-patch bounds a limiter type needs
+__host__ __device__ inline int limiter_fields(int type, int p) { return type == LT_LINEAR ? 1 : p * (p + 1) / 2; }
 
 struct StageArgs {
   Mesh m;
@@ -90,20 +100,22 @@ struct VertexArgs {
   int numV;
   int maxDeg;            // columns of the padded vertex -> element lists
   const int* v2tPad;     // maxDeg x numV, element ids (device order), -1 padded
-  const double* centroid;
-  const double* partials;  // producer's per-block min / max
+  const double* fields;  // Taylor coefficients (N x K); the first F are reduced
+  int K;
+  int F;
+  const double* partials;  // producer's per-block min / max, F x nPartials x 2
   int nPartials;
-  double* gmm;             // out: global min, max (written by block 0)
-  double* vmin;            // out: raw patch min / max per vertex
+  double* gmm;             // out: global min, max per coefficient (2F, written by block 0)
+  double* vmin;            // out: raw patch min / max per coefficient and vertex (F x numV)
   double* vmax;
 };
 
 struct LimitArgs {
   Mesh m;
   const double* taylor;  // Taylor coefficients to limit (N x K)
-  const double* vmin;    // raw patch min / max per vertex
+  const double* vmin;    // raw patch min / max per coefficient and vertex (F x numV)
   const double* vmax;
-  const double* gmm;     // global min / max of the centroid values
+  const double* gmm;     // global min / max per coefficient (2F)
   const double* vertexD; // Dirichlet vertex values (NaN = none) or null
   const double* tab;
   TabOff off;
@@ -170,7 +182,7 @@ hipError_t launch_mass(int p, int K, const double* B, const double* x, const dou
                        double* out, hipStream_t s);
 hipError_t launch_fill(size_t n, double v, double* x, hipStream_t s);
 hipError_t launch_vertex(const VertexArgs& a, hipStream_t s);
-hipError_t launch_limit(int p, int mode, const LimitArgs& a, hipStream_t s);
+hipError_t launch_limit(int p, int mode, int type, const LimitArgs& a, hipStream_t s);
 hipError_t launch_to_taylor(int p, const ToTaylorArgs& a, hipStream_t s);
 hipError_t launch_project(int p, bool ordF, int K, const double* fq, const double* tab, const TabOff& off,
                           double* out, hipStream_t s);

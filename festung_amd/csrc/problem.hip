@@ -138,7 +138,9 @@ struct festung_problem {
   int maxDeg = 0;
   DBuf tauK;
   bool hasTauK = false;
-  DBuf X, Y, Z, acc, T1, T2, cent, partials, gmm, vmin, vmax, vertexD, u1, u2, vn, rhs, alpha;
+  DBuf X, Y, Z, acc, T1, T2, partials, gmm, vmin, vmax, vertexD, u1, u2, vn, rhs, alpha;
+  int limType = LT_LINEAR;  // limiter type (core/applySlopeLimiterTaylor.m:69-85)
+  int nF = 1;               // Taylor coefficients with vertex-patch bounds (limiter_fields)
   bool hasRhs = false, hasVertexD = false, hasVelocity = false;
   int cur = 0;           // buffer holding the current state (0 = X)
   int nextStage = -1;    // next sub-step of a step begun with festung_begin_step
@@ -195,7 +197,7 @@ struct festung_problem {
     return Mesh{K, numV, B.d(), area.d(), nbr.i(), flg.i(), V0T.i(), bs};
   }
   MinMaxSink sink() const {
-    return MinMaxSink{cent.d(), partials.d()};
+    return MinMaxSink{partials.d(), nF};
   }
   void set_device() const { HIPCHK(hipSetDevice(device)); }
   void drop_graph() {
@@ -255,10 +257,11 @@ struct festung_problem {
     from_dev(v.data(), cols, dst);
   }
 
-  // raw vertex-patch min / max of the centroid values just written to `cent`,
-  // and the global min / max from the producer's per-block partials
-  void enqueue_vertex() {
-    VertexArgs va{numV, maxDeg, v2tPad.i(), cent.d(), partials.d(), nblocks, gmm.d(), vmin.d(), vmax.d()};
+  // raw vertex-patch min / max of the first nF Taylor coefficients just
+  // written to `fields` (the centroid values first), and their global min /
+  // max from the producer's per-block partials
+  void enqueue_vertex(const double* fields) {
+    VertexArgs va{numV, maxDeg, v2tPad.i(), fields, K, nF, partials.d(), nblocks, gmm.d(), vmin.d(), vmax.d()};
     timed(FESTUNG_KERNEL_VERTEX, [&] { HIPCHK(launch_vertex(va, stream)); });
   }
 
@@ -305,7 +308,7 @@ struct festung_problem {
       // solution limiting (:86-97)
       a.out = T1.d();
       timed(FESTUNG_KERNEL_STAGE, [&] { HIPCHK(launch_stage(p, OUT_DOT_TAYLOR, a, stream)); });
-      enqueue_vertex();
+      enqueue_vertex(T1.d());
       l.taylor = T1.d();
       l.c = cin;
       l.c0 = X.d();
@@ -316,17 +319,17 @@ struct festung_problem {
       l.rk = sp.rk;
       l.out = T2.d();
       l.alphaOut = nullptr;
-      timed(FESTUNG_KERNEL_LIMIT_DOT, [&] { HIPCHK(launch_limit(p, LIM_DOT_RK_TAYLOR, l, stream)); });
+      timed(FESTUNG_KERNEL_LIMIT_DOT, [&] { HIPCHK(launch_limit(p, LIM_DOT_RK_TAYLOR, limType, l, stream)); });
     } else {
       a.out = T2.d();
       timed(FESTUNG_KERNEL_STAGE, [&] { HIPCHK(launch_stage(p, OUT_RK_TAYLOR, a, stream)); });
     }
-    enqueue_vertex();
+    enqueue_vertex(T2.d());
     l.taylor = T2.d();
     l.out = cout;
     l.vertexD = hasVertexD ? vertexD.d() : nullptr;
     l.alphaOut = keepAlpha ? alpha.d() : nullptr;
-    timed(FESTUNG_KERNEL_LIMIT, [&] { HIPCHK(launch_limit(p, LIM_SOLUTION, l, stream)); });
+    timed(FESTUNG_KERNEL_LIMIT, [&] { HIPCHK(launch_limit(p, LIM_SOLUTION, limType, l, stream)); });
   }
 
   void enqueue_step(double tau) {
@@ -482,6 +485,7 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
         if (hasNb) fl[d] |= (g.nbrEdge[kn] & 3) << (2 * n);
         if (notN && markElemEdges) fl[d] |= 1 << (6 + n);
         if (g.markE0TbdrD[kn]) fl[d] |= 1 << (9 + n);
+        if (g.markV0TbdrD[kn]) fl[d] |= 1 << (12 + n);
       }
     h->upload(h->nbr, nb.data(), nb.size());
     h->upload(h->flg, fl.data(), fl.size());
@@ -501,19 +505,21 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     h->bs = K >= 1024 * 256 ? 256 : K >= 1024 * 128 ? 128 : 64;
     h->nblocks = (K + h->bs - 1) / h->bs;
     h->nred = (K + block_size() - 1) / block_size();
+    if ((flags & FESTUNG_FLAG_LIM_HIERARCH) && (flags & FESTUNG_FLAG_LIM_STRICT))
+      fail(FESTUNG_ERR_ARG, "choose one limiter type");
+    h->limType = (flags & FESTUNG_FLAG_LIM_HIERARCH) ? LT_HIERARCH : (flags & FESTUNG_FLAG_LIM_STRICT) ? LT_STRICT : LT_LINEAR;
+    h->nF = limiter_fields(h->limType, p);
     if (flags & FESTUNG_FLAG_LIMITER) {
       h->T1.alloc(nk);
       h->T2.alloc(nk);
-      h->cent.alloc(size_t(K) * sizeof(double));
       h->alpha.alloc(size_t(K) * sizeof(double));
-      h->vmin.alloc(size_t(g.numV) * sizeof(double));
-      h->vmax.alloc(size_t(g.numV) * sizeof(double));
+      h->vmin.alloc(size_t(h->nF) * g.numV * sizeof(double));
+      h->vmax.alloc(size_t(h->nF) * g.numV * sizeof(double));
       h->vertexD.alloc(size_t(g.numV) * sizeof(double));
-    } else {
-      h->cent.alloc(size_t(K) * sizeof(double));
     }
-    h->partials.alloc(size_t(2) * ((K + 63) / 64 + 1) * sizeof(double));  // room for 64 elements per workgroup
-    h->gmm.alloc(2 * sizeof(double));
+    // room for 64 elements per workgroup
+    h->partials.alloc(size_t(2) * h->nF * ((K + 63) / 64 + 1) * sizeof(double));
+    h->gmm.alloc(2 * size_t(h->nF) * sizeof(double));
     HIPCHK(hipDeviceSynchronize());
     *out = h.release();
   });
@@ -752,7 +758,7 @@ int festung_limit(festung_problem* h) {
     h->finish_stage_sequence();
     ToTaylorArgs ta{h->K, h->B.d(), h->X.d(), h->tab.d(), h->off, h->T2.d(), h->sink(), h->bs};
     HIPCHK(launch_to_taylor(h->p, ta, h->stream));
-    h->enqueue_vertex();
+    h->enqueue_vertex(h->T2.d());
     LimitArgs l{};
     l.m = h->mesh();
     l.taylor = h->T2.d();
@@ -765,7 +771,7 @@ int festung_limit(festung_problem* h) {
     l.out = h->X.d();
     l.sink = h->sink();
     l.alphaOut = h->alpha.d();
-    HIPCHK(launch_limit(h->p, LIM_SOLUTION, l, h->stream));
+    HIPCHK(launch_limit(h->p, LIM_SOLUTION, h->limType, l, h->stream));
     sync(h);
     h->cur = 0;
   });

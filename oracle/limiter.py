@@ -3,15 +3,21 @@ core/phiTaylorPhy.m, core/phiTaylorRef.m, core/computeTaylorBasesV0T.m,
 core/assembleMatElemPhiTaylorPhiTaylor.m, core/assembleMatElemPhiDiscPhiTaylor.m,
 core/projectDataDisc2DataTaylor.m, core/projectDataTaylor2DataDisc.m,
 core/computeMinMaxV0TElementPatch.m, core/computeVertexBasedLimiter.m,
-core/applySlopeLimiterTaylorLinear.m, core/applySlopeLimiterTaylor.m and
-core/applySlopeLimiterDisc.m.
+core/applySlopeLimiterTaylorLinear.m,
+core/applySlopeLimiterTaylorHierarchicalVertex.m,
+core/applySlopeLimiterTaylorStrict.m, core/multiindex.m, core/mult2ind.m,
+core/applySlopeLimiterTaylor.m and core/applySlopeLimiterDisc.m.
 
 Pinned by: brute-force patch min/max on small meshes (dense markV0TV0T path),
 the invariants of the limiter (cell means unchanged; alpha in [0, 1]; linear
 data that respects the patch bounds is untouched), the exactness of the
 Disc <-> Taylor change of basis on polynomials (round trip is the identity to
 rounding, and Taylor coefficients equal centroid derivatives scaled by dX),
-and the mean-free property of the Taylor basis.
+and the mean-free property of the Taylor basis.  The hierarchical and strict
+types are pinned by their reduction to the linear limiter at p = 1, by a
+loop-based evaluation on small meshes that forms every patch, vertex value
+and Taylor basis value directly from coordinates, and by their invariants
+(cell means unchanged, data inside its bounds untouched).
 """
 import numpy as np
 import scipy.sparse as sp
@@ -180,11 +186,79 @@ def applySlopeLimiterTaylorLinear(g, dataTaylor, markV0TbdrD, dataV0T, basesOnQu
     return lim, minMax
 
 
+def multiindex(p):
+    """Multi-indices (a, b) of total degree <= p in the Taylor order;
+    core/multiindex.m:52-61."""
+    mult = [(0, 0)]
+    for o in range(1, p + 1):
+        for i in range(1, o + 2):
+            mult.append((o - i + 1, i - 1))
+    return np.array(mult, dtype=int)
+
+
+def mult2ind(a):
+    """1-based linear Taylor index of multi-indices; core/mult2ind.m:49-53."""
+    a = np.atleast_2d(a)
+    p = a.sum(axis=1)
+    return p * (p + 1) // 2 + 1 + a[:, 1]
+
+
+def applySlopeLimiterTaylorHierarchicalVertex(g, dataTaylor, markV0TbdrD, dataV0T, basesOnQuad):
+    """core/applySlopeLimiterTaylorHierarchicalVertex.m:96-131: for ord = p..1
+    the linear reconstructions of the degree-(ord-1) coefficients give
+    alphaOrd; alpha = max(alpha, alphaOrd) scales the degree-ord block."""
+    K, N = dataTaylor.shape
+    p = p_of_N(N)
+    lim = np.zeros_like(dataTaylor)
+    lim[:, 0] = dataTaylor[:, 0]
+    alpha = np.zeros(K)
+    minMax = None
+    for o in range(p, 0, -1):
+        alphaOrd = np.ones(K)
+        indDOF = np.arange(o * (o + 1) // 2, (o + 1) * (o + 2) // 2)
+        for i in range(1, o + 1):
+            ind = mult2ind(np.array([o - i, i - 1]) + multiindex(1)) - 1
+            valV0T = computeFuncDiscAtPoints(dataTaylor[:, ind], basesOnQuad['phiTaylorV0T'][:, :, 0:3])
+            dV = valV0T if o > 1 else dataV0T
+            alphaTmp, minMax = computeVertexBasedLimiter(g, dataTaylor[:, ind[0]], valV0T, markV0TbdrD, dV)
+            alphaOrd = np.minimum(alphaOrd, alphaTmp)
+        alpha = np.maximum(alpha, alphaOrd)
+        lim[:, indDOF] = alpha[:, None] * dataTaylor[:, indDOF]
+    return lim, minMax
+
+
+def applySlopeLimiterTaylorStrict(g, dataTaylor, markV0TbdrD, dataV0T, basesOnQuad):
+    """core/applySlopeLimiterTaylorStrict.m:93-127: for ord = p..1 the
+    reconstructions of degree p - ord + 1 about the degree-(ord-1)
+    coefficients (from the data limited so far) give alphaOrd, which scales
+    every coefficient of degree >= ord."""
+    K, N = dataTaylor.shape
+    p = p_of_N(N)
+    lim = dataTaylor.copy()
+    minMax = None
+    for o in range(p, 0, -1):
+        alphaOrd = np.ones(K)
+        indDOF = np.arange(o * (o + 1) // 2, N)
+        for i in range(1, o + 1):
+            pR = p - o + 1
+            ind = mult2ind(np.array([o - i, i - 1]) + multiindex(pR)) - 1
+            valV0T = computeFuncDiscAtPoints(lim[:, ind], basesOnQuad['phiTaylorV0T'][:, :, 0:len(ind)])
+            dV = valV0T if o > 1 else dataV0T
+            alphaTmp, minMax = computeVertexBasedLimiter(g, lim[:, ind[0]], valV0T, markV0TbdrD, dV)
+            alphaOrd = np.minimum(alphaOrd, alphaTmp)
+        lim[:, indDOF] = alphaOrd[:, None] * lim[:, indDOF]
+    return lim, minMax
+
+
 def applySlopeLimiterTaylor(g, dataTaylor, markV0TbdrD, dataV0T, basesOnQuad, type_='linear'):
-    """core/applySlopeLimiterTaylor.m:69-85 ('linear' only)."""
-    if type_ != 'linear':
-        raise NotImplementedError('limiter type %r' % type_)
-    return applySlopeLimiterTaylorLinear(g, dataTaylor, markV0TbdrD, dataV0T, basesOnQuad)
+    """core/applySlopeLimiterTaylor.m:69-85."""
+    if type_ == 'linear':
+        return applySlopeLimiterTaylorLinear(g, dataTaylor, markV0TbdrD, dataV0T, basesOnQuad)
+    if type_ == 'hierarch_vert':
+        return applySlopeLimiterTaylorHierarchicalVertex(g, dataTaylor, markV0TbdrD, dataV0T, basesOnQuad)
+    if type_ == 'strict':
+        return applySlopeLimiterTaylorStrict(g, dataTaylor, markV0TbdrD, dataV0T, basesOnQuad)
+    raise ValueError('Unknown limiter type')
 
 
 def applySlopeLimiterDisc(g, dataDisc, markV0TbdrD, dataV0T, globM, globMDiscTaylor, basesOnQuad, type_='linear'):

@@ -13,10 +13,11 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / (n if n > 0 else 1.0)
 
 
-def oracle_problem(tc, p, hmax, ordRK, lim, numSteps=10, tEnd=None):
+def oracle_problem(tc, p, hmax, ordRK, lim, numSteps=10, tEnd=None, typeSlopeLim='linear'):
     """configure + preprocess + initialize in the oracle with the closures of
     the product testcase ``tc`` (a festung_amd.advection.testcases.Testcase)."""
-    pd = OA.Struct(testcase='custom', p=p, hmax=hmax, ordRK=ordRK, isSlopeLim=lim, numSteps=numSteps)
+    pd = OA.Struct(testcase='custom', p=p, hmax=hmax, ordRK=ordRK, isSlopeLim=lim, numSteps=numSteps,
+                   typeSlopeLim=typeSlopeLim)
     if tEnd is not None:
         pd.tEnd = tEnd
     for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):

@@ -16,9 +16,9 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def product_problem(tc, p, hmax, ordRK, lim, numSteps=10, tEnd=None, graph=False):
+def product_problem(tc, p, hmax, ordRK, lim, numSteps=10, tEnd=None, graph=False, typeSlopeLim='linear'):
     pd = S.Struct(testcase='custom', p=p, hmax=hmax, ordRK=ordRK, isSlopeLim=lim, numSteps=numSteps,
-                  useGraph=graph)
+                  useGraph=graph, typeSlopeLim=typeSlopeLim)
     if tEnd is not None:
         pd.tEnd = tEnd
     pd.update(tc)
@@ -123,6 +123,40 @@ def test_limiter_alone(gpu, p):
     assert rel_l2(got, ref) < TOL
     assert np.max(np.abs(gmn - mn)) < 1e-14 and np.max(np.abs(gmx - mx)) < 1e-14
     assert np.all((alpha >= 0) & (alpha <= 1))
+
+
+@pytest.mark.parametrize('type_', ['hierarch_vert', 'strict'])
+@pytest.mark.parametrize('p', [1, 2, 3, 4])
+def test_limiter_types_alone(gpu, type_, p):
+    """applySlopeLimiterDisc with the 'hierarch_vert' and 'strict' types
+    (vertex-patch bounds of every derivative coefficient below degree p)."""
+    tc = TC.rotating_humps()
+    o = oracle_problem(tc, p, 1 / 8, min(p + 1, 3), True, typeSlopeLim=type_)
+    d = product_problem(tc, p, 1 / 8, min(p + 1, 3), True, typeSlopeLim=type_)
+    rng = np.random.default_rng(41 + p)
+    c = o.cDisc + 0.05 * rng.standard_normal(o.cDisc.shape)
+    ref, (mn, mx) = oracle_limit(o, c, 0.0)
+    d.problem.set_state(c)
+    d.problem.limit()
+    got = d.problem.get_state()
+    alpha, gmn, gmx = d.problem.limiter_info()
+    assert rel_l2(got, ref) < TOL
+    assert np.max(np.abs(gmn - mn)) < 1e-14 and np.max(np.abs(gmx - mx)) < 1e-14
+    assert np.all((alpha >= 0) & (alpha <= 1))
+
+
+@pytest.mark.parametrize('type_', ['hierarch_vert', 'strict'])
+@pytest.mark.parametrize('p,ordRK', [(2, 3), (3, 3), (4, 4), (2, 2)])
+def test_limiter_types_step(gpu, type_, p, ordRK):
+    """One full step with dc/dt and solution limiting of the given type."""
+    tc = TC.rotating_humps()
+    o = oracle_problem(tc, p, 1 / 8, ordRK, True, numSteps=20, tEnd=0.5, typeSlopeLim=type_)
+    d = product_problem(tc, p, 1 / 8, ordRK, True, numSteps=20, tEnd=0.5, typeSlopeLim=type_)
+    c = o.cDisc.copy()
+    d.problem.set_state(c)
+    got = product_step(d, 1)
+    ref = oracle_step(o, c, 1)
+    assert rel_l2(got, ref) < TOL
 
 
 def test_determinism(gpu):

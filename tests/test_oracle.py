@@ -479,6 +479,103 @@ def test_limiter_keeps_linear_data_inside_bounds():
     assert np.all(alpha[inside] == 1)
 
 
+def test_multiindex_and_mult2ind():
+    """core/multiindex.m order (degree, then the second exponent) and its
+    inverse core/mult2ind.m."""
+    m = L.multiindex(4)
+    assert [tuple(r) for r in m] == L.TAYLOR_EXP
+    assert np.array_equal(L.mult2ind(m), np.arange(1, 16))
+
+
+@pytest.mark.parametrize('type_', ['hierarch_vert', 'strict'])
+def test_limiter_types_reduce_to_linear_at_p1(type_):
+    pd = _setup(1, h=1 / 5, lim=True)
+    rng = np.random.default_rng(21)
+    t = rng.standard_normal((pd.K, pd.N))
+    cD = np.where(pd.g.markV0TbdrD, rng.standard_normal((pd.K, 3)), np.nan)
+    a, mma = L.applySlopeLimiterTaylor(pd.g, t, pd.g.markV0TbdrD, cD, pd.basesOnQuad, 'linear')
+    b, mmb = L.applySlopeLimiterTaylor(pd.g, t, pd.g.markV0TbdrD, cD, pd.basesOnQuad, type_)
+    assert np.max(np.abs(a - b)) < 1e-14        # same algorithm; only the summation order may differ
+    assert np.array_equal(mma[0], mmb[0]) and np.array_equal(mma[1], mmb[1])
+
+
+def _limit_by_loops(g, t, markD, cD, type_):
+    """The hierarchical / strict vertex-based limiters evaluated element by
+    element: patches by searching V0T, Taylor basis values at the vertices
+    from the coordinates (monomials about the centroid, means by a 64-point
+    rule), bounds without the shift trick."""
+    K, N = t.shape
+    p = int(round((np.sqrt(8 * N + 1) - 3) / 2))
+    exps = L.TAYLOR_EXP[:N]
+    q1, q2, qw = Q.quadRule2D(8)
+    patches = {v: [k for k in range(K) if v in g.V0T[k]] for v in range(g.numV)}
+
+    def basis_at_vertices(k):
+        c = g.coordV[g.V0T[k]]
+        xc = c.mean(axis=0)
+        dX = 2 / np.sqrt(2 * g.areaT[k])
+        P = c[0] + np.outer(q1, c[1] - c[0]) + np.outer(q2, c[2] - c[0])   # rule points in the element
+        out = np.zeros((3, N))
+        for m, (a, b) in enumerate(exps):
+            f = math.factorial(a) * math.factorial(b)
+            mean = 2 * np.sum(qw * ((P[:, 0] - xc[0]) * dX) ** a * ((P[:, 1] - xc[1]) * dX) ** b)
+            for n in range(3):
+                val = ((c[n, 0] - xc[0]) * dX) ** a * ((c[n, 1] - xc[1]) * dX) ** b
+                out[n, m] = (val - (mean if a + b >= 2 else 0.0)) / f
+        return out
+
+    phiV = [basis_at_vertices(k) for k in range(K)]
+    ind = lambda a, b: (a + b) * (a + b + 1) // 2 + b
+    lim = t.copy() if type_ == 'strict' else np.zeros_like(t)
+    if type_ == 'hierarch_vert':
+        lim[:, 0] = t[:, 0]
+    alphaH = np.zeros(K)
+    for o in range(p, 0, -1):
+        alphaOrd = np.ones(K)
+        for i in range(1, o + 1):
+            a0, b0 = o - i, i - 1
+            src = t if type_ == 'hierarch_vert' else lim
+            pR = 1 if type_ == 'hierarch_vert' else p - o + 1
+            rec = [(a0 + a, b0 + b) for (a, b) in exps[:(pR + 1) * (pR + 2) // 2]]
+            f0 = ind(a0, b0)
+            for k in range(K):
+                al = 1.0
+                for n in range(3):
+                    val = sum(src[k, ind(*e)] * phiV[k][n, m] for m, e in enumerate(rec))
+                    patch = patches[g.V0T[k, n]]
+                    lo, hi = min(src[j, f0] for j in patch), max(src[j, f0] for j in patch)
+                    if markD[k, n]:
+                        dv = val if o > 1 else cD[k, n]
+                        if not np.isnan(dv):
+                            lo, hi = min(lo, dv), max(hi, dv)
+                    d, dMin, dMax = val - src[k, f0], lo - src[k, f0], hi - src[k, f0]
+                    if d < dMin + 1e-8:
+                        al = min(al, max(0.0, min(1.0, dMin / (d - 1e-8))))
+                    if d > dMax - 1e-8:
+                        al = min(al, max(0.0, min(1.0, dMax / (d + 1e-8))))
+                alphaOrd[k] = min(alphaOrd[k], al)
+        if type_ == 'hierarch_vert':
+            alphaH = np.maximum(alphaH, alphaOrd)
+            blk = slice(o * (o + 1) // 2, (o + 1) * (o + 2) // 2)
+            lim[:, blk] = alphaH[:, None] * t[:, blk]
+        else:
+            lim[:, o * (o + 1) // 2:] *= alphaOrd[:, None]
+    return lim
+
+
+@pytest.mark.parametrize('type_,p', [('hierarch_vert', 2), ('hierarch_vert', 3), ('strict', 2), ('strict', 3)])
+def test_limiter_types_vs_loop_evaluation(type_, p):
+    pd = _setup(p, h=1 / 3, lim=True)
+    rng = np.random.default_rng(30 + p)
+    t = rng.standard_normal((pd.K, pd.N)) * np.array([1.0] + [0.5] * (pd.N - 1))
+    cD = np.where(pd.g.markV0TbdrD, rng.standard_normal((pd.K, 3)), np.nan)
+    got, _ = L.applySlopeLimiterTaylor(pd.g, t, pd.g.markV0TbdrD, cD, pd.basesOnQuad, type_)
+    want = _limit_by_loops(pd.g, t, pd.g.markV0TbdrD, cD, type_)
+    assert np.max(np.abs(got - want)) < 1e-11
+    assert np.array_equal(got[:, 0], t[:, 0])                # cell means unchanged
+    assert np.all(np.abs(got) <= np.abs(t) + 1e-15)          # every coefficient scaled by a factor in [0, 1]
+
+
 # -------------------------------------------------------------------- EOC --
 @pytest.mark.slow
 def test_convergence_order_transient():
