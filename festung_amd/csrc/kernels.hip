@@ -315,7 +315,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 
   // element term  (G1 + G2) c  in quadrature form on quadRule2D(max(2p,1))
   if constexpr (P >= 1) {
-    constexpr int UQ = RE <= 16 ? RE : 4;
+    constexpr int UQ = RE <= 6 ? RE : 1;  // p >= 3: not unrolled (measured fastest: fewer VGPRs, LDS/scalar loads overlap)
 #pragma unroll UQ
     for (int q = 0; q < RE; ++q) {
       double cq = 0.0, u1q = 0.0, u2q = 0.0;
@@ -917,7 +917,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
     for (int i = 0; i < 3 * Re; ++i) gvn[i] = 0.0;
   }
   if constexpr (P >= 1) {
-    constexpr int UQ = RE <= 16 ? RE : 4;
+    constexpr int UQ = RE <= 6 ? RE : 1;
 #pragma unroll UQ
     for (int q = 0; q < RE; ++q) {
       double m1 = 0.0, m2 = 0.0, u1q = 0.0, u2q = 0.0, cq = 0.0;
