@@ -315,7 +315,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 
   // element term  (G1 + G2) c  in quadrature form on quadRule2D(max(2p,1))
   if constexpr (P >= 1) {
-    constexpr int UQ = RE <= 6 ? RE : 1;  // p >= 3: not unrolled (measured fastest: fewer VGPRs, LDS/scalar loads overlap)
+    constexpr int UQ = RE <= 6 ? RE : 1;  // p >= 3: not unrolled (measured fastest at p = 3, 4)
 #pragma unroll UQ
     for (int q = 0; q < RE; ++q) {
       double cq = 0.0, u1q = 0.0, u2q = 0.0;
