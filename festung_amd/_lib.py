@@ -61,6 +61,9 @@ SIGNATURES = {
     'festung_set_velocity_points': (C.c_int, [vp, dp, dp, dp, dp]),
     'festung_set_source_points': (C.c_int, [vp, dp, dp]),
     'festung_set_vertex_dirichlet': (C.c_int, [vp, dp]),
+    'festung_set_velocity_points_dev': (C.c_int, [vp, vp, vp, vp, vp]),
+    'festung_set_source_points_dev': (C.c_int, [vp, vp, vp]),
+    'festung_set_vertex_dirichlet_dev': (C.c_int, [vp, vp]),
     'festung_project': (C.c_int, [vp, C.c_int, dp, dp, C.c_int]),
     'festung_set_state': (C.c_int, [vp, dp]),
     'festung_get_state': (C.c_int, [vp, dp]),

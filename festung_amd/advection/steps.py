@@ -108,7 +108,50 @@ def preprocessProblem(pd):
     pd.XEdge = (X1, X2)
     pd.dirichletVertices = np.unique(pd.g.V0T[pd.g.markV0TbdrD])
     pd.velocitySet = False
+    pd.deviceEval = _device_points(pd) if pd.get('deviceEval', True) else None
     return pd
+
+
+def _device_points(pd):
+    """Evaluation points on the GPU (torch, [R][K] layouts of the C-ABI) when
+    the closures that are evaluated every stage accept torch tensors; then
+    time-dependent data are computed and handed over on the device.  None
+    when torch has no GPU or a closure needs numpy arrays (the host evaluates
+    them then, as before)."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+    except ImportError:
+        return None
+    dev = torch.device('cuda')
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to(dev)
+    X1, X2 = pd.XEdge
+    xy = pd.g.coordV[pd.dirichletVertices]
+    D = dict(XE=(t(pd.XE[0]), t(pd.XE[1])), XEdge=(t(X1), t(X2)),
+             dirIdx=torch.from_numpy(pd.dirichletVertices.astype(np.int64)).to(dev),
+             dirXY=(torch.from_numpy(xy[:, 0].copy()).to(dev), torch.from_numpy(xy[:, 1].copy()).to(dev)))
+    probe = (torch.zeros(2, dtype=torch.float64, device=dev), torch.zeros(2, dtype=torch.float64, device=dev))
+    need = []
+    if not pd.isVelocityStationary:
+        need += [pd.u1Cont, pd.u2Cont]
+    if not pd.fZero:
+        need.append(pd.fCont)
+    if not pd.cDZero:
+        need.append(pd.cDCont)
+    try:
+        for f in need:
+            _as_device(f(0.0, *probe), probe[0])
+    except Exception:
+        return None
+    return D
+
+
+def _as_device(r, like):
+    """A closure's result as a contiguous float64 tensor shaped like `like`."""
+    import torch
+    r = torch.as_tensor(r, dtype=torch.float64, device=like.device)
+    return torch.broadcast_to(r, like.shape).contiguous()
 
 
 def vertexDirichlet(pd, t):
@@ -170,15 +213,27 @@ def preprocessSubStep(pd, nStep, nSubStep):
     identically-zero data are uploaded once / skipped."""
     prob = pd.problem
     ts = pd.t[nSubStep - 1]
+    D = pd.get('deviceEval')
     if not (pd.velocitySet and pd.isVelocityStationary):
-        X1, X2 = pd.XEdge
-        prob.set_velocity_points(pd.u1Cont(ts, *pd.XE), pd.u2Cont(ts, *pd.XE),
-                                 pd.u1Cont(ts, X1, X2), pd.u2Cont(ts, X1, X2))
+        if D is not None and not pd.isVelocityStationary:   # evaluated on the device
+            E, Q = D['XEdge'], D['XE']
+            prob.set_velocity_points_dev(_as_device(pd.u1Cont(ts, *Q), Q[0]), _as_device(pd.u2Cont(ts, *Q), Q[0]),
+                                         _as_device(pd.u1Cont(ts, *E), E[0]), _as_device(pd.u2Cont(ts, *E), E[0]))
+        else:
+            X1, X2 = pd.XEdge
+            prob.set_velocity_points(pd.u1Cont(ts, *pd.XE), pd.u2Cont(ts, *pd.XE),
+                                     pd.u1Cont(ts, X1, X2), pd.u2Cont(ts, X1, X2))
         pd.velocitySet = True
     if not (pd.fZero and pd.cDZero):
-        fq = None if pd.fZero else pd.fCont(ts, *pd.XE)
-        cDe = None if pd.cDZero else pd.cDCont(ts, *pd.XEdge)
-        prob.set_source_points(fq, cDe)
+        if D is not None:
+            E, Q = D['XEdge'], D['XE']
+            fq = None if pd.fZero else _as_device(pd.fCont(ts, *Q), Q[0])
+            cDe = None if pd.cDZero else _as_device(pd.cDCont(ts, *E), E[0])
+            prob.set_source_points_dev(fq, cDe)
+        else:
+            fq = None if pd.fZero else pd.fCont(ts, *pd.XE)
+            cDe = None if pd.cDZero else pd.cDCont(ts, *pd.XEdge)
+            prob.set_source_points(fq, cDe)
     return pd
 
 
@@ -188,7 +243,14 @@ def solveSubStep(pd, nStep, nSubStep):
     if pd.isSlopeLim:
         tBC = pd.t[nSubStep] if nSubStep < len(pd.t) else pd.t[0] + pd.tau
         if not (pd.cDZero and pd.get('vertexDTime') is not None):
-            pd.problem.set_vertex_dirichlet(vertexDirichlet(pd, tBC))
+            D = pd.get('deviceEval')
+            if D is not None and not pd.cDZero:
+                import torch
+                v = torch.full((pd.g.numV,), float('nan'), dtype=torch.float64, device=D['dirIdx'].device)
+                v[D['dirIdx']] = _as_device(pd.cDCont(tBC, *D['dirXY']), D['dirXY'][0])
+                pd.problem.set_vertex_dirichlet_dev(v)
+            else:
+                pd.problem.set_vertex_dirichlet(vertexDirichlet(pd, tBC))
             pd.vertexDTime = tBC
     pd.problem.stage(nSubStep, pd.tau)
     return pd

@@ -8,6 +8,8 @@ u1/u2/f/cD/gN(t, x1, x2), elementwise on numpy arrays.  A testcase also
 states which closures are identically zero and whether the data depend on
 time, so the host can skip uploading zero or unchanged data.
 """
+import math
+
 import numpy as np
 
 
@@ -44,15 +46,23 @@ def solid_body():
 
 def transient():
     """Analytical transient case; advection/configureProblem.m:119-132."""
-    cCont = lambda t, x1, x2: np.cos(7 * x1) * np.cos(7 * x2) + np.exp(-t)
-    u1 = lambda t, x1, x2: np.exp(0.5 * (x1 + x2))
-    u2 = lambda t, x1, x2: np.exp(0.5 * (x1 - x2))
-    f = lambda t, x1, x2: (-np.exp(-t) - 7 * u1(t, x1, x2) * np.sin(7 * x1) * np.cos(7 * x2)
-                           - 7 * u2(t, x1, x2) * np.cos(7 * x1) * np.sin(7 * x2)
-                           + 0.5 * (u1(t, x1, x2) - u2(t, x1, x2)) * cCont(t, x1, x2))
+    def cCont(t, x1, x2):
+        xp = _xp(x1)
+        return xp.cos(7 * x1) * xp.cos(7 * x2) + math.exp(-t)
+
+    def u1(t, x1, x2):
+        return _xp(x1).exp(0.5 * (x1 + x2))
+
+    def u2(t, x1, x2):
+        return _xp(x1).exp(0.5 * (x1 - x2))
+
+    def f(t, x1, x2):
+        xp = _xp(x1)
+        return (-math.exp(-t) - 7 * u1(t, x1, x2) * xp.sin(7 * x1) * xp.cos(7 * x2)
+                - 7 * u2(t, x1, x2) * xp.cos(7 * x1) * xp.sin(7 * x2)
+                + 0.5 * (u1(t, x1, x2) - u2(t, x1, x2)) * cCont(t, x1, x2))
     return Testcase(isAnalytical=True, cCont=cCont, c0Cont=lambda x1, x2: cCont(0, x1, x2),
-                    u1Cont=u1, u2Cont=u2, fCont=f, cDCont=cCont,
-                    gNCont=lambda t, x1, x2: np.zeros_like(x1),
+                    u1Cont=u1, u2Cont=u2, fCont=f, cDCont=cCont, gNCont=_zeros,
                     fZero=False, cDZero=False, isStationary=False, isVelocityStationary=True)
 
 

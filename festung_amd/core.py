@@ -202,6 +202,39 @@ class Problem:
         c = None if cDe is None else fortran(cDe, (self.K, 3, self.Re))
         check(lib().festung_set_source_points(self._h, ptr(f), ptr(c)))
 
+    def _dev_args(self, tensors, shapes):
+        """Device pointers of float64 CUDA tensors in [cols][K] layout, made
+        ready for the problem's stream (it waits for the producing torch
+        stream; the caching allocator is told the problem's stream uses them)."""
+        import torch
+        ours = torch.cuda.ExternalStream(self.stream)
+        ours.wait_stream(torch.cuda.current_stream())
+        out = []
+        for t, shape in zip(tensors, shapes):
+            if t is None:
+                out.append(None)
+                continue
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == int(np.prod(shape))):
+                raise ValueError('expected a contiguous float64 CUDA tensor with %d values' % int(np.prod(shape)))
+            t.record_stream(ours)
+            out.append(C.c_void_p(t.data_ptr()))
+        return out
+
+    def set_velocity_points_dev(self, u1q, u2q, u1e, u2e):
+        """set_velocity_points from CUDA tensors laid out [RE][K] and [Re][3][K]
+        (reference element order)."""
+        K = self.K
+        a = self._dev_args([u1q, u2q, u1e, u2e], [(self.RE, K)] * 2 + [(self.Re, 3, K)] * 2)
+        check(lib().festung_set_velocity_points_dev(self._h, *a))
+
+    def set_source_points_dev(self, fq=None, cDe=None):
+        a = self._dev_args([fq, cDe], [(self.RE, self.K), (self.Re, 3, self.K)])
+        check(lib().festung_set_source_points_dev(self._h, *a))
+
+    def set_vertex_dirichlet_dev(self, vertexD):
+        a = self._dev_args([vertexD], [(self.grid.numV,)])
+        check(lib().festung_set_vertex_dirichlet_dev(self._h, *a))
+
     def set_tau_field(self, tauK):
         t = None if tauK is None else fortran(tauK, (self.K,))
         check(lib().festung_set_tau_field(self._h, ptr(t)))

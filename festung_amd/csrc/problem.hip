@@ -132,7 +132,8 @@ struct festung_problem {
   // vperm / vinv the same for vertices.  All C-ABI arrays stay in the
   // reference's order and are permuted here.
   std::vector<int32_t> eperm, einv, vperm, vinv, V0Tref;
-  DBuf dEperm, dEinv;
+  DBuf dEperm, dEinv, dVperm;
+  DBuf devQ1, devQ2, devE1, devE2;  // device-order copies of point values given by device pointer
   std::vector<StagePlan> plan;
   DBuf tab, B, area, nbr, flg, V0T, v2tPad, nuE0T;
   int maxDeg = 0;
@@ -448,6 +449,7 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     for (int32_t d = 0; d < g.numV; ++d) h->vinv[h->vperm[d]] = d;
     h->V0Tref = g.V0T;
     h->upload(h->dEperm, h->eperm.data(), h->eperm.size());
+    h->upload(h->dVperm, h->vperm.data(), h->vperm.size());
     h->upload(h->dEinv, h->einv.data(), h->einv.size());
 
     // mesh, in device order
@@ -581,6 +583,67 @@ int festung_set_velocity_points(festung_problem* h, const double* u1q, const dou
     HIPCHK(launch_vnormal(K, h->tabs.Re, h->nuE0T.d(), e1.d(), e2.d(), h->vn.d(), h->stream));
     sync(h);
     h->hasVelocity = true;
+  });
+}
+
+int festung_set_velocity_points_dev(festung_problem* h, const double* u1q, const double* u2q, const double* u1e,
+                                    const double* u2e) {
+  return guard([&] {
+    check_handle(h);
+    if (!u1q || !u2q || !u1e || !u2e) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    const int K = h->K, RE = h->tabs.RE, Re3 = 3 * h->tabs.Re;
+    if (!h->devQ1.p) h->devQ1.alloc(size_t(RE) * K * sizeof(double));
+    if (!h->devE1.p) h->devE1.alloc(size_t(Re3) * K * sizeof(double));
+    if (!h->devE2.p) h->devE2.alloc(size_t(Re3) * K * sizeof(double));
+    // kernels still in flight read u1, u2, vn: the stream orders the update after them
+    HIPCHK(launch_permute(K, RE, u1q, h->dEperm.i(), h->devQ1.d(), h->stream));
+    HIPCHK(launch_project(h->p, false, K, h->devQ1.d(), h->tab.d(), h->off, h->u1.d(), h->stream));
+    HIPCHK(launch_permute(K, RE, u2q, h->dEperm.i(), h->devQ1.d(), h->stream));
+    HIPCHK(launch_project(h->p, false, K, h->devQ1.d(), h->tab.d(), h->off, h->u2.d(), h->stream));
+    HIPCHK(launch_permute(K, Re3, u1e, h->dEperm.i(), h->devE1.d(), h->stream));
+    HIPCHK(launch_permute(K, Re3, u2e, h->dEperm.i(), h->devE2.d(), h->stream));
+    HIPCHK(launch_vnormal(K, h->tabs.Re, h->nuE0T.d(), h->devE1.d(), h->devE2.d(), h->vn.d(), h->stream));
+    h->hasVelocity = true;
+  });
+}
+
+int festung_set_source_points_dev(festung_problem* h, const double* fq, const double* cDe) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    if (!fq && !cDe) {
+      h->hasRhs = false;
+      return;
+    }
+    if (cDe) h->require_velocity();
+    const int K = h->K, RE = h->tabs.RE, Re3 = 3 * h->tabs.Re;
+    if (!h->rhs.p) h->rhs.alloc(size_t(h->N) * K * sizeof(double));
+    if (fq) {
+      if (!h->devQ2.p) h->devQ2.alloc(size_t(RE) * K * sizeof(double));
+      HIPCHK(launch_permute(K, RE, fq, h->dEperm.i(), h->devQ2.d(), h->stream));
+    }
+    if (cDe) {
+      if (!h->devE1.p) h->devE1.alloc(size_t(Re3) * K * sizeof(double));
+      HIPCHK(launch_permute(K, Re3, cDe, h->dEperm.i(), h->devE1.d(), h->stream));
+    }
+    HIPCHK(launch_source(h->p, h->mesh(), fq ? h->devQ2.d() : nullptr, cDe ? h->devE1.d() : nullptr, h->vn.d(),
+                         h->tab.d(), h->off, h->rhs.d(), h->stream));
+    h->hasRhs = true;
+  });
+}
+
+int festung_set_vertex_dirichlet_dev(festung_problem* h, const double* vertexD) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    if (!(h->flags & FESTUNG_FLAG_LIMITER)) fail(FESTUNG_ERR_STATE, "problem created without the limiter");
+    if (!vertexD) {
+      h->hasVertexD = false;
+      return;
+    }
+    HIPCHK(launch_permute(h->numV, 1, vertexD, h->dVperm.i(), h->vertexD.d(), h->stream));
+    h->hasVertexD = true;
   });
 }
 

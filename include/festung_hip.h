@@ -164,6 +164,18 @@ int festung_set_source_points(festung_problem* h, const double* fq, const double
  * (core/computeMinMaxV0TElementPatch.m:81-82). NULL clears (all NaN). */
 int festung_set_vertex_dirichlet(festung_problem* h, const double* vertexD);
 
+/* Device-pointer variants of the three set-ups above: the arrays live in
+ * device memory, in the same layouts and the reference's element / vertex
+ * order (e.g. point values computed by torch on the GPU), so time-dependent
+ * data never pass through the host.  The work is enqueued on the problem's
+ * stream (festung_problem_stream): the arrays must be ready when that stream
+ * reaches it and stay valid until it has run.  The same
+ * advection/preprocessSubStep.m:61-98 and core/computeFuncContV0T.m as above. */
+int festung_set_velocity_points_dev(festung_problem* h, const double* u1q, const double* u2q, const double* u1e,
+                                    const double* u2e);
+int festung_set_source_points_dev(festung_problem* h, const double* fq, const double* cDe);
+int festung_set_vertex_dirichlet_dev(festung_problem* h, const double* vertexD);
+
 /* L2 projection of point values fq (K x R of quadRule2D(ord), ord in
  * {2p, 2p+1}) to DG coefficients (K x N): core/projectFuncCont2DataDisc.m:94-102.
  * If set_state != 0 the result also becomes the current state. */

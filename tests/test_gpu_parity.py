@@ -271,3 +271,36 @@ def test_set_state_waits_for_running_steps(gpu):
     d.problem.step(20, d.tau)       # returns before the steps have run
     d.problem.set_state(c1)
     assert np.array_equal(d.problem.get_state(), c1)
+
+
+def _timedep_velocity_case():
+    import math
+    from festung_amd.advection.testcases import Testcase, _xp, gaussian_humps
+    s = lambda t: 1.0 + 0.5 * math.sin(4.0 * t)
+    return Testcase(isAnalytical=False, c0Cont=gaussian_humps(61),
+                    u1Cont=lambda t, x1, x2: s(t) * (0.5 - x2) + 0.1 * _xp(x1).sin(3.0 * x1) * (1 - 0 * x2),
+                    u2Cont=lambda t, x1, x2: s(t) * (x1 - 0.5) + 0 * x1,
+                    fCont=lambda t, x1, x2: 0.2 * math.cos(t) * _xp(x1).exp(-x1 * x2),
+                    cDCont=lambda t, x1, x2: 0.1 * (1 + x1 - x2) * math.exp(-t),
+                    gNCont=lambda t, x1, x2: 0 * x1,
+                    fZero=False, cDZero=False, isStationary=False, isVelocityStationary=False)
+
+
+@pytest.mark.parametrize('lim', [False, True])
+def test_time_dependent_data_on_device(gpu, lim):
+    """Velocity, source and Dirichlet data that change every stage, evaluated
+    by torch on the device and handed over by device pointer: against the
+    oracle, and against the same run with host evaluation."""
+    import festung_amd
+    tc = _timedep_velocity_case()
+    kw = dict(p=2, hmax=1 / 8, isSlopeLim=lim, numSteps=6, tEnd=0.06, ordRK=3)
+    d = festung_amd.main('advection', dict(testcase='custom', **tc, **kw))
+    assert d.deviceEval is not None
+    h = festung_amd.main('advection', dict(testcase='custom', deviceEval=False, **tc, **kw))
+    assert h.deviceEval is None
+    ocfg = OA.Struct(testcase='custom', **kw)
+    for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):
+        ocfg[k] = tc[k]
+    o = OA.main('advection', ocfg)
+    assert rel_l2(d.cDisc, o.cDisc) < 10 * TOL
+    assert rel_l2(d.cDisc, h.cDisc) < 1e-13
