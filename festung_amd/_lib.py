@@ -52,6 +52,7 @@ SIGNATURES = {
     'festung_grid_map_ref_to_phy': (C.c_int, [vp, C.c_int32, dp, dp, dp, dp]),
     'festung_grid_destroy': (None, [vp]),
     'festung_ref_size': (C.c_int, [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int64)]),
+    'festung_phi_at_points': (C.c_int, [C.c_int, C.c_int32, dp, dp, dp]),
     'festung_ref_get': (C.c_int, [C.c_int, C.c_char_p, C.c_int, dp]),
     'festung_problem_create': (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
     'festung_problem_destroy': (None, [vp]),

@@ -5,11 +5,16 @@ the host only evaluates the user's closures at the points the library
 returns (as the reference does inside projectFuncCont2DataDisc and
 computeFuncContNuOnQuadEdge) and sequences the calls.
 
-Visualisation output (outputStep, the VTK frames of initializeProblem and
-postprocessProblem) is not produced.
+Visualisation (initializeProblem, outputStep, postprocessProblem) writes the
+reference's VTK / Tecplot files when ``isVisSol`` is set; its default is
+false here (the reference's is true), so that runs write no files unless
+asked to.
 """
+import os
+
 import numpy as np
 
+from .. import output as OUT
 from ..core import Grid, Problem, quadRule1D, quadRule2D
 from . import testcases as TC
 
@@ -63,6 +68,12 @@ def configureProblem(pd):
     pd.setdefault('isSlopeLim', False)
     pd.setdefault('typeSlopeLim', 'linear')
     pd.setdefault('useGraph', True)
+    # visualisation, advection/configureProblem.m:91-96 (isVisSol defaults to false here)
+    pd.setdefault('isVisGrid', False)
+    pd.setdefault('isVisSol', False)
+    pd.setdefault('outputFrequency', 100)
+    pd.setdefault('outputBasename', os.path.join('output', 'advection_' + pd.get('typeSlopeLim', 'linear')))
+    pd.setdefault('outputTypes', ['vtk'])
     assert 0 <= pd.p <= 4, 'Polynomial order must be zero to four.'
     assert 1 <= pd.ordRK <= 4, 'Order of Runge Kutta must be one to four.'
     assert pd.hmax > 0, 'Maximum edge length must be positive.'
@@ -175,7 +186,15 @@ def initializeProblem(pd):
         pd.vertexDTime = 0.0
         prob.limit()
     pd.initialError = prob.l2_error(prob.ordE, pd.c0Cont(*pd.XE))
+    if pd.isVisSol:
+        _visualize(pd, prob.get_state(), 0)
     return pd
+
+
+def _visualize(pd, cDisc, tLvl):
+    """advection/initializeProblem.m:76-83, outputStep.m:55-64, postprocessProblem.m:45-54."""
+    OUT.visualizeDataLagr(pd.g, OUT.projectDataDisc2DataLagr(cDisc), 'u_h', pd.outputBasename, tLvl,
+                          pd.outputTypes)
 
 
 def preprocessStep(pd, nStep):
@@ -270,7 +289,10 @@ def postprocessStep(pd, nStep):
 
 
 def outputStep(pd, nStep):
-    """advection/outputStep.m:53-66 (visualisation is not produced)."""
+    """advection/outputStep.m:53-66: Lagrange-point output every
+    outputFrequency steps."""
+    if pd.isVisSol and nStep % pd.outputFrequency == 0:
+        _visualize(pd, pd.problem.get_state(), nStep)
     return pd
 
 
@@ -279,6 +301,8 @@ def postprocessProblem(pd):
     qOrd = 2p+1 against c(tEnd) (analytical) or c0."""
     prob = pd.problem
     pd.cDisc = prob.get_state()
+    if pd.isVisSol:
+        _visualize(pd, pd.cDisc, pd.numSteps)
     if pd.isAnalytical:
         cq = pd.cCont(pd.tEnd, *pd.XF)
     else:

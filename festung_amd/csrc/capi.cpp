@@ -246,6 +246,16 @@ int festung_ref_size(int p, const char* name, int ord, int64_t* count) {
   });
 }
 
+int festung_phi_at_points(int p, int32_t npts, const double* X1, const double* X2, double* out) {
+  return guard([&] {
+    if (p < 0 || p > 4) fail(FESTUNG_ERR_UNSUPPORTED, "polynomial degree p = %d (must be 0..4)", p);
+    if (npts < 0 || (npts > 0 && (!X1 || !X2 || !out))) fail(FESTUNG_ERR_ARG, "invalid argument");
+    const int N = nbasis(p);
+    for (int i = 0; i < N; ++i)
+      for (int32_t r = 0; r < npts; ++r) out[r + size_t(i) * npts] = phi(i + 1, X1[r], X2[r]);
+  });
+}
+
 int festung_ref_get(int p, const char* name, int ord, double* out) {
   return guard([&] {
     if (!name || !out) fail(FESTUNG_ERR_ARG, "null pointer argument");

@@ -107,6 +107,9 @@ void festung_grid_destroy(festung_grid* g);
  *  "taylorF", "taylorFinv" (N x N)    Taylor change-of-basis tables
  * `ord` selects the quadrature order for the quad/phi tables (ignored
  * otherwise).  festung_ref_size gives the element count. */
+/* The orthonormal basis at reference points: out (npts x N, column-major)
+ * = phi_i(X1_r, X2_r), core/phi.m:40-61. */
+int festung_phi_at_points(int p, int32_t npts, const double* X1, const double* X2, double* out);
 int festung_ref_size(int p, const char* name, int ord, int64_t* count);
 int festung_ref_get(int p, const char* name, int ord, double* out);
 

@@ -304,3 +304,20 @@ def test_time_dependent_data_on_device(gpu, lim):
     o = OA.main('advection', ocfg)
     assert rel_l2(d.cDisc, o.cDisc) < 10 * TOL
     assert rel_l2(d.cDisc, h.cDisc) < 1e-13
+
+
+def test_main_writes_visualisation(gpu, tmp_path):
+    """isVisSol: the initial, every outputFrequency-th and final frames, in
+    both formats, holding the device state's Lagrange values."""
+    import festung_amd
+    from festung_amd import output as O
+    base = str(tmp_path / 'vis' / 'adv')
+    d = festung_amd.main('advection', dict(testcase='solid_body', p=2, hmax=1 / 8, numSteps=4, isVisSol=True,
+                                          outputFrequency=2, outputBasename=base, outputTypes=['vtk', 'tec']))
+    names = sorted(p.name for p in (tmp_path / 'vis').iterdir())
+    assert names == ['adv.%d.%s' % (n, e) for n in (0, 2, 4) for e in ('plt', 'vtu')]
+    import xml.etree.ElementTree as ET
+    piece = ET.parse(base + '.4.vtu').getroot().find('UnstructuredGrid/Piece')
+    u = np.array([a for a in piece.iter('DataArray') if a.get('Name') == 'u_h'][0].text.split(), dtype=float)
+    want = O.projectDataDisc2DataLagr(d.cDisc)[:, [0, 1, 2, 5, 3, 4]].ravel()
+    assert np.allclose(u, want, rtol=1e-8, atol=1e-12)

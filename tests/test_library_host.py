@@ -194,3 +194,44 @@ def test_host_mirror_defaults_follow_reference():
         S.configureProblem(S.Struct(p=0, isSlopeLim=True))
     with pytest.raises(NotImplementedError):
         S.configureProblem(S.Struct(isQuadri=True))
+
+
+def test_lagrange_projection_and_vtk_tecplot_files(tmp_path):
+    """Lagrange-point values of DG data (p = 1: a linear function's projection
+    is exact at the vertices) and the reference's .vtu / .plt layouts, read
+    back with an XML parser and by counting the Tecplot blocks."""
+    import xml.etree.ElementTree as ET
+    from festung_amd import output as O
+    g = Grid.domainSquare(1 / 3)
+    f = lambda x1, x2: 0.3 * x1 - 0.7 * x2 + 0.1
+    # DG coefficients of f for p = 1 from its values at the vertices (exact interpolation)
+    ph = O.phi_at_points(1, [0, 1, 0], [0, 0, 1])                     # 3 x 3
+    vals = f(g.coordV0T[:, :, 0], g.coordV0T[:, :, 1])                # K x 3
+    c = np.linalg.solve(ph, vals.T).T
+    lag = O.projectDataDisc2DataLagr(c)
+    assert np.allclose(lag, vals, atol=1e-14)
+    c2 = np.zeros((g.numT, 6)); c2[:, :3] = c                          # the same function at p = 2
+    lag2 = O.projectDataDisc2DataLagr(c2)
+    mid = lambda a, b: f(0.5 * (g.coordV0T[:, a, 0] + g.coordV0T[:, b, 0]), 0.5 * (g.coordV0T[:, a, 1] + g.coordV0T[:, b, 1]))
+    assert np.allclose(lag2[:, 3], mid(1, 2)) and np.allclose(lag2[:, 4], mid(0, 2)) and np.allclose(lag2[:, 5], mid(0, 1))
+    for d, nodes, cid in ((lag, 3, 5), (lag2, 6, 22)):
+        vtu, plt = O.visualizeDataLagr(g, d, 'u_h', str(tmp_path / 'out' / 'sol'), 7, ['vtk', 'tec'])
+        assert vtu.endswith('sol.7.vtu') and plt.endswith('sol.7.plt')
+        piece = ET.parse(vtu).getroot().find('UnstructuredGrid/Piece')
+        assert int(piece.get('NumberOfPoints')) == nodes * g.numT and int(piece.get('NumberOfCells')) == g.numT
+        arrays = {a.get('Name'): np.array(a.text.split(), dtype=float) for a in piece.iter('DataArray')}
+        assert np.all(arrays['types'] == cid)
+        pts = np.array(piece.find('Points/DataArray').text.split(), dtype=float).reshape(-1, 3)
+        # every node value is f at the node (the data are exact for the linear f)
+        assert np.allclose(arrays['u_h'], f(pts[:, 0], pts[:, 1]), atol=1e-8)
+        txt = open(plt).read().split()
+        assert 'ET=TRIANGLE,' in txt and ('E=%d,' % (g.numT if nodes == 3 else 4 * g.numT)) in txt
+
+
+def test_hotstart_roundtrip(tmp_path):
+    from festung_amd import output as O
+    c = np.arange(12.0).reshape(4, 3)
+    fn = O.outputHotstart(str(tmp_path / 'hs' / 'state'), cDisc=c, t=0.25, nStep=7)
+    assert fn.endswith('state.mat')
+    d = O.readHotstart(fn)
+    assert np.array_equal(d['cDisc'], c) and d['t'] == 0.25 and d['nStep'] == 7
