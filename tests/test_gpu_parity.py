@@ -321,3 +321,50 @@ def test_main_writes_visualisation(gpu, tmp_path):
     u = np.array([a for a in piece.iter('DataArray') if a.get('Name') == 'u_h'][0].text.split(), dtype=float)
     want = O.projectDataDisc2DataLagr(d.cDisc)[:, [0, 1, 2, 5, 3, 4]].ravel()
     assert np.allclose(u, want, rtol=1e-8, atol=1e-12)
+
+
+def _delaunay(n, seed):
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    pts = np.vstack([rng.uniform(0, 1, (n, 2)), [[0, 0], [1, 0], [0, 1], [1, 1]]])
+    tri = Delaunay(pts).simplices.astype(np.int64)
+    a, b, c = pts[tri[:, 0]], pts[tri[:, 1]], pts[tri[:, 2]]
+    cw = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]) < 0
+    tri[cw] = tri[cw][:, [0, 2, 1]]
+    return pts, tri
+
+
+@pytest.mark.parametrize('p,ordRK,lim', [(1, 2, True), (2, 3, True), (3, 3, False), (2, 3, False)])
+def test_unstructured_mesh_steps(gpu, p, ordRK, lim):
+    """Random Delaunay meshes (irregular vertex degrees, every orientation of
+    the neighbour's local edge): four steps against the oracle."""
+    from festung_amd.core import Grid
+    from oracle import grid as OG
+    pts, tri = _delaunay(400, 7 + p)
+    og = OG.generateGridData(pts, tri)
+    idE = (og.E0E[:, 1] < 0).astype(np.int32)       # every boundary edge Dirichlet
+    og.idE = idE
+    og.idE0T = idE[og.E0T]
+    g = Grid.generateGridData(pts, tri)
+    g.set_boundary(idE)
+    tc = TC.rotating_humps()
+    kw = dict(p=p, hmax=0.05, ordRK=ordRK, isSlopeLim=lim, numSteps=4, tEnd=4 * 0.1 * 0.02 / (2 * p + 1))
+    ocfg = OA.Struct(testcase='custom', **kw)
+    for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):
+        ocfg[k] = tc[k]
+    ocfg = OA.configureProblem(ocfg)
+    ocfg.generateGridData = lambda h: og
+    o = OA.preprocessProblem(ocfg)
+    o = OA.initializeProblem(o)
+    d = S.Struct(testcase='custom', useGraph=True, **kw)
+    d.update(tc)
+    d = S.configureProblem(d)
+    d.generateGridData = lambda h: g
+    d = S.preprocessProblem(d)
+    d = S.initializeProblem(d)
+    assert rel_l2(d.problem.get_state(), o.cDisc) < TOL
+    c = o.cDisc.copy()
+    for n in range(1, 5):
+        c = oracle_step(o, c, n)
+        got = product_step(d, n)
+        assert rel_l2(got, c) < TOL * n, n
