@@ -152,7 +152,8 @@ def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6):
         wall = time.perf_counter() - t0
         key = 'ckpt%d' % c if c else 'all_states'
         out[key] = {'adjoint_forward_ms': r['forward_ms'], 'adjoint_reverse_ms': r['backward_ms'],
-                    'cost_vs_forward': (r['forward_ms'] + r['backward_ms']) / fwd_ms, 'wall_s': wall}
+                    'cost_vs_forward': (r['forward_ms'] + r['backward_ms']) / fwd_ms,
+                    'reverse_vs_forward': r['backward_ms'] / fwd_ms, 'wall_s': wall}
     # the other sweep variants, same checkpoint interval
     for serial, kern in ((True, 'fused'), (False, 'fused'), (True, 'split'), (False, 'split')):
         mf = AD.StreamModel(hmax, p, modes=modes, c0=TC.gaussian_humps(51), adj_serial=serial, adj_kernel=kern)

@@ -886,11 +886,15 @@ __global__ __launch_bounds__(BLOCK) void l2_kernel(int K, const double* __restri
 // selector and its trace, with the neighbour-trace table indexed by
 // (neighbour's local edge, own local edge).  No atomics.
 // W = ADJ_STAGE: transposed stage only; ADJ_FUSED: with the stage's velocity
-// sensitivities; ADJ_SENS: the sensitivities only.
+// sensitivities; ADJ_SENS: the sensitivities only; ADJ_SENS_U / ADJ_SENS_V:
+// only those w.r.t. u1Disc, u2Disc (element term) / u.nu (edge term).
 // UNR: unroll the edge loop (more loads in flight per thread, more registers).
 template <int P, int W, bool UNR = false>
 __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* sTheta, const double* sPhi1D) {
-  constexpr bool SENS = W != ADJ_STAGE, ADJ = W != ADJ_SENS;
+  constexpr bool ADJ = W == ADJ_STAGE || W == ADJ_FUSED;
+  constexpr bool SENS_U = W == ADJ_FUSED || W == ADJ_SENS || W == ADJ_SENS_U;
+  constexpr bool SENS_V = W == ADJ_FUSED || W == ADJ_SENS || W == ADJ_SENS_V;
+  constexpr bool SENS = SENS_U || SENS_V;
   constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
   const int K = a.m.K;
   const double* __restrict__ T = a.tab;
@@ -906,17 +910,20 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
   g.load(a.m.B, k, K);
   const double tk = a.tauK ? a.tauK[k] : a.tau;
   const double al = (1.0 - a.omega) * tk;   // weight of this stage's sensitivity
-  double c[SENS ? N : 1], gu1[SENS ? N : 1], gu2[SENS ? N : 1], gvn[SENS ? 3 * Re : 1];
+  double c[SENS ? N : 1], gu1[SENS_U ? N : 1], gu2[SENS_U ? N : 1], gvn[SENS_V ? 3 * Re : 1];
   if constexpr (SENS) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      c[i] = a.c[i * K + k];
-      gu1[i] = gu2[i] = 0.0;
-    }
+    for (int i = 0; i < N; ++i) c[i] = a.c[i * K + k];
+  }
+  if constexpr (SENS_U) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) gu1[i] = gu2[i] = 0.0;
+  }
+  if constexpr (SENS_V) {
 #pragma unroll
     for (int i = 0; i < 3 * Re; ++i) gvn[i] = 0.0;
   }
-  if constexpr (P >= 1) {
+  if constexpr (P >= 1 && (ADJ || SENS_U)) {
     constexpr int UQ = RE <= 6 ? RE : 1;
 #pragma unroll UQ
     for (int q = 0; q < RE; ++q) {
@@ -928,7 +935,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
         m2 += mu[j] * T[a.off.elemQ + QIdx<P>::d2(q, j)];
         u1q += u1[j] * ph;
         u2q += u2[j] * ph;
-        if constexpr (SENS) cq += c[j] * ph;
+        if constexpr (SENS_U) cq += c[j] * ph;
       }
       const double wq = T[a.off.elemQ + QIdx<P>::w(q)];
       if constexpr (ADJ) {
@@ -936,7 +943,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
 #pragma unroll
         for (int j = 0; j < N; ++j) acc[j] += T[a.off.elemQ + QIdx<P>::phi(q, j)] * f;
       }
-      if constexpr (SENS) {
+      if constexpr (SENS_U) {
         const double wc = al * wq * cq;
         const double f1 = wc * (g.B22 * m1 - g.B21 * m2);
         const double f2 = wc * (g.B11 * m2 - g.B12 * m1);
@@ -950,7 +957,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
   }
   const int fl = a.m.flags[k];
 #pragma unroll(UNR ? 3 : 1)
-  for (int n = 0; n < 3; ++n) {
+  for (int n = 0; n < (ADJ || SENS_V ? 3 : 0); ++n) {
     const double area = a.m.areaE[n * K + k];
     const int nb = a.m.nbr[n * K + k];
     const int np = flag_np(fl, n);  // neighbour's local edge
@@ -961,7 +968,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
       const double v = a.vn[(r * 3 + n) * K + k];
       const bool up = v > 0.0 && outOn;
       const bool in = v < 0.0 && nb >= 0;
-      if ((ADJ && up) || (SENS && (up || in))) {
+      if ((ADJ && up) || (SENS_V && (up || in))) {
         double t = 0.0;
 #pragma unroll
         for (int i = 0; i < N; ++i) t += mu[i] * T[a.off.edgeQ + QIdx<P>::ephi(n, r, i)];
@@ -971,7 +978,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
 #pragma unroll
           for (int j = 0; j < N; ++j) acc[j] -= T[a.off.edgeQ + QIdx<P>::ephi(n, r, j)] * fx;
         }
-        if constexpr (SENS) {
+        if constexpr (SENS_V) {
           double cu = 0.0;
           if (up) {
 #pragma unroll
@@ -1005,12 +1012,14 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
       }
     }
   }
-  if constexpr (SENS) {
+  if constexpr (SENS_U) {
 #pragma unroll
     for (int l = 0; l < N; ++l) {
       a.Gu1[l * K + k] += gu1[l];
       a.Gu2[l * K + k] += gu2[l];
     }
+  }
+  if constexpr (SENS_V) {
 #pragma unroll
     for (int i = 0; i < 3 * Re; ++i) a.Gvn[i * K + k] += gvn[i];
   }
@@ -1071,23 +1080,26 @@ __global__ __launch_bounds__(BLOCK) void adj_stage_kernel(AdjArgs a) {
   adj_body<P, W>(a, k, sTheta, sPhi1D);
 }
 
-// Two threads per element: the first half of the workgroup applies the
-// transposed stage to E elements, the second half accumulates the same
-// elements' velocity sensitivities.  The halves share only read-only inputs,
-// so they need no synchronisation; on small meshes this doubles the waves in
-// flight.
+// Three threads per element: the first third of the workgroup applies the
+// transposed stage to E elements, the second accumulates the same elements'
+// sensitivities w.r.t. u1Disc, u2Disc, the third those w.r.t. u.nu.  They
+// share only read-only inputs and write disjoint outputs, so they need no
+// synchronisation; on small meshes this triples the waves in flight.  Each
+// output is formed by the same operations as in the one-thread variant.
 template <int P>
 __global__ __launch_bounds__(BLOCK) void adj_split_kernel(AdjArgs a) {
   constexpr int N = Dims<P>::N, Re = Dims<P>::Re;
   __shared__ double sTheta[Re * N * 9];
   __shared__ double sPhi1D[Re * N * 3];
   adj_tables<P>(a, sTheta, sPhi1D);
-  const int E = int(blockDim.x) / 2;
+  const int E = int(blockDim.x) / 3;
   const int t = threadIdx.x;
-  const int k = logical_block() * E + (t < E ? t : t - E);
+  const int role = t / E;
+  const int k = logical_block() * E + (t - role * E);
   if (k >= a.m.K) return;
-  if (t < E) adj_body<P, ADJ_STAGE, true>(a, k, sTheta, sPhi1D);
-  else adj_body<P, ADJ_SENS, true>(a, k, sTheta, sPhi1D);
+  if (role == 0) adj_body<P, ADJ_STAGE, true>(a, k, sTheta, sPhi1D);
+  else if (role == 1) adj_body<P, ADJ_SENS_U, true>(a, k, sTheta, sPhi1D);
+  else adj_body<P, ADJ_SENS_V, true>(a, k, sTheta, sPhi1D);
 }
 
 template <int P>
@@ -1311,9 +1323,9 @@ hipError_t launch_adj_stage(int p, const AdjArgs& a, hipStream_t s) {
     case ADJ_FUSED: FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((adj_stage_kernel<PP, ADJ_FUSED>), gr, dim3(BS), 0, s, a)); break;
     case ADJ_SENS: FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((adj_stage_kernel<PP, ADJ_SENS>), gr, dim3(BS), 0, s, a)); break;
     case ADJ_SPLIT: {
-      const int E = BS < 128 ? BS : 128;
+      constexpr int E = 64;  // three wave-sized roles per workgroup
       dim3 g2 = grid_for(a.m.K, E);
-      FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((adj_split_kernel<PP>), g2, dim3(2 * E), 0, s, a));
+      FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((adj_split_kernel<PP>), g2, dim3(3 * E), 0, s, a));
     } break;
     default: return hipErrorInvalidValue;
   }

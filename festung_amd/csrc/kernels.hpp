@@ -70,7 +70,7 @@ struct Mesh {
 };
 
 enum StageOut { OUT_RK_DG = 0, OUT_RK_TAYLOR = 1, OUT_DOT_TAYLOR = 2, OUT_DOT_DG = 3 };
-enum AdjWhat { ADJ_STAGE = 0, ADJ_FUSED = 1, ADJ_SENS = 2, ADJ_SPLIT = 3 };
+enum AdjWhat { ADJ_STAGE = 0, ADJ_FUSED = 1, ADJ_SENS = 2, ADJ_SPLIT = 3, ADJ_SENS_U = 4, ADJ_SENS_V = 5 };
 enum LimitMode { LIM_SOLUTION = 0, LIM_DOT_RK_DG = 1, LIM_DOT_RK_TAYLOR = 2 };
 // Limiter types of core/applySlopeLimiterTaylor.m:69-85
 enum LimiterType { LT_LINEAR = 0, LT_HIERARCH = 1, LT_STRICT = 2 };
@@ -162,7 +162,7 @@ struct AdjArgs {
   int mode;
   double* L0;
   double* out;
-  int what;              // ADJ_STAGE, ADJ_FUSED, ADJ_SENS or ADJ_SPLIT (stage + sensitivities, two threads per element)
+  int what;              // ADJ_STAGE, ADJ_FUSED, ADJ_SENS or ADJ_SPLIT (stage + sensitivities, three threads per element)
   // velocity sensitivities of this stage (ADJ_FUSED, ADJ_SENS):
   //   G += (1 - omega) tau_k d/d(u1Disc, u2Disc, vn) [mu^T A c]
   const double* c;       // stage state c_s
