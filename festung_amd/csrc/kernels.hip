@@ -50,23 +50,31 @@ __device__ __forceinline__ int gtid() { return logical_block() * int(blockDim.x)
 // derivative multi-indices.  Row b (target derivative d1^(D-b) d2^b) holds
 // the coefficients of (A11 X + A21)^(D-b) (A12 X + A22)^b, column b' the
 // coefficient of X^(D-b') (source derivative d1^(D-b') d2^b').
+// Row b of E_D(A): pw[k] = coefficient of X^k in (A11 X + A21)^(D-b)
+// (A12 X + A22)^b, built one factor at a time; after t factors only
+// pw[0 .. t] are non-zero, so only those are updated (no products with the
+// zero padding, which IEEE arithmetic cannot drop).
+template <int D>
+__device__ __forceinline__ void sym_row(int b, double A11, double A21, double A12, double A22, double (&pw)[D + 1]) {
+  pw[0] = 1.0;
+#pragma unroll
+  for (int t = 0; t < D; ++t) {
+    const double c0 = t < D - b ? A21 : A22;
+    const double c1 = t < D - b ? A11 : A12;
+    pw[t + 1] = pw[t] * c1;
+#pragma unroll
+    for (int i = t; i >= 1; --i) pw[i] = pw[i] * c0 + pw[i - 1] * c1;
+    pw[0] = pw[0] * c0;
+  }
+}
+
 // y[0 .. D] = E_D(A) x[0 .. D], each row formed and contracted in turn
 template <int D>
 __device__ __forceinline__ void apply_sym(double A11, double A21, double A12, double A22, const double* x, double* y) {
 #pragma unroll
   for (int b = 0; b <= D; ++b) {
     double pw[D + 1];
-    pw[0] = 1.0;
-#pragma unroll
-    for (int i = 1; i <= D; ++i) pw[i] = 0.0;
-#pragma unroll
-    for (int t = 0; t < D; ++t) {
-      const double c0 = t < D - b ? A21 : A22;
-      const double c1 = t < D - b ? A11 : A12;
-#pragma unroll
-      for (int i = D; i >= 1; --i) pw[i] = pw[i] * c0 + pw[i - 1] * c1;
-      pw[0] = pw[0] * c0;
-    }
+    sym_row<D>(b, A11, A21, A12, A22, pw);
     double s = 0.0;
 #pragma unroll
     for (int bp = 0; bp <= D; ++bp) s += pw[D - bp] * x[bp];
@@ -156,17 +164,7 @@ __device__ __forceinline__ void moments_deg(const Geo<P>& g, const double* __res
 #pragma unroll
   for (int b = 0; b <= D; ++b) {
     double pw[D + 1];
-    pw[0] = 1.0;
-#pragma unroll
-    for (int i = 1; i <= D; ++i) pw[i] = 0.0;
-#pragma unroll
-    for (int t = 0; t < D; ++t) {
-      const double c0 = t < D - b ? A21 : A22;
-      const double c1 = t < D - b ? A11 : A12;
-#pragma unroll
-      for (int i = D; i >= 1; --i) pw[i] = pw[i] * c0 + pw[i - 1] * c1;
-      pw[0] = pw[0] * c0;
-    }
+    sym_row<D>(b, A11, A21, A12, A22, pw);
     double s = 0.0;
 #pragma unroll
     for (int bp = 0; bp <= D; ++bp) s += pw[D - bp] * nu[(D - bp) * 9 + bp];
@@ -285,13 +283,89 @@ struct ElemTabInLDS {
   static constexpr int size = value ? Dims<P>::RE * (3 * Dims<P>::N + 1) : 1;
 };
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Element term  (G1 + G2) c  of the 64 elements e0 .. e0+63 of one wave on
+// the matrix cores, as two GEMMs per 16-element group and 16-point block:
+//   V[q][e] = sum_j phi_j(q) X_j[e]            for X = c, u1, u2
+//   acc[i][e] += sum_q w_q d1phi_i(q) F1[q][e] + w_q d2phi_i(q) F2[q][e]
+// with F1 = c (B22 u1 - B12 u2), F2 = c (B11 u2 - B21 u1) pointwise.  In the
+// 16x16x4 FP64 MFMA, lane l holds A[l % 16][l / 16], B[l / 16][l % 16] and
+// D[l / 16 + 4 t][l % 16] in accumulator register t, so register t of V is
+// the B operand of the second GEMM for the points q = 16 qb + 4 t + l / 16
+// (the table D1 / D2 is laid out to match): no data moves between lanes.
+// The result is handed to the one-element-per-lane code through the wave's
+// LDS buffer X (8 rows at a time); acc[i] of this lane's element is set.
+// Every lane of the wave must call this (the MFMA is a wave operation).
+template <int P>
+__device__ __forceinline__ void elem_term_mfma(const StageArgs& a, const double* __restrict__ sM,
+                                               double* __restrict__ X, int e0, double (&acc)[Dims<P>::N]) {
+  using M = MfmaElem<P>;
+  constexpr int N = Dims<P>::N, S = M::S;
+  const int K = a.m.K, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+  dbl4 res[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int e = e0 + 16 * g + lr < K ? e0 + 16 * g + lr : 0;
+    double cB[S], u1B[S], u2B[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int j = 4 * s + lk;
+      const bool in = j < N;
+      cB[s] = in ? a.c[j * K + e] : 0.0;
+      u1B[s] = in ? a.u1[j * K + e] : 0.0;
+      u2B[s] = in ? a.u2[j * K + e] : 0.0;
+    }
+    const double B11 = a.m.B[e], B21 = a.m.B[e + K], B12 = a.m.B[e + 2 * K], B22 = a.m.B[e + 3 * K];
+    dbl4 r1 = {0.0, 0.0, 0.0, 0.0}, r2 = r1;
+#pragma unroll 1
+    for (int qb = 0; qb < M::QB; ++qb) {
+      dbl4 vc = {0.0, 0.0, 0.0, 0.0}, v1 = vc, v2 = vc;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const double ta = sM[(qb * S + s) * 64 + l];
+        vc = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, cB[s], vc, 0, 0, 0);
+        v1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, u1B[s], v1, 0, 0, 0);
+        v2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, u2B[s], v2, 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const double f1 = vc[t] * (B22 * v1[t] - B12 * v2[t]);
+        const double f2 = vc[t] * (B11 * v2[t] - B21 * v1[t]);
+        r1 = __builtin_amdgcn_mfma_f64_16x16x4f64(sM[M::offD1 + (qb * 4 + t) * 64 + l], f1, r1, 0, 0, 0);
+        r2 = __builtin_amdgcn_mfma_f64_16x16x4f64(sM[M::offD2 + (qb * 4 + t) * 64 + l], f2, r2, 0, 0, 0);
+      }
+    }
+    res[g] = r1 + r2;
+  }
+  // hand-over: rows i = lk + 4t of element 16 g + lr -> lane of that element
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      X[lk * 64 + 16 * g + lr] = res[g][2 * h];
+      X[(lk + 4) * 64 + 16 * g + lr] = res[g][2 * h + 1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (8 * h + i < N) acc[8 * h + i] = X[i * 64 + l];
+    __syncthreads();
+  }
+}
+
 template <int P, int MODE>
 __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
+  constexpr bool MF = MfmaElem<P>::value;
   __shared__ double sTheta[Re * N * 9];
-  __shared__ double sQ[ElemTabInLDS<P>::size];
+  __shared__ double sQ[MF ? 1 : ElemTabInLDS<P>::size];
+  __shared__ double sM[MF ? MfmaElem<P>::size : 1];
+  extern __shared__ double sXchg[];  // MF: MfmaElem<P>::xchg per wave (dynamic)
   for (int i = threadIdx.x; i < Re * N * 9; i += blockDim.x) sTheta[i] = a.tab[a.off.theta + i];
-  if constexpr (ElemTabInLDS<P>::value)
+  if constexpr (MF)
+    for (int i = threadIdx.x; i < MfmaElem<P>::size; i += blockDim.x) sM[i] = a.tab[a.off.mfmaE + i];
+  else if constexpr (ElemTabInLDS<P>::value)
     for (int i = threadIdx.x; i < ElemTabInLDS<P>::size; i += blockDim.x) sQ[i] = a.tab[a.off.elemQ + i];
   __syncthreads();
   const double* __restrict__ EQ = ElemTabInLDS<P>::value ? sQ : a.tab + a.off.elemQ;
@@ -302,19 +376,25 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   const int kk = active ? k : 0;
   const double* __restrict__ T = a.tab;
 
+  // element term  (G1 + G2) c  in quadrature form on quadRule2D(max(2p,1));
+  // on the matrix cores first (before this lane's own loads, which are not
+  // needed there), else in the lane's loop below
   double c[N], u1[N], u2[N], acc[N];
+  if constexpr (MF)
+    elem_term_mfma<P>(a, sM, sXchg + (threadIdx.x >> 6) * MfmaElem<P>::xchg, k - (threadIdx.x & 63), acc);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     c[i] = a.c[i * K + kk];
-    u1[i] = a.u1[i * K + kk];
-    u2[i] = a.u2[i * K + kk];
-    acc[i] = 0.0;
+    if constexpr (!MF) {
+      u1[i] = a.u1[i * K + kk];
+      u2[i] = a.u2[i * K + kk];
+      acc[i] = 0.0;
+    }
   }
   Geo<P> g;
   g.load(a.m.B, kk, K);
 
-  // element term  (G1 + G2) c  in quadrature form on quadRule2D(max(2p,1))
-  if constexpr (P >= 1) {
+  if constexpr (!MF && P >= 1) {
     constexpr int UQ = RE <= 6 ? RE : 1;  // p >= 3: not unrolled (measured fastest at p = 3, 4)
 #pragma unroll UQ
     for (int q = 0; q < RE; ++q) {
@@ -1206,10 +1286,12 @@ hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s) {
   const int BS = a.m.bs;
   if (!valid_elem_block(BS)) return hipErrorInvalidValue;
   dim3 gr = grid_for(a.m.K, BS);
-#define L_STAGE(MODE_)                                                                  \
-  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((stage_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
-#define L_STAGE1(MODE_)                                                                 \
-  FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((stage_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
+  // dynamic LDS: the MFMA element term's hand-over buffer, one per wave
+  const size_t lds = mfma_elem(p) ? size_t(BS / 64) * MfmaElem<4>::xchg * sizeof(double) : 0;
+#define L_STAGE(MODE_)                                                                    \
+  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((stage_kernel<PP, MODE_>), gr, dim3(BS), lds, s, a))
+#define L_STAGE1(MODE_)                                                                   \
+  FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((stage_kernel<PP, MODE_>), gr, dim3(BS), lds, s, a))
   switch (mode) {
     case OUT_RK_DG: L_STAGE(OUT_RK_DG); break;
     case OUT_RK_TAYLOR: L_STAGE1(OUT_RK_TAYLOR); break;

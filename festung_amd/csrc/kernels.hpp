@@ -15,6 +15,22 @@ struct Dims {
   static constexpr int Re = P + 1;  // points of quadRule1D(2P + 1)
 };
 
+// The stage kernel's element term on the matrix cores (FP64 16x16x4 MFMA)
+// when the element rule splits into 16-point blocks and the basis fits 16
+// rows: p = 4 (64 points, N = 15).  Lower degrees stay on the vector ALU,
+// where the padding to 16 would cost more than the unit saves.
+inline constexpr bool mfma_elem(int p) { return p == 4; }
+template <int P>
+struct MfmaElem {
+  static constexpr bool value = mfma_elem(P);
+  static constexpr int QB = Dims<P>::RE / 16;    // point blocks
+  static constexpr int S = (Dims<P>::N + 3) / 4;  // 4-wide blocks of basis functions
+  // table (doubles, lane index fastest): A[qb][s][lane], D1[qb][r][lane], D2[qb][r][lane]
+  static constexpr int offD1 = QB * S * 64, offD2 = offD1 + QB * 4 * 64, size = offD2 + QB * 4 * 64;
+  // exchange buffer of one wave: 8 rows x 64 elements
+  static constexpr int xchg = 8 * 64;
+};
+
 // Offsets (in doubles) of the reference tables inside one device buffer.
 struct TabOff {
   int wE, phiE, gradE;      // quadRule2D(max(2p,1)): weights, phi (RE x N), gradPhi (RE x N x 2)
@@ -28,6 +44,10 @@ struct TabOff {
   int elemQ;                // per point q of quadRule2D(max(2p,1)): w_q, phi_j(q), d1phi_j(q), d2phi_j(q)
   int edgeQ;                // per edge n, point r: w_r, phi_j(gamma_n(s_r))
   int hatMinvR, FR, FinvR;  // N x N row-major
+  // operands of the MFMA element term (MfmaElem), per lane l (r = l % 16,
+  // k = l / 16): A[qb][s] = phi_(4s+k)(q = 16qb + r); D1/D2[qb][t] =
+  // w_q d1phi_r(q) / w_q d2phi_r(q) at q = 16qb + 4t + k (zero past N); -1 if unused
+  int mfmaE;
   int total;
 };
 

@@ -436,6 +436,29 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
       o.hatMinvR = put(mi);
       o.FR = put(fr);
       o.FinvR = put(fir);
+      o.mfmaE = -1;
+      if (mfma_elem(p)) {
+        // lane-distributed operands of the MFMA element term (see TabOff::mfmaE)
+        const int QBn = RE / 16, S = (N + 3) / 4;
+        std::vector<double> mt(size_t(QBn) * (S + 8) * 64, 0.0);
+        for (int qb = 0; qb < QBn; ++qb)
+          for (int l = 0; l < 64; ++l) {
+            const int r = l % 16, k = l / 16;
+            for (int s = 0; s < S; ++s) {
+              const int q = 16 * qb + r, j = 4 * s + k;
+              if (j < N) mt[(size_t(qb) * S + s) * 64 + l] = t.phiE[q + size_t(j) * RE];
+            }
+            for (int u = 0; u < 4; ++u) {
+              const int q = 16 * qb + 4 * u + k, i = r;
+              if (i >= N) continue;
+              const size_t d1 = size_t(QBn) * S * 64 + (size_t(qb) * 4 + u) * 64 + l;
+              const size_t d2 = d1 + size_t(QBn) * 4 * 64;
+              mt[d1] = t.ruleE.w[q] * t.gradE[q + size_t(i) * RE];
+              mt[d2] = t.ruleE.w[q] * t.gradE[q + size_t(i) * RE + size_t(RE) * N];
+            }
+          }
+        o.mfmaE = put(mt);
+      }
     }
     o.total = int(tb.size());
     h->upload(h->tab, tb.data(), tb.size());
