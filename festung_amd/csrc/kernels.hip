@@ -285,6 +285,23 @@ struct ElemTabInLDS {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+// Copy n doubles of a table into LDS with the whole workgroup; up to 8 loads
+// per thread are in flight before the first store (a plain strided loop
+// waits for each load in turn).
+__device__ __forceinline__ void fill_lds(double* __restrict__ dst, const double* __restrict__ src, int n) {
+  constexpr int U = 8;
+  const int step = blockDim.x;
+#pragma unroll 1
+  for (int i0 = threadIdx.x; i0 < n; i0 += U * step) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = i0 + u * step < n ? src[i0 + u * step] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * step < n) dst[i0 + u * step] = v[u];
+  }
+}
+
 // Element term  (G1 + G2) c  of the 64 elements e0 .. e0+63 of one wave on
 // the matrix cores, as two GEMMs per 16-element group and 16-point block:
 //   V[q][e] = sum_j phi_j(q) X_j[e]            for X = c, u1, u2
@@ -294,29 +311,37 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // D[l / 16 + 4 t][l % 16] in accumulator register t, so register t of V is
 // the B operand of the second GEMM for the points q = 16 qb + 4 t + l / 16
 // (the table D1 / D2 is laid out to match): no data moves between lanes.
-// The result is handed to the one-element-per-lane code through the wave's
-// LDS buffer X (8 rows at a time); acc[i] of this lane's element is set.
-// Every lane of the wave must call this (the MFMA is a wave operation).
+// The result goes to the wave's LDS buffer X as X[i][element - e0] (16 rows),
+// where the one-element-per-lane code picks it up.  Every lane of the wave
+// must call this (the MFMA is a wave operation).
 template <int P>
 __device__ __forceinline__ void elem_term_mfma(const StageArgs& a, const double* __restrict__ sM,
-                                               double* __restrict__ X, int e0, double (&acc)[Dims<P>::N]) {
+                                               double* __restrict__ X, int e0) {
   using M = MfmaElem<P>;
   constexpr int N = Dims<P>::N, S = M::S;
   const int K = a.m.K, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
-  dbl4 res[4];
-#pragma unroll
-  for (int g = 0; g < 4; ++g) {
+  // operands of group g in buffer g & 1; group g + 1's loads are issued
+  // before group g's MFMAs
+  double cB[2][S], u1B[2][S], u2B[2][S], Bg[2][4];
+  auto load = [&](int g, int b) {
     const int e = e0 + 16 * g + lr < K ? e0 + 16 * g + lr : 0;
-    double cB[S], u1B[S], u2B[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const int j = 4 * s + lk;
       const bool in = j < N;
-      cB[s] = in ? a.c[j * K + e] : 0.0;
-      u1B[s] = in ? a.u1[j * K + e] : 0.0;
-      u2B[s] = in ? a.u2[j * K + e] : 0.0;
+      cB[b][s] = in ? a.c[j * K + e] : 0.0;
+      u1B[b][s] = in ? a.u1[j * K + e] : 0.0;
+      u2B[b][s] = in ? a.u2[j * K + e] : 0.0;
     }
-    const double B11 = a.m.B[e], B21 = a.m.B[e + K], B12 = a.m.B[e + 2 * K], B22 = a.m.B[e + 3 * K];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) Bg[b][m] = a.m.B[e + m * K];
+  };
+  load(0, 0);
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int b = g & 1;
+    if (g < 3) load(g + 1, b ^ 1);
+    const double B11 = Bg[b][0], B21 = Bg[b][1], B12 = Bg[b][2], B22 = Bg[b][3];
     dbl4 r1 = {0.0, 0.0, 0.0, 0.0}, r2 = r1;
 #pragma unroll 1
     for (int qb = 0; qb < M::QB; ++qb) {
@@ -324,9 +349,9 @@ __device__ __forceinline__ void elem_term_mfma(const StageArgs& a, const double*
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         const double ta = sM[(qb * S + s) * 64 + l];
-        vc = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, cB[s], vc, 0, 0, 0);
-        v1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, u1B[s], v1, 0, 0, 0);
-        v2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, u2B[s], v2, 0, 0, 0);
+        vc = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, cB[b][s], vc, 0, 0, 0);
+        v1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, u1B[b][s], v1, 0, 0, 0);
+        v2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, u2B[b][s], v2, 0, 0, 0);
       }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
@@ -336,21 +361,9 @@ __device__ __forceinline__ void elem_term_mfma(const StageArgs& a, const double*
         r2 = __builtin_amdgcn_mfma_f64_16x16x4f64(sM[M::offD2 + (qb * 4 + t) * 64 + l], f2, r2, 0, 0, 0);
       }
     }
-    res[g] = r1 + r2;
-  }
-  // hand-over: rows i = lk + 4t of element 16 g + lr -> lane of that element
+    const dbl4 r = r1 + r2;
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      X[lk * 64 + 16 * g + lr] = res[g][2 * h];
-      X[(lk + 4) * 64 + 16 * g + lr] = res[g][2 * h + 1];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-      if (8 * h + i < N) acc[8 * h + i] = X[i * 64 + l];
-    __syncthreads();
+    for (int t = 0; t < 4; ++t) X[(lk + 4 * t) * 64 + 16 * g + lr] = r[t];
   }
 }
 
@@ -362,11 +375,11 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   __shared__ double sQ[MF ? 1 : ElemTabInLDS<P>::size];
   __shared__ double sM[MF ? MfmaElem<P>::size : 1];
   extern __shared__ double sXchg[];  // MF: MfmaElem<P>::xchg per wave (dynamic)
-  for (int i = threadIdx.x; i < Re * N * 9; i += blockDim.x) sTheta[i] = a.tab[a.off.theta + i];
+  fill_lds(sTheta, a.tab + a.off.theta, Re * N * 9);
   if constexpr (MF)
-    for (int i = threadIdx.x; i < MfmaElem<P>::size; i += blockDim.x) sM[i] = a.tab[a.off.mfmaE + i];
+    fill_lds(sM, a.tab + a.off.mfmaE, MfmaElem<P>::size);
   else if constexpr (ElemTabInLDS<P>::value)
-    for (int i = threadIdx.x; i < ElemTabInLDS<P>::size; i += blockDim.x) sQ[i] = a.tab[a.off.elemQ + i];
+    fill_lds(sQ, a.tab + a.off.elemQ, ElemTabInLDS<P>::size);
   __syncthreads();
   const double* __restrict__ EQ = ElemTabInLDS<P>::value ? sQ : a.tab + a.off.elemQ;
 
@@ -376,20 +389,24 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   const int kk = active ? k : 0;
   const double* __restrict__ T = a.tab;
 
-  // element term  (G1 + G2) c  in quadrature form on quadRule2D(max(2p,1));
-  // on the matrix cores first (before this lane's own loads, which are not
-  // needed there), else in the lane's loop below
+  // element term  (G1 + G2) c  in quadrature form on quadRule2D(max(2p,1)):
+  // on the matrix cores, into the wave's LDS buffer, before the edge term in
+  // even waves and after it in odd ones, so that the two waves sharing a SIMD
+  // tend to overlap one's MFMA work with the other's gathers; else in this
+  // lane's loop below.  Either way acc = edge term + element term in the end.
   double c[N], u1[N], u2[N], acc[N];
+  [[maybe_unused]] double* X = sXchg + (threadIdx.x >> 6) * MfmaElem<P>::xchg;
+  [[maybe_unused]] const bool mfmaFirst = ((k >> 6) & 1) == 0;  // wave-uniform
   if constexpr (MF)
-    elem_term_mfma<P>(a, sM, sXchg + (threadIdx.x >> 6) * MfmaElem<P>::xchg, k - (threadIdx.x & 63), acc);
+    if (mfmaFirst) elem_term_mfma<P>(a, sM, X, k - (threadIdx.x & 63));
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     c[i] = a.c[i * K + kk];
     if constexpr (!MF) {
       u1[i] = a.u1[i * K + kk];
       u2[i] = a.u2[i * K + kk];
-      acc[i] = 0.0;
     }
+    acc[i] = 0.0;
   }
   Geo<P> g;
   g.load(a.m.B, kk, K);
@@ -443,6 +460,12 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 #pragma unroll
       for (int i = 0; i < N; ++i) acc[i] -= T[a.off.edgeQ + QIdx<P>::ephi(n, r, i)] * fx;
     }
+  }
+  if constexpr (MF) {
+    if (!mfmaFirst) elem_term_mfma<P>(a, sM, X, k - (threadIdx.x & 63));
+    __builtin_amdgcn_wave_barrier();  // X is written and read by this wave only
+#pragma unroll
+    for (int i = 1; i < N; ++i) acc[i] += X[i * 64 + (threadIdx.x & 63)];
   }
   if (a.rhs) {
 #pragma unroll
@@ -1143,8 +1166,8 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
 template <int P>
 __device__ __forceinline__ void adj_tables(const AdjArgs& a, double* sTheta, double* sPhi1D) {
   constexpr int N = Dims<P>::N, Re = Dims<P>::Re;
-  for (int i = threadIdx.x; i < Re * N * 9; i += blockDim.x) sTheta[i] = a.tab[a.off.theta + i];
-  for (int i = threadIdx.x; i < Re * N * 3; i += blockDim.x) sPhi1D[i] = a.tab[a.off.phi1D + i];
+  fill_lds(sTheta, a.tab + a.off.theta, Re * N * 9);
+  fill_lds(sPhi1D, a.tab + a.off.phi1D, Re * N * 3);
   __syncthreads();
 }
 

@@ -27,8 +27,8 @@ struct MfmaElem {
   static constexpr int S = (Dims<P>::N + 3) / 4;  // 4-wide blocks of basis functions
   // table (doubles, lane index fastest): A[qb][s][lane], D1[qb][r][lane], D2[qb][r][lane]
   static constexpr int offD1 = QB * S * 64, offD2 = offD1 + QB * 4 * 64, size = offD2 + QB * 4 * 64;
-  // exchange buffer of one wave: 8 rows x 64 elements
-  static constexpr int xchg = 8 * 64;
+  // hand-over buffer of one wave: 16 rows x 64 elements
+  static constexpr int xchg = 16 * 64;
 };
 
 // Offsets (in doubles) of the reference tables inside one device buffer.
