@@ -153,25 +153,85 @@ __device__ __forceinline__ void taylor_to_dg(const double (&T)[Dims<P>::N], cons
   }
 }
 
-// Element moments mu_(a,b) = mean over the element of xi^a eta^b for
-// a + b <= 2P, stored at d(d+1)/2 + b (d = a + b); moments_deg writes the
-// D + 1 moments of degree D to out[b].
-template <int P, int D>
-__device__ __forceinline__ void moments_deg(const Geo<P>& g, const double* __restrict__ nu, double* out) {
-  // xi^a eta^b = (dX B11 s + dX B12 t)^a (dX B21 s + dX B22 t)^b, s, t centred
-  // reference coordinates; contract the coefficients with the reference moments
-  const double A11 = g.dX * g.B11, A21 = g.dX * g.B12, A12 = g.dX * g.B21, A22 = g.dX * g.B22;
+// Element moments mu_(a,b) = mean over the element of xi^a eta^b, xi = dX
+// (x - x_c), degree by degree.  For the linear form z = s xi + t eta the mean
+// of z^D over a triangle is 2 D! / (D + 2)! h_D(z_0, z_1, z_2), with z_m its
+// values at the vertices and h_D the complete homogeneous symmetric
+// polynomial of degree D; hence mu_(D-b,b) = 2 / ((D+1)(D+2)) [s^(D-b) t^b]
+// h_D / C(D, b).  The vertices are centred (e_1 = 0), so Newton's identities
+// give h_D = -e_2 h_(D-2) + e_3 h_(D-3) (forms in (s, t); e_2, e_3 the
+// elementary symmetric polynomials of the z_m, quadratic and cubic forms).
+// Coefficient k of a form of degree d multiplies s^(d-k) t^k.
+template <int D>
+struct Deg {
+  static constexpr int value = D;
+};
+__host__ __device__ constexpr double binom(int n, int k) {
+  double r = 1.0;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+struct TriForms {
+  double e2[3], e3[4];
+};
+
+template <int P>
+__device__ __forceinline__ TriForms tri_forms(const Geo<P>& g) {
+  // centred, scaled vertex coordinates (vertex 0 at -(b1 + b2) / 3)
+  const double ex = (g.B11 + g.B12) / 3.0, ey = (g.B21 + g.B22) / 3.0;
+  const double X[3] = {-ex * g.dX, (g.B11 - ex) * g.dX, (g.B12 - ex) * g.dX};
+  const double Y[3] = {-ey * g.dX, (g.B21 - ey) * g.dX, (g.B22 - ey) * g.dX};
+  TriForms f;
+  f.e2[0] = X[0] * X[1] + X[0] * X[2] + X[1] * X[2];
+  f.e2[1] = X[0] * Y[1] + Y[0] * X[1] + X[0] * Y[2] + Y[0] * X[2] + X[1] * Y[2] + Y[1] * X[2];
+  f.e2[2] = Y[0] * Y[1] + Y[0] * Y[2] + Y[1] * Y[2];
+  const double q0 = X[0] * X[1], q1 = X[0] * Y[1] + Y[0] * X[1], q2 = Y[0] * Y[1];  // z_0 z_1
+  f.e3[0] = q0 * X[2];
+  f.e3[1] = q0 * Y[2] + q1 * X[2];
+  f.e3[2] = q1 * Y[2] + q2 * X[2];
+  f.e3[3] = q2 * Y[2];
+  return f;
+}
+
+// h (degree D) = -e2 hm2 + e3 hm3 (degrees D - 2, D - 3)
+template <int D>
+__device__ __forceinline__ void h_next(const TriForms& f, const double* hm2, const double* hm3, double* h) {
 #pragma unroll
-  for (int b = 0; b <= D; ++b) {
-    double pw[D + 1];
-    sym_row<D>(b, A11, A21, A12, A22, pw);
+  for (int k = 0; k <= D; ++k) {
     double s = 0.0;
 #pragma unroll
-    for (int bp = 0; bp <= D; ++bp) s += pw[D - bp] * nu[(D - bp) * 9 + bp];
-    out[b] = s;
+    for (int u = 0; u < 3; ++u)
+      if (k - u >= 0 && k - u <= D - 2) s -= f.e2[u] * hm2[k - u];
+    if constexpr (D >= 3) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (k - u >= 0 && k - u <= D - 3) s += f.e3[u] * hm3[k - u];
+    }
+    h[k] = s;
   }
 }
 
+template <int D, int DMAX, typename F>
+__device__ __forceinline__ void moment_degrees_from(const TriForms& tf, const double* hm3, const double* hm2,
+                                                    const double* hm1, F& fn) {
+  double h[D + 1], m[D + 1];
+  h_next<D>(tf, hm2, hm3, h);
+#pragma unroll
+  for (int b = 0; b <= D; ++b) m[b] = h[b] * (2.0 / ((D + 1) * (D + 2)) / binom(D, b));
+  fn(Deg<D>{}, m);
+  if constexpr (D < DMAX) moment_degrees_from<D + 1, DMAX>(tf, hm2, hm1, h, fn);
+}
+
+// fn(Deg<D>, m) for D = 0 .. DMAX, m[b] = mu_(D-b,b)
+template <int DMAX, int P, typename F>
+__device__ __forceinline__ void moment_degrees(const Geo<P>& g, F&& fn) {
+  const TriForms tf = tri_forms<P>(g);
+  const double h0[1] = {1.0}, h1[2] = {0.0, 0.0};
+  fn(Deg<0>{}, h0);
+  if constexpr (DMAX >= 1) fn(Deg<1>{}, h1);
+  if constexpr (DMAX >= 2) moment_degrees_from<2, DMAX>(tf, nullptr, h0, h1, fn);
+}
 
 // Taylor index i -> total degree d, exponent b of the second variable, and
 // a! b! (a = d - b), for i = 0 .. 14 in the order (0,0),(1,0),(0,1),(2,0),...
@@ -179,33 +239,6 @@ __host__ __device__ constexpr int tdeg(int i) { return i == 0 ? 0 : i < 3 ? 1 : 
 __host__ __device__ constexpr int tb(int i) { return i - tdeg(i) * (tdeg(i) + 1) / 2; }
 __host__ __device__ constexpr double fact4(int n) { return n == 4 ? 24.0 : n == 3 ? 6.0 : n == 2 ? 2.0 : 1.0; }
 __host__ __device__ constexpr double tfab(int i) { return fact4(tdeg(i) - tb(i)) * fact4(tb(i)); }
-
-// Rows of the element's Taylor mass matrix applied to d, degree by degree:
-// s_i += sum over j with deg(i) + deg(j) = D of mu_(i+j) d_j, and the
-// diagonal den_i = mu_(2i) - mu_i^2 once D = 2 deg(i).  Only the moments of
-// one degree above P are live at a time (mlo holds degrees 0..P); for each i
-// the j are visited in increasing order.
-template <int P, int D>
-__device__ __forceinline__ void lump_rows(const Geo<P>& g, const double* __restrict__ nu,
-                                          const double (&mlo)[Dims<P>::N], const double (&d)[Dims<P>::N],
-                                          double (&s)[Dims<P>::N], double (&den)[Dims<P>::N]) {
-  constexpr int N = Dims<P>::N;
-  double m[D + 1];
-  if constexpr (D <= P) {
-#pragma unroll
-    for (int b = 0; b <= D; ++b) m[b] = mlo[D * (D + 1) / 2 + b];
-  } else {
-    moments_deg<P, D>(g, nu, m);
-  }
-#pragma unroll
-  for (int i = 1; i < N; ++i) {
-#pragma unroll
-    for (int j = 1; j < N; ++j)
-      if (tdeg(i) + tdeg(j) == D) s[i] += m[tb(i) + tb(j)] * d[j];
-    if (2 * tdeg(i) == D) den[i] = m[2 * tb(i)] - mlo[i] * mlo[i];
-  }
-  if constexpr (D < 2 * P) lump_rows<P, D + 1>(g, nu, mlo, d, s, den);
-}
 
 // ------------------------------------------------------ min / max reduce --
 // Workgroup min / max of the first s.F of the N Taylor coefficients Tc of
@@ -590,8 +623,14 @@ __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
 
 // ----------------------------------------------------------------- limit --
 
+// Occupancy floor of the limiter at p = 3, 4 (waves per SIMD): the register
+// allocator needs the hint to stay within 168 / 256 registers there.
+template <int P>
+constexpr int limit_min_waves() { return P == 3 ? 3 : P == 4 ? 2 : 1; }
+
 template <int P, int MODE, int LT>
-__global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min_waves<P>()))) void limit_kernel(
+    LimitArgs a) {
   constexpr int N = Dims<P>::N;
   const int K = a.m.K;
   const int k = gtid();
@@ -677,11 +716,11 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
     }
     if constexpr (LT == LT_STRICT && P >= 2) {
       double mlo[N];
-      mlo[0] = 1.0;
-      moments_deg<P, 1>(g, T + a.off.nu, mlo + 1);
-      moments_deg<P, 2>(g, T + a.off.nu, mlo + 3);
-      if constexpr (P >= 3) moments_deg<P, 3>(g, T + a.off.nu, mlo + 6);
-      if constexpr (P >= 4) moments_deg<P, 4>(g, T + a.off.nu, mlo + 10);
+      moment_degrees<P>(g, [&](auto dc, const double* m) {
+        constexpr int D = decltype(dc)::value;
+#pragma unroll
+        for (int b = 0; b <= D; ++b) mlo[D * (D + 1) / 2 + b] = m[b];
+      });
 #pragma unroll
       for (int n = 0; n < 3; ++n) {
 #pragma unroll
@@ -772,23 +811,35 @@ __global__ __launch_bounds__(BLOCK) void limit_kernel(LimitArgs a) {
       // corr_i = f_i / (mu_2i - mu_i^2) * (sum_j mu_(i+j) d_j - mu_i sum_j mu_j d_j),
       // d_j = (orig_j - lim_j) / f_j, f = a! b!: row i of diag(MT)^-1 MT applied
       // to (orig - lim), with MT the element's Taylor mass matrix
-      double mlo[N];
-      mlo[0] = 1.0;
-      if constexpr (P >= 1) moments_deg<P, 1>(g, T + a.off.nu, mlo + 1);
-      if constexpr (P >= 2) moments_deg<P, 2>(g, T + a.off.nu, mlo + 3);
-      if constexpr (P >= 3) moments_deg<P, 3>(g, T + a.off.nu, mlo + 6);
-      if constexpr (P >= 4) moments_deg<P, 4>(g, T + a.off.nu, mlo + 10);
       double dj[N];
       dj[0] = 0.0;
 #pragma unroll
       for (int j = 1; j < N; ++j) dj[j] = (x[j] - lim[j]) * (1.0 / tfab(j));
+      // one pass over the moment degrees D = 0 .. 2p: the moments of degree
+      // <= p are kept (mlo); degree D adds mu_(i+j) d_j to the rows i with
+      // deg i + deg j = D and sets den_i = mu_(2i) - mu_i^2 at D = 2 deg i
+      double mlo[N], s[N], den[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) s[i] = den[i] = 0.0;
+      moment_degrees<2 * P>(g, [&](auto dc, const double* m) {
+        constexpr int D = decltype(dc)::value;
+        if constexpr (D <= P) {
+#pragma unroll
+          for (int b = 0; b <= D; ++b) mlo[D * (D + 1) / 2 + b] = m[b];
+        }
+        if constexpr (D >= 2) {
+#pragma unroll
+          for (int i = 1; i < N; ++i) {
+#pragma unroll
+            for (int j = 1; j < N; ++j)
+              if (tdeg(i) + tdeg(j) == D) s[i] += m[tb(i) + tb(j)] * dj[j];
+            if (2 * tdeg(i) == D) den[i] = m[2 * tb(i)] - mlo[i] * mlo[i];
+          }
+        }
+      });
       double S0 = 0.0;
 #pragma unroll
       for (int j = 1; j < N; ++j) S0 += mlo[j] * dj[j];
-      double s[N], den[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) s[i] = den[i] = 0.0;
-      lump_rows<P, 2>(g, T + a.off.nu, mlo, dj, s, den);
 #pragma unroll
       for (int i = 1; i < N; ++i) lim[i] += (s[i] - mlo[i] * S0) * tfab(i) / den[i];
     }
