@@ -38,7 +38,6 @@ struct TabOff {
   int wEdge, phi1D, theta;  // quadRule1D(2p+1): weights, phi1D (Re x N x 3), thetaPhi1D (Re x N x 3 x 3)
   int hatM, hatMinv;        // N x N
   int F, Finv;              // Taylor change of basis, N x N
-  int nu;                   // 9 x 9 centred reference moments
   // the same data laid out for the element kernels' loop order, so that the
   // values one loop iteration needs are contiguous (wide uniform loads):
   int elemQ;                // per point q of quadRule2D(max(2p,1)): w_q, phi_j(q), d1phi_j(q), d2phi_j(q)

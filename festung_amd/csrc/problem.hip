@@ -409,7 +409,6 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     o.hatMinv = put(t.hatMinv);
     o.F = put(t.F);
     o.Finv = put(t.Finv);
-    o.nu = put(t.nu);
     {
       // loop-ordered copies (see TabOff)
       const int RE = t.RE, Re = t.Re;

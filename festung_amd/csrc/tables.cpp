@@ -455,8 +455,6 @@ static double fact(int n) {
   return r;
 }
 
-static double binom(int n, int k) { return fact(n) / (fact(k) * fact(n - k)); }
-
 RefTables build_ref_tables(int p) {
   if (p < 0 || p > 4) fail(FESTUNG_ERR_UNSUPPORTED, "polynomial degree p = %d (must be 0..4)", p);
   RefTables t;
@@ -554,18 +552,6 @@ RefTables build_ref_tables(int p) {
       }
   }
   t.Finv = invert(t.F, N);
-  // centred moments of the reference triangle: mean of x1^k x2^l is
-  // 2 k! l! / (k + l + 2)!
-  t.nu.assign(81, 0.0);
-  for (int i = 0; i <= 2 * p && i < 9; ++i)
-    for (int j = 0; i + j <= 2 * p && j < 9; ++j) {
-      double s = 0;
-      for (int k = 0; k <= i; ++k)
-        for (int l = 0; l <= j; ++l)
-          s += binom(i, k) * std::pow(-1.0 / 3, i - k) * binom(j, l) * std::pow(-1.0 / 3, j - l) * 2 * fact(k) *
-               fact(l) / fact(k + l + 2);
-      t.nu[i * 9 + j] = s;
-    }
   return t;
 }
 

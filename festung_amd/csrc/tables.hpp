@@ -46,9 +46,8 @@ struct RefTables {
   std::vector<double> hatRoff;     // N x N x 3 x 3 x Re, core/integrateRefEdgePhiIntPhiExtPerQuad.m
   // Taylor change of basis (limiter): F (N x N) maps DG coefficients to
   // [mean; reference-coordinate derivatives at the centroid], Finv its
-  // inverse; nu(i, j) = mean over the reference triangle of
-  // (x1 - 1/3)^i (x2 - 1/3)^j for i + j <= 2p (stored nu[i * 9 + j]).
-  std::vector<double> F, Finv, nu;
+  // inverse.
+  std::vector<double> F, Finv;
 };
 
 RefTables build_ref_tables(int p);
