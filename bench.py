@@ -155,7 +155,8 @@ def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6):
                     'cost_vs_forward': (r['forward_ms'] + r['backward_ms']) / fwd_ms,
                     'reverse_vs_forward': r['backward_ms'] / fwd_ms, 'wall_s': wall}
     # the other sweep variants, same checkpoint interval
-    for serial, kern in ((True, 'fused'), (False, 'fused'), (True, 'split'), (False, 'split')):
+    for serial, kern in ((True, 'fused'), (False, 'fused'), (True, 'split'), (False, 'split'),
+                         (True, 'deferred'), (False, 'deferred')):
         mf = AD.StreamModel(hmax, p, modes=modes, c0=TC.gaussian_humps(51), adj_serial=serial, adj_kernel=kern)
         mf.gradient(mf.c0, a, 2, tau, cObs, ckpt=1)  # warm-up
         r = mf.gradient(mf.c0, a, nsteps, tau, cObs, ckpt=ckpt)[3]

@@ -18,6 +18,7 @@ FLAG_GRAPH = 2
 FLAG_ADJ_SERIAL = 4
 FLAG_ADJ_FUSED = 8
 FLAG_ADJ_SPLIT = 16
+FLAG_ADJ_DEFER = 32
 FLAG_LIM_HIERARCH = 64
 FLAG_LIM_STRICT = 128
 LIMITER_TYPES = {'linear': 0, 'hierarch_vert': FLAG_LIM_HIERARCH, 'strict': FLAG_LIM_STRICT}

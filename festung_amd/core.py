@@ -149,15 +149,18 @@ class Problem:
         """limiter_type: 'linear', 'hierarch_vert' or 'strict'
         (core/applySlopeLimiterTaylor.m:69-85); adj_serial: recompute
         checkpointed segments on the problem's own stream; adj_kernel: None (by
-        mesh size), 'fused' or 'split' -- the transposed stage kernel of the
-        adjoint."""
-        if adj_kernel not in (None, 'fused', 'split'):
-            raise ValueError('adj_kernel must be None, "fused" or "split"')
+        mesh size), 'fused', 'split' or 'deferred' -- where the adjoint forms
+        the velocity sensitivities (in the transposed stage kernel's thread, on
+        two more threads per element, or once per checkpoint segment after its
+        sweep)."""
+        if adj_kernel not in (None, 'fused', 'split', 'deferred'):
+            raise ValueError('adj_kernel must be None, "fused", "split" or "deferred"')
         if limiter_type not in L.LIMITER_TYPES:
             raise ValueError('Unknown limiter type %r' % (limiter_type,))
         flags = ((L.FLAG_LIMITER if limiter else 0) | (L.FLAG_GRAPH if graph else 0)
                  | (L.FLAG_ADJ_SERIAL if adj_serial else 0)
-                 | {None: 0, 'fused': L.FLAG_ADJ_FUSED, 'split': L.FLAG_ADJ_SPLIT}[adj_kernel]
+                 | {None: 0, 'fused': L.FLAG_ADJ_FUSED, 'split': L.FLAG_ADJ_SPLIT,
+                    'deferred': L.FLAG_ADJ_DEFER}[adj_kernel]
                  | L.LIMITER_TYPES[limiter_type])
         h = C.c_void_p()
         check(lib().festung_problem_create(grid._h, int(p), int(ordRK), flags, C.byref(h)))

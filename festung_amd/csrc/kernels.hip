@@ -1256,6 +1256,196 @@ __global__ __launch_bounds__(BLOCK) void adj_split_kernel(AdjArgs a) {
   else adj_body<P, ADJ_SENS_V, true>(a, k, sTheta, sPhi1D);
 }
 
+// ----------------------------------------- deferred velocity sensitivities --
+// One adjoint stage's sensitivity contributions, formed from zero with the
+// operations of adj_body's SENS_U / SENS_V parts (so that adding them to the
+// running sums in the same order gives the same bits).
+template <int P>
+__device__ __forceinline__ void sens_elem_stage(const double* __restrict__ T, const TabOff& off, const Geo<P>& g,
+                                                const double (&mu)[Dims<P>::N], const double (&c)[Dims<P>::N],
+                                                double al, double (&gu1)[Dims<P>::N], double (&gu2)[Dims<P>::N]) {
+  constexpr int N = Dims<P>::N, RE = Dims<P>::RE;
+#pragma unroll
+  for (int l = 0; l < N; ++l) gu1[l] = gu2[l] = 0.0;
+  constexpr int UQ = RE <= 6 ? RE : 1;
+#pragma unroll UQ
+  for (int q = 0; q < RE; ++q) {
+    double m1 = 0.0, m2 = 0.0, cq = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const double ph = T[off.elemQ + QIdx<P>::phi(q, j)];
+      m1 += mu[j] * T[off.elemQ + QIdx<P>::d1(q, j)];
+      m2 += mu[j] * T[off.elemQ + QIdx<P>::d2(q, j)];
+      cq += c[j] * ph;
+    }
+    const double wq = T[off.elemQ + QIdx<P>::w(q)];
+    const double wc = al * wq * cq;
+    const double f1 = wc * (g.B22 * m1 - g.B21 * m2);
+    const double f2 = wc * (g.B11 * m2 - g.B12 * m1);
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      gu1[l] += T[off.elemQ + QIdx<P>::phi(q, l)] * f1;
+      gu2[l] += T[off.elemQ + QIdx<P>::phi(q, l)] * f2;
+    }
+  }
+}
+
+// Partial sums of the velocity sensitivities over one chunk of a segment's
+// adjoint stages (blockIdx.y = chunk, stages in sweep order): role 0 the
+// element-term ones (G_u1, G_u2), role 1 the edge-term ones (G_vn), 64
+// elements per role; role 0 loads the next stage's inputs while it forms the
+// current one.  part[(chunk * F + f) * K + k], F = 2N + 3(p+1).
+template <int P>
+__global__ __launch_bounds__(BLOCK) void sens_chunk_kernel(SensArgs a) {
+  constexpr int N = Dims<P>::N, Re = Dims<P>::Re, F = 2 * N + 3 * Re;
+  __shared__ double sTheta[Re * N * 9];
+  fill_lds(sTheta, a.tab + a.off.theta, Re * N * 9);
+  __syncthreads();
+  constexpr int E = 64;
+  const int role = threadIdx.x / E;
+  const int k = logical_block() * E + (threadIdx.x - role * E);
+  const int K = a.m.K;
+  if (k >= K) return;
+  const int chunk = blockIdx.y;
+  const int len = (a.count + gridDim.y - 1) / gridDim.y;
+  const int e0 = a.first + chunk * len, e1 = min(a.first + a.count, e0 + len);
+  double* __restrict__ part = a.part + size_t(chunk) * F * K;
+  const double* __restrict__ T = a.tab;
+  Geo<P> g;
+  g.load(a.m.B, k, K);
+  const double tk = a.tauK ? a.tauK[k] : a.tau;
+  if (role == 0) {
+    double G1[N], G2[N], mu[N], c[N];
+#pragma unroll
+    for (int l = 0; l < N; ++l) G1[l] = G2[l] = 0.0;
+    if (e0 < e1) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        mu[i] = a.mus[e0][i * K + k];
+        c[i] = a.cs[e0][i * K + k];
+      }
+    }
+#pragma unroll 1
+    for (int e = e0; e < e1; ++e) {
+      double mun[N], cnx[N];
+      const bool more = e + 1 < e1;
+      if (more) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          mun[i] = a.mus[e + 1][i * K + k];
+          cnx[i] = a.cs[e + 1][i * K + k];
+        }
+      }
+      const double al = (1.0 - a.omegas[e]) * tk;
+      // the table reads stay inside the loop (hoisted, they would not fit
+      // in registers): an opaque zero offset per iteration
+      int tz = 0;
+      asm volatile("" : "+s"(tz));
+      double g1[N], g2[N];
+      sens_elem_stage<P>(T + tz, a.off, g, mu, c, al, g1, g2);
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        G1[l] += g1[l];
+        G2[l] += g2[l];
+      }
+      if (more) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          mu[i] = mun[i];
+          c[i] = cnx[i];
+        }
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      part[l * K + k] = G1[l];
+      part[(N + l) * K + k] = G2[l];
+    }
+  } else {
+    // the edge data do not change over the segment
+    const int fl = a.m.flags[k];
+    int nb[3];
+    double area[3], v[3][Re];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      nb[n] = a.m.nbr[n * K + k];
+      area[n] = a.m.areaE[n * K + k];
+#pragma unroll
+      for (int r = 0; r < Re; ++r) v[n][r] = a.vn[(r * 3 + n) * K + k];
+    }
+    double G[3 * Re];
+#pragma unroll
+    for (int i = 0; i < 3 * Re; ++i) G[i] = 0.0;
+#pragma unroll 1
+    for (int e = e0; e < e1; ++e) {
+      const double* __restrict__ mp = a.mus[e];
+      const double* __restrict__ cp = a.cs[e];
+      const double al = (1.0 - a.omegas[e]) * tk;
+      int tz = 0;  // as above
+      asm volatile("" : "+s"(tz));
+      const double* __restrict__ Te = T + tz;
+      const double* thb = sTheta + tz;
+      double mu[N], c[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        mu[i] = mp[i * K + k];
+        c[i] = cp[i * K + k];
+      }
+      double gv[3 * Re];
+#pragma unroll
+      for (int i = 0; i < 3 * Re; ++i) gv[i] = 0.0;
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+        const int np = flag_np(fl, n);
+        const bool outOn = flag_out(fl, n);
+        double cn[N];
+        const int q = nb[n] >= 0 ? nb[n] : k;
+#pragma unroll
+        for (int j = 0; j < N; ++j) cn[j] = cp[j * K + q];
+#pragma unroll
+        for (int r = 0; r < Re; ++r) {
+          const bool up = v[n][r] > 0.0 && outOn;
+          const bool in = v[n][r] < 0.0 && nb[n] >= 0;
+          if (up || in) {
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) t += mu[i] * Te[a.off.edgeQ + QIdx<P>::ephi(n, r, i)];
+            const double wr = Te[a.off.edgeQ + QIdx<P>::ew(n, r)];
+            double cu = 0.0;
+            if (up) {
+#pragma unroll
+              for (int j = 0; j < N; ++j) cu += c[j] * Te[a.off.edgeQ + QIdx<P>::ephi(n, r, j)];
+            } else {
+              const double* th = thb + r + n * Re * N + np * Re * N * 3;
+#pragma unroll
+              for (int j = 0; j < N; ++j) cu += cn[j] * th[j * Re];
+            }
+            gv[r * 3 + n] -= al * area[n] * wr * t * cu;
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 3 * Re; ++i) G[i] += gv[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3 * Re; ++i) part[(2 * N + i) * K + k] = G[i];
+  }
+}
+
+// G += the chunks' partial sums, in chunk order.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void sens_fold_kernel(SensArgs a, int nchunk) {
+  constexpr int N = Dims<P>::N, Re = Dims<P>::Re, F = 2 * N + 3 * Re;
+  const int K = a.m.K;
+  const int k = gtid();
+  if (k >= K) return;
+  const int f = blockIdx.y;
+  double* G = f < N ? a.Gu1 + size_t(f) * K : f < 2 * N ? a.Gu2 + size_t(f - N) * K : a.Gvn + size_t(f - 2 * N) * K;
+  double s = G[k];
+  for (int c = 0; c < nchunk; ++c) s += a.part[(size_t(c) * F + f) * K + k];
+  G[k] = s;
+}
+
 template <int P>
 __global__ __launch_bounds__(BLOCK) void loss_kernel(int K, const double* __restrict__ B, const double* __restrict__ c,
                                                      const double* __restrict__ cObs, const double* __restrict__ tab,
@@ -1485,6 +1675,17 @@ hipError_t launch_adj_stage(int p, const AdjArgs& a, hipStream_t s) {
     } break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_sens_segment(int p, const SensArgs& a, hipStream_t s) {
+  if (a.count < 1 || a.first < 0 || a.nchunk < 1 || a.nchunk > a.count) return hipErrorInvalidValue;
+  constexpr int E = 64;  // two wave-sized roles per workgroup
+  const int N = (p + 1) * (p + 2) / 2, F = 2 * N + 3 * (p + 1);
+  dim3 gr(grid_for(a.m.K, E).x, a.nchunk);
+  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((sens_chunk_kernel<PP>), gr, dim3(2 * E), 0, s, a));
+  dim3 gf(grid_for(a.m.K).x, F);
+  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((sens_fold_kernel<PP>), gf, dim3(BLOCK), 0, s, a, a.nchunk));
   return hipGetLastError();
 }
 

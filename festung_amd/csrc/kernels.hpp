@@ -90,6 +90,9 @@ struct Mesh {
 
 enum StageOut { OUT_RK_DG = 0, OUT_RK_TAYLOR = 1, OUT_DOT_TAYLOR = 2, OUT_DOT_DG = 3 };
 enum AdjWhat { ADJ_STAGE = 0, ADJ_FUSED = 1, ADJ_SENS = 2, ADJ_SPLIT = 3, ADJ_SENS_U = 4, ADJ_SENS_V = 5 };
+// festung_adjoint's sweep with the sensitivities deferred to one pass per
+// checkpoint segment (launch_sens_segment); its stage kernels run ADJ_STAGE
+constexpr int ADJ_DEFER = 6;
 enum LimitMode { LIM_SOLUTION = 0, LIM_DOT_RK_DG = 1, LIM_DOT_RK_TAYLOR = 2 };
 // Limiter types of core/applySlopeLimiterTaylor.m:69-85
 enum LimiterType { LT_LINEAR = 0, LT_HIERARCH = 1, LT_STRICT = 2 };
@@ -190,8 +193,34 @@ struct AdjArgs {
   double* Gvn;
 };
 
+// Velocity sensitivities of one checkpoint segment, formed after its reverse
+// sweep from the adjoint stage inputs kept for it: for every element and the
+// segment's adjoint stages e = first .. first + count - 1,
+//   G += (1 - omega_e) tau_k d/d(u1Disc, u2Disc, vn) [mu_e^T A c_e]
+// summed in nchunk equal consecutive chunks of stages (partial sums in part,
+// nchunk x (2N + 3(p+1)) x K), then added to G in chunk order: the same
+// per-stage contributions as the stage kernels' ADJ_FUSED / ADJ_SPLIT
+// variants, added in another (fixed) order.
+struct SensArgs {
+  Mesh m;
+  const double* vn;
+  const double* tab;
+  TabOff off;
+  double tau;
+  const double* tauK;
+  const double* const* mus;  // per adjoint stage: its input mu
+  const double* const* cs;   // per adjoint stage: the forward stage state
+  const double* omegas;      // per adjoint stage: omega of its RK stage
+  int first, count, nchunk;
+  double* part;
+  double* Gu1;
+  double* Gu2;
+  double* Gvn;
+};
+
 hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s);
 hipError_t launch_adj_stage(int p, const AdjArgs& a, hipStream_t s);
+hipError_t launch_sens_segment(int p, const SensArgs& a, hipStream_t s);
 // J partials: part[b] = sum over the block of det_k d_k^T hatM d_k, d = c - cObs;
 // muT = 2 d
 hipError_t launch_loss(int p, int K, const double* B, const double* c, const double* cObs, const double* tab,

@@ -1073,12 +1073,17 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
     const size_t nk = size_t(N) * K * sizeof(double);
     const int C = (ckpt <= 0 || ckpt > nsteps) ? nsteps : ckpt;
     const int nseg = (nsteps + C - 1) / C;
-    // Transposed stage kernel: the velocity sensitivities fused into it (one
-    // thread per element), or split over two threads per element, which
-    // doubles the waves in flight where the mesh is too small to fill the GPU.
-    int what = h->nblocks < 2048 ? ADJ_SPLIT : ADJ_FUSED;
+    // Velocity sensitivities: fused into the transposed stage kernel (one
+    // thread per element), split over three threads per element, or deferred
+    // to one pass per checkpoint segment after its sweep (the sweep's kernels
+    // then only apply the transposed stages; the segment's adjoint stage
+    // inputs are kept for the pass).  Small meshes cannot fill the GPU with
+    // one thread per element: there the sweep's critical path is what counts.
+    int what = h->nblocks < 2048 ? ADJ_DEFER : ADJ_FUSED;
     if (h->flags & FESTUNG_FLAG_ADJ_FUSED) what = ADJ_FUSED;
     if (h->flags & FESTUNG_FLAG_ADJ_SPLIT) what = ADJ_SPLIT;
+    if (h->flags & FESTUNG_FLAG_ADJ_DEFER) what = ADJ_DEFER;
+    const bool defer = what == ADJ_DEFER;
     // Streams: A = the problem's stream (the reverse sweep); Cs = the
     // recomputation of the next segment's stage states, into the other of
     // two buffer sets, overlapping the sweep (unless FESTUNG_FLAG_ADJ_SERIAL).
@@ -1099,10 +1104,15 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
         sb[j].reset(new DBuf);
         sb[j]->alloc(nk);
       }
-    // ring of adjoint stage states: stage i reads ring[(i-1) mod R], writes ring[i mod R]
-    constexpr int R = 4;
-    DBuf ring[R];
-    for (auto& b : ring) b.alloc(nk);
+    // ring of adjoint stage states: stage i reads ring[(i-1) mod R], writes
+    // ring[i mod R]; deferred sensitivities keep a whole segment's inputs
+    const int R = defer ? C * S + 2 : 4;
+    std::vector<std::unique_ptr<DBuf>> ringb(R);
+    for (auto& b : ringb) {
+      b.reset(new DBuf);
+      b->alloc(nk);
+    }
+    auto ring = [&](long j) -> double* { return ringb[size_t(j % R)]->d(); };
     DBuf dObs, part, L0, Gu1, Gu2, Gvn, tmp;
     h->upload_elem(dObs, cObs, N);
     part.alloc(size_t(h->nred) * sizeof(double));
@@ -1169,13 +1179,40 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
       }
     };
 
+    // deferred sensitivities: per adjoint stage (sweep order) its input mu,
+    // forward stage state and omega, on the device for the segment passes
+    // The pass sums the stages of each time step as one chunk, all chunks in
+    // parallel, and adds the chunk sums in sweep order: the same order for
+    // any checkpoint interval.  At most nChunk steps per pass (partial sums
+    // of at most 512 MB).
+    const size_t chunkBytes = size_t(2 * N + 3 * Re) * K * sizeof(double);
+    const int nChunk = int(std::max<size_t>(1, std::min<size_t>(32, (size_t(512) << 20) / chunkBytes)));
+    DBuf dMus, dCs, dOm, dPart;
+    if (defer) {
+      std::vector<const double*> hm, hc;
+      std::vector<double> ho;
+      long j = 0;
+      for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int len = std::min(C, nsteps - seg * C);
+        for (int l = len - 1; l >= 0; --l)
+          for (int st = S - 1; st >= 0; --st, ++j) {
+            hm.push_back(ring(j + R - 1));
+            hc.push_back(state(seg, l, len, st));
+            ho.push_back(h->plan[st].rk.a0);
+          }
+      }
+      h->upload(dMus, hm.data(), hm.size());
+      h->upload(dCs, hc.data(), hc.size());
+      h->upload(dOm, ho.data(), ho.size());
+      dPart.alloc(size_t(nChunk) * chunkBytes);
+    }
     // forward, storing checkpoints (and the stage states of the last segments)
     HIPCHK(hipEventRecord(ev[0], A));
     HIPCHK(hipMemcpyAsync(ck[0]->p, h->buf(h->cur), nk, hipMemcpyDeviceToDevice, A));
     for (int seg = 0; seg < nseg; ++seg) run_segment(seg, A, seg < nseg - nsets ? SEG_PASS : SEG_STORE);
     HIPCHK(hipEventRecord(ev[1], A));
     // loss and terminal adjoint (into ring[R-1], read by stage 0)
-    HIPCHK(launch_loss(h->p, K, h->B.d(), ck[nseg]->d(), dObs.d(), h->tab.d(), h->off, part.d(), ring[R - 1].d(), A));
+    HIPCHK(launch_loss(h->p, K, h->B.d(), ck[nseg]->d(), dObs.d(), h->tab.d(), h->off, part.d(), ring(R - 1), A));
     HIPCHK(launch_fill(size_t(N) * K, 0.0, Gu1.d(), A));
     HIPCHK(launch_fill(size_t(N) * K, 0.0, Gu2.d(), A));
     HIPCHK(launch_fill(size_t(3) * Re * K, 0.0, Gvn.d(), A));
@@ -1188,6 +1225,7 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
     }
     long i = 0;  // adjoint stage counter
     for (int seg = nseg - 1; seg >= 0; --seg) {
+      const long segFirst = i;
       const int len = std::min(C, nsteps - seg * C);
       // The forward pass leaves the stage states of the last nsets segments
       // in place; earlier ones are recomputed from their checkpoints.
@@ -1206,7 +1244,7 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
         for (int st = S - 1; st >= 0; --st, ++i) {
           AdjArgs a{};
           a.m = h->mesh();
-          a.mu = ring[(i + R - 1) % R].d();
+          a.mu = ring(i + R - 1);
           a.u1 = h->u1.d();
           a.u2 = h->u2.d();
           a.vn = h->vn.d();
@@ -1217,18 +1255,41 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
           a.omega = h->plan[st].rk.a0;
           a.L0 = L0.d();
           a.mode = S == 1 ? 3 : (st == S - 1 ? 0 : (st > 0 ? 1 : 2));
-          a.out = ring[i % R].d();
+          a.out = ring(i);
           a.c = state(seg, l, len, st);
           a.Gu1 = Gu1.d();
           a.Gu2 = Gu2.d();
           a.Gvn = Gvn.d();
-          a.what = what;
+          a.what = defer ? ADJ_STAGE : what;
           HIPCHK(launch_adj_stage(h->p, a, A));
+        }
+      }
+      if (defer) {
+        SensArgs sa{};
+        sa.m = h->mesh();
+        sa.vn = h->vn.d();
+        sa.tab = h->tab.d();
+        sa.off = h->off;
+        sa.tau = tau;
+        sa.tauK = h->hasTauK ? h->tauK.d() : nullptr;
+        sa.mus = static_cast<const double* const*>(dMus.p);
+        sa.cs = static_cast<const double* const*>(dCs.p);
+        sa.omegas = dOm.d();
+        sa.part = dPart.d();
+        sa.Gu1 = Gu1.d();
+        sa.Gu2 = Gu2.d();
+        sa.Gvn = Gvn.d();
+        for (int s0 = 0; s0 < len; s0 += nChunk) {  // steps s0 .. of the segment, in sweep order
+          const int ns = std::min(nChunk, len - s0);
+          sa.first = int(segFirst) + s0 * S;
+          sa.count = ns * S;
+          sa.nchunk = ns;
+          HIPCHK(launch_sens_segment(h->p, sa, A));
         }
       }
       if (overlap) HIPCHK(hipEventRecord(evDone[seg % 2], A));
     }
-    const double* muFinal = ring[(i + R - 1) % R].d();
+    const double* muFinal = ring(i + R - 1);
     HIPCHK(launch_mass(h->p, K, h->B.d(), muFinal, h->tab.d(), h->off, tmp.d(), A));
     HIPCHK(hipEventRecord(ev[2], A));
     // final state becomes the problem's state

@@ -123,8 +123,11 @@ enum {
                                     stream (default: on a second stream, overlapping the sweep) */
   FESTUNG_FLAG_ADJ_FUSED = 8,    /* festung_adjoint: velocity sensitivities in the transposed stage
                                     kernel's thread (default on large meshes) */
-  FESTUNG_FLAG_ADJ_SPLIT = 16,   /* festung_adjoint: stage and sensitivities on two threads per
-                                    element (default on small meshes) */
+  FESTUNG_FLAG_ADJ_SPLIT = 16,   /* festung_adjoint: stage, element-term and edge-term sensitivities
+                                    on three threads per element */
+  FESTUNG_FLAG_ADJ_DEFER = 32,   /* festung_adjoint: transposed stages alone during the sweep, the
+                                    sensitivities of each checkpoint segment in one pass after it
+                                    (default on small meshes) */
   FESTUNG_FLAG_LIM_HIERARCH = 64, /* limiter type 'hierarch_vert' (core/applySlopeLimiterTaylorHierarchicalVertex.m) */
   FESTUNG_FLAG_LIM_STRICT = 128   /* limiter type 'strict' (core/applySlopeLimiterTaylorStrict.m); default 'linear' */
 };

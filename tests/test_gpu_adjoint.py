@@ -143,20 +143,32 @@ def test_adjoint_elem_block_invariance(gpu):
 
 @pytest.mark.parametrize('nsteps,ckpt', [(12, 2), (12, 5), (7, 0), (9, 4), (10, 3)])
 def test_adjoint_variants_agree(gpu, nsteps, ckpt):
-    """Every way of running the sweep gives the same results bit for bit:
-    the velocity sensitivities fused into the transposed stage kernel or on
-    a second thread per element, checkpointed segments recomputed on the
-    problem's stream or on a second stream overlapping the sweep."""
+    """Every way of running the sweep gives the same results: bit for bit
+    with the velocity sensitivities fused into the transposed stage kernel or
+    on more threads per element, with checkpointed segments recomputed on the
+    problem's stream or on a second stream overlapping the sweep.  Deferred
+    to one pass per checkpoint segment, the sensitivities are the same
+    per-stage terms summed per time step first (another fixed order): equal
+    to 1e-13 relative, and bit for bit between its serial and overlapped
+    sweeps."""
     res = []
-    for serial, kern in ((True, 'fused'), (False, 'fused'), (False, 'split'), (True, 'split')):
+    kinds = ((True, 'fused'), (False, 'fused'), (False, 'split'), (True, 'split'),
+             (False, 'deferred'), (True, 'deferred'))
+    for serial, kern in kinds:
         modes = AD.stream_modes()[:4]
         m = AD.StreamModel(1 / 16, 2, modes=modes, c0=TC.gaussian_humps(51), adj_serial=serial, adj_kernel=kern)
         a = 0.05 * np.random.default_rng(52).standard_normal(len(modes))
         tau = 0.1 / (16 * 0.8 * 5)
         cObs = m.run(m.c0, 1.1 * a, nsteps, tau)
         res.append(m.gradient(m.c0, a, nsteps, tau, cObs, ckpt=ckpt))
-    for r in res[1:]:
+    for (serial, kern), r in zip(kinds[1:], res[1:]):
         assert r[0] == res[0][0]
-        assert np.array_equal(r[1], res[0][1]) and np.array_equal(r[2], res[0][2])
+        assert np.array_equal(r[1], res[0][1])
+        ref = res[4] if kern == 'deferred' else res[0]
+        assert np.array_equal(r[2], ref[2])
         for key in ('gradU1', 'gradU2', 'gradVn'):
-            assert np.array_equal(r[3][key], res[0][3][key])
+            assert np.array_equal(r[3][key], ref[3][key])
+    d, f = res[4], res[0]
+    assert np.allclose(d[2], f[2], rtol=0, atol=1e-13 * np.abs(f[2]).max())
+    for key in ('gradU1', 'gradU2', 'gradVn'):
+        assert np.allclose(d[3][key], f[3][key], rtol=0, atol=1e-13 * np.abs(f[3][key]).max())
