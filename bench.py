@@ -45,17 +45,17 @@ def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6):
     for (c0, ai, ao) in rk:
         comb = N * (c0 + ai + ao)    # RK-combination operands besides c and the output
         if kind == 'stage':
-            if lim and p >= 2:       # cdot -> Taylor: c, u1, u2, vn, B, |E|, nbr, flags -> cdotT, centroid
-                tot += K * (8 * (N + 2 * N + 3 * Re + 4 + 3 + N + 1) + 4 * 4)
+            if lim and p >= 2:       # cdot -> Taylor: c, u1, u2, vn, B, |E|, nbr, flags -> cdotT
+                tot += K * (8 * (N + 2 * N + 3 * Re + 4 + 3 + N) + 4 * 4)
             elif lim:                # RK -> Taylor
-                tot += K * (8 * (N + comb + 2 * N + 3 * Re + 7 + N + 1) + 16)
+                tot += K * (8 * (N + comb + 2 * N + 3 * Re + 7 + N) + 16)
             else:                    # RK -> DG
                 tot += K * (8 * (N + comb + 2 * N + 3 * Re + 7 + N) + 16)
         elif kind == 'vertex' and lim:
             nv = 2 if p >= 2 else 1
             tot += nv * (4 * maxDeg * numV + 8 * K + 16 * numV)   # padded lists, centroids, bounds
-        elif kind == 'limit_dot' and lim and p >= 2:
-            tot += K * (8 * (N + 4 + N + comb + N + 1) + 12) + 16 * numV
+        elif kind == 'limit_dot' and lim and p >= 2:   # cdotT, B, c, RK operands -> Taylor
+            tot += K * (8 * (N + 4 + N + comb + N) + 12) + 16 * numV
         elif kind == 'limit' and lim:
             tot += K * (8 * (N + 4 + N) + 12) + 16 * numV
     return tot
