@@ -261,6 +261,16 @@ struct festung_problem {
   // raw vertex-patch min / max of the first nF Taylor coefficients just
   // written to `fields` (the centroid values first), and their global min /
   // max from the producer's per-block partials
+  // Captured step sequences bake in whether Dirichlet vertex data are used:
+  // a change invalidates them.
+  void set_has_vertexD(bool v) {
+    if (v == hasVertexD) return;
+    HIPCHK(hipStreamSynchronize(stream));
+    drop_graph();
+    ++version;
+    hasVertexD = v;
+  }
+
   void enqueue_vertex(const double* fields) {
     VertexArgs va{numV, maxDeg, v2tPad.i(), fields, K, nF, partials.d(), nblocks, gmm.d(), vmin.d(), vmax.d()};
     timed(FESTUNG_KERNEL_VERTEX, [&] { HIPCHK(launch_vertex(va, stream)); });
@@ -660,12 +670,9 @@ int festung_set_vertex_dirichlet_dev(festung_problem* h, const double* vertexD) 
     check_handle(h);
     h->set_device();
     if (!(h->flags & FESTUNG_FLAG_LIMITER)) fail(FESTUNG_ERR_STATE, "problem created without the limiter");
-    if (!vertexD) {
-      h->hasVertexD = false;
-      return;
-    }
+    h->set_has_vertexD(vertexD != nullptr);
+    if (!vertexD) return;
     HIPCHK(launch_permute(h->numV, 1, vertexD, h->dVperm.i(), h->vertexD.d(), h->stream));
-    h->hasVertexD = true;
   });
 }
 
@@ -695,14 +702,14 @@ int festung_set_vertex_dirichlet(festung_problem* h, const double* vertexD) {
     h->set_device();
     if (!(h->flags & FESTUNG_FLAG_LIMITER)) fail(FESTUNG_ERR_STATE, "problem created without the limiter");
     if (!vertexD) {
-      h->hasVertexD = false;
+      h->set_has_vertexD(false);
       return;
     }
     std::vector<double> v(h->numV);
     for (int32_t d = 0; d < h->numV; ++d) v[d] = vertexD[h->vperm[d]];
     sync(h);  // kernels in flight may still read the old values
     HIPCHK(hipMemcpy(h->vertexD.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
-    h->hasVertexD = true;
+    h->set_has_vertexD(true);
   });
 }
 

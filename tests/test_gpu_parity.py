@@ -218,6 +218,22 @@ def test_elem_block_invariance(gpu, p, ordRK, lim):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
+def test_vertex_dirichlet_change_recaptures_graph(gpu):
+    """Dropping the Dirichlet vertex data after whole steps were captured in a
+    graph: the next steps use the new limiter input (bit for bit as eager
+    steps do)."""
+    tc = TC.rotating_humps()
+    outs = []
+    for graph in (True, False):
+        d = product_problem(tc, 2, 1 / 16, 3, True, graph=graph)
+        product_step(d, 1)
+        d.problem.step(2, d.tau)
+        d.problem.set_vertex_dirichlet(None)
+        d.problem.step(2, d.tau)
+        outs.append(d.problem.get_state())
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_elem_block_rejects_invalid(gpu):
     from festung_amd._lib import FestungError
     d = product_problem(TC.rotating_humps(), 1, 1 / 8, 2, False)
