@@ -45,12 +45,11 @@ def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6):
     for (c0, ai, ao) in rk:
         comb = N * (c0 + ai + ao)    # RK-combination operands besides c and the output
         if kind == 'stage':
-            if lim and p >= 2:       # cdot -> Taylor: c, u1, u2, vn, B, |E|, nbr, flags -> cdotT
-                tot += K * (8 * (N + 2 * N + 3 * Re + 4 + 3 + N) + 4 * 4)
-            elif lim:                # RK -> Taylor
-                tot += K * (8 * (N + comb + 2 * N + 3 * Re + 7 + N) + 16)
-            else:                    # RK -> DG
-                tot += K * (8 * (N + comb + 2 * N + 3 * Re + 7 + N) + 16)
+            # edge flux factors |E| w_r u.nu: 3 Re; B: 4; nbr, flags: 16 B
+            if lim and p >= 2:       # cdot -> Taylor: c, u1, u2, flux factors, B, nbr, flags -> cdotT
+                tot += K * (8 * (N + 2 * N + 3 * Re + 4 + N) + 4 * 4)
+            else:                    # RK -> Taylor / DG
+                tot += K * (8 * (N + comb + 2 * N + 3 * Re + 4 + N) + 16)
         elif kind == 'vertex' and lim:
             nv = 2 if p >= 2 else 1
             tot += nv * (4 * maxDeg * numV + 8 * K + 16 * numV)   # padded lists, centroids, bounds

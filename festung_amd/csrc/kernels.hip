@@ -468,7 +468,6 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   const int fl = a.m.flags[kk];
 #pragma unroll 1
   for (int n = 0; n < 3; ++n) {
-    const double area = a.m.areaE[n * K + kk];
     const int nb = a.m.nbr[n * K + kk];
     const int np = flag_np(fl, n);
     const bool outOn = flag_out(fl, n);
@@ -479,7 +478,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
     }
 #pragma unroll
     for (int r = 0; r < Re; ++r) {
-      const double v = a.vn[(r * 3 + n) * K + kk];
+      const double v = a.vnw[(r * 3 + n) * K + kk];  // |E| w_r u.nu: the sign of u.nu
       double tr = 0.0;
       if (v > 0.0 && outOn) {
 #pragma unroll
@@ -489,7 +488,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 #pragma unroll
         for (int j = 0; j < N; ++j) tr += cn[j] * th[j * Re];
       }
-      const double fx = area * v * T[a.off.edgeQ + QIdx<P>::ew(n, r)] * tr;
+      const double fx = v * tr;
 #pragma unroll
       for (int i = 0; i < N; ++i) acc[i] -= T[a.off.edgeQ + QIdx<P>::ephi(n, r, i)] * fx;
     }
@@ -934,6 +933,23 @@ __global__ __launch_bounds__(BLOCK) void vnormal_kernel(int K, int Re, const dou
     for (int r = 0; r < Re; ++r) {
       const int idx = (r * 3 + n) * K + k;
       vn[idx] = n1 * u1e[idx] + n2 * u2e[idx];
+    }
+  }
+}
+
+// The stage kernel's edge-flux factors (|E_kn| u.nu) w_r, rounded as the
+// product area * v * w_r is (so the flux factor * trace gives the same bits).
+__global__ __launch_bounds__(BLOCK) void edge_flux_weights_kernel(int K, int Re, const double* __restrict__ areaE,
+                                                                  const double* __restrict__ vn,
+                                                                  const double* __restrict__ wEdge,
+                                                                  double* __restrict__ vnw) {
+  const int k = gtid();
+  if (k >= K) return;
+  for (int n = 0; n < 3; ++n) {
+    const double area = areaE[n * K + k];
+    for (int r = 0; r < Re; ++r) {
+      const int idx = (r * 3 + n) * K + k;
+      vnw[idx] = area * vn[idx] * wEdge[r];
     }
   }
 }
@@ -1565,6 +1581,12 @@ hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s) {
   }
 #undef L_STAGE
 #undef L_STAGE1
+  return hipGetLastError();
+}
+
+hipError_t launch_edge_flux_weights(int K, int Re, const double* areaE, const double* vn, const double* wEdge,
+                                    double* vnw, hipStream_t s) {
+  FESTUNG_LAUNCH(edge_flux_weights_kernel, grid_for(K), dim3(BLOCK), 0, s, K, Re, areaE, vn, wEdge, vnw);
   return hipGetLastError();
 }
 

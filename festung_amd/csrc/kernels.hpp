@@ -107,7 +107,7 @@ struct StageArgs {
   double* accOut;        // RK4 accumulator (out) or null
   const double* u1;
   const double* u2;
-  const double* vn;      // Re x 3 x K
+  const double* vnw;     // Re x 3 x K: |E_kn| w_r (u . nu)(x_knr), edge_flux_weights
   const double* rhs;     // N x K or null (globL - globKD - globKN)
   const double* tab;
   TabOff off;
@@ -219,6 +219,9 @@ struct SensArgs {
 };
 
 hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s);
+// vnw = |E_kn| w_r vn (the stage kernel's edge-flux factors; the sign is vn's)
+hipError_t launch_edge_flux_weights(int K, int Re, const double* areaE, const double* vn, const double* wEdge,
+                                    double* vnw, hipStream_t s);
 hipError_t launch_adj_stage(int p, const AdjArgs& a, hipStream_t s);
 hipError_t launch_sens_segment(int p, const SensArgs& a, hipStream_t s);
 // J partials: part[b] = sum over the block of det_k d_k^T hatM d_k, d = c - cObs;

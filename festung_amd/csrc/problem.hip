@@ -140,6 +140,7 @@ struct festung_problem {
   DBuf tauK;
   bool hasTauK = false;
   DBuf X, Y, Z, acc, T1, T2, partials, gmm, vmin, vmax, vertexD, u1, u2, vn, rhs, alpha;
+  DBuf vnw;  // the stage kernel's edge-flux factors |E| w_r vn, formed whenever vn is set
   int limType = LT_LINEAR;  // limiter type (core/applySlopeLimiterTaylor.m:69-85)
   int nF = 1;               // Taylor coefficients with vertex-patch bounds (limiter_fields)
   bool hasRhs = false, hasVertexD = false, hasVelocity = false;
@@ -271,6 +272,10 @@ struct festung_problem {
     hasVertexD = v;
   }
 
+  void refresh_vnw() {
+    HIPCHK(launch_edge_flux_weights(K, tabs.Re, area.d(), vn.d(), tab.d() + off.wEdge, vnw.d(), stream));
+  }
+
   void enqueue_vertex(const double* fields) {
     VertexArgs va{numV, maxDeg, v2tPad.i(), fields, K, nF, partials.d(), nblocks, gmm.d(), vmin.d(), vmax.d()};
     timed(FESTUNG_KERNEL_VERTEX, [&] { HIPCHK(launch_vertex(va, stream)); });
@@ -291,7 +296,7 @@ struct festung_problem {
     a.accOut = sp.accOut ? acc.d() : nullptr;
     a.u1 = u1.d();
     a.u2 = u2.d();
-    a.vn = vn.d();
+    a.vnw = vnw.d();
     a.rhs = hasRhs ? rhs.d() : nullptr;
     a.tab = tab.d();
     a.off = off;
@@ -533,6 +538,7 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     h->u1.alloc(nk);
     h->u2.alloc(nk);
     h->vn.alloc(size_t(3) * t.Re * K * sizeof(double));
+    h->vnw.alloc(size_t(3) * t.Re * K * sizeof(double));
     HIPCHK(hipMemset(h->X.p, 0, nk));
     // default elements per workgroup: the largest size that still gives at
     // least 1024 workgroups (4 per compute unit), 64 below that
@@ -593,6 +599,7 @@ int festung_set_velocity_disc(festung_problem* h, const double* u1Disc, const do
     h->upload_elem(h->u1, u1Disc, h->N, false);
     h->upload_elem(h->u2, u2Disc, h->N, false);
     h->upload_elem(h->vn, vNormal, 3 * h->tabs.Re, false);
+    h->refresh_vnw();
     h->hasVelocity = true;
   });
 }
@@ -613,6 +620,7 @@ int festung_set_velocity_points(festung_problem* h, const double* u1q, const dou
     h->upload_elem(e1, u1e, 3 * h->tabs.Re);
     h->upload_elem(e2, u2e, 3 * h->tabs.Re);
     HIPCHK(launch_vnormal(K, h->tabs.Re, h->nuE0T.d(), e1.d(), e2.d(), h->vn.d(), h->stream));
+    h->refresh_vnw();
     sync(h);
     h->hasVelocity = true;
   });
@@ -636,6 +644,7 @@ int festung_set_velocity_points_dev(festung_problem* h, const double* u1q, const
     HIPCHK(launch_permute(K, Re3, u1e, h->dEperm.i(), h->devE1.d(), h->stream));
     HIPCHK(launch_permute(K, Re3, u2e, h->dEperm.i(), h->devE2.d(), h->stream));
     HIPCHK(launch_vnormal(K, h->tabs.Re, h->nuE0T.d(), h->devE1.d(), h->devE2.d(), h->vn.d(), h->stream));
+    h->refresh_vnw();
     h->hasVelocity = true;
   });
 }
@@ -883,7 +892,7 @@ int festung_rhs(festung_problem* h, double* cDot) {
     a.c0 = h->X.d();
     a.u1 = h->u1.d();
     a.u2 = h->u2.d();
-    a.vn = h->vn.d();
+    a.vnw = h->vnw.d();
     a.rhs = h->hasRhs ? h->rhs.d() : nullptr;
     a.tab = h->tab.d();
     a.off = h->off;
@@ -1173,7 +1182,7 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
           }
           a.u1 = h->u1.d();
           a.u2 = h->u2.d();
-          a.vn = h->vn.d();
+          a.vnw = h->vnw.d();
           a.tab = h->tab.d();
           a.off = h->off;
           a.tau = tau;
