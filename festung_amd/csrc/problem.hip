@@ -1101,15 +1101,19 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
     if (h->flags & FESTUNG_FLAG_ADJ_DEFER) what = ADJ_DEFER;
     const bool defer = what == ADJ_DEFER;
     // Streams: A = the problem's stream (the reverse sweep); Cs = the
-    // recomputation of the next segment's stage states, into the other of
-    // two buffer sets, overlapping the sweep (unless FESTUNG_FLAG_ADJ_SERIAL).
+    // recomputation of the next segment's stage states into another buffer
+    // set, overlapping the sweep (unless FESTUNG_FLAG_ADJ_SERIAL); Ss = the
+    // deferred sensitivity passes, overlapping both.  A buffer set is reused
+    // once its segment is done (swept, and its sensitivity pass run): two
+    // sets, three with deferred sensitivities.
     const bool overlap = !(h->flags & FESTUNG_FLAG_ADJ_SERIAL) && nseg > 2;
-    hipStream_t A = h->stream, Cst = A;
+    hipStream_t A = h->stream, Cst = A, Sst = A;
     if (overlap) {
       h->ensure_aux_streams();
       Cst = h->aux[0];
+      if (defer) Sst = h->aux[1];
     }
-    const int nsets = overlap ? 2 : 1;
+    const int nsets = overlap ? (defer ? 3 : 2) : 1;
     std::vector<std::unique_ptr<DBuf>> ck(nseg + 1), sb(size_t(nsets) * C * S);
     for (auto& b : ck) {
       b.reset(new DBuf);
@@ -1121,8 +1125,10 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
         sb[j]->alloc(nk);
       }
     // ring of adjoint stage states: stage i reads ring[(i-1) mod R], writes
-    // ring[i mod R]; deferred sensitivities keep a whole segment's inputs
-    const int R = defer ? C * S + 2 : 4;
+    // ring[i mod R]; deferred sensitivities keep the inputs of as many
+    // segments as may be in flight (a segment's pass can overlap the sweeps
+    // of the next nsets - 1 segments)
+    const int R = defer ? nsets * C * S + 2 : 4;
     std::vector<std::unique_ptr<DBuf>> ringb(R);
     for (auto& b : ringb) {
       b.reset(new DBuf);
@@ -1135,17 +1141,18 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
     for (DBuf* b : {&L0, &Gu1, &Gu2, &tmp}) b->alloc(nk);
     Gvn.alloc(size_t(3) * Re * K * sizeof(double));
     // events: 0..2 timing; segment hand-offs between A and Cs
-    std::vector<hipEvent_t> ev(3, nullptr), evSeg(2, nullptr), evDone(2, nullptr), evFwd(1, nullptr);
+    std::vector<hipEvent_t> ev(3, nullptr), evSeg(nsets, nullptr), evDone(nsets, nullptr), evFwd(1, nullptr),
+        evSwept(nsets, nullptr), evFin(1, nullptr);
     struct EvGuard {
-      std::vector<hipEvent_t>* v[4];
+      std::vector<hipEvent_t>* v[6];
       ~EvGuard() {
         for (auto* x : v)
           for (auto e : *x)
             if (e) (void)hipEventDestroy(e);
       }
-    } evg{{&ev, &evSeg, &evDone, &evFwd}};
+    } evg{{&ev, &evSeg, &evDone, &evFwd, &evSwept, &evFin}};
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    for (auto* x : {&evSeg, &evDone, &evFwd})
+    for (auto* x : {&evSeg, &evDone, &evFwd, &evSwept, &evFin})
       for (auto& e : *x) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     auto state = [&](int seg, int l, int len, int st) -> double* {
@@ -1246,15 +1253,15 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
       // The forward pass leaves the stage states of the last nsets segments
       // in place; earlier ones are recomputed from their checkpoints.
       if (seg < nseg - nsets) {
-        if (overlap) HIPCHK(hipStreamWaitEvent(A, evSeg[seg % 2], 0));
+        if (overlap) HIPCHK(hipStreamWaitEvent(A, evSeg[seg % nsets], 0));
         else run_segment(seg, A, SEG_RECOMPUTE);
       }
-      // recompute the previous segment on Cs, into the other set, once the
-      // sweep over the segment that used that set (seg + 1) is done
+      // recompute the previous segment on Cs, into its set, once the segment
+      // that used that set last (seg - 1 + nsets) is done
       if (overlap && seg >= 1 && seg - 1 < nseg - nsets) {
-        HIPCHK(hipStreamWaitEvent(Cst, evDone[(seg + 1) % 2], 0));
+        HIPCHK(hipStreamWaitEvent(Cst, evDone[(seg - 1) % nsets], 0));
         run_segment(seg - 1, Cst, SEG_RECOMPUTE);
-        HIPCHK(hipEventRecord(evSeg[(seg - 1) % 2], Cst));
+        HIPCHK(hipEventRecord(evSeg[(seg - 1) % nsets], Cst));
       }
       for (int l = len - 1; l >= 0; --l) {
         for (int st = S - 1; st >= 0; --st, ++i) {
@@ -1281,6 +1288,12 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
         }
       }
       if (defer) {
+        hipStream_t sm = A;
+        if (overlap) {
+          HIPCHK(hipEventRecord(evSwept[seg % nsets], A));
+          HIPCHK(hipStreamWaitEvent(Sst, evSwept[seg % nsets], 0));
+          sm = Sst;
+        }
         SensArgs sa{};
         sa.m = h->mesh();
         sa.vn = h->vn.d();
@@ -1300,10 +1313,16 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
           sa.first = int(segFirst) + s0 * S;
           sa.count = ns * S;
           sa.nchunk = ns;
-          HIPCHK(launch_sens_segment(h->p, sa, A));
+          HIPCHK(launch_sens_segment(h->p, sa, sm));
         }
+        if (overlap) HIPCHK(hipEventRecord(evDone[seg % nsets], Sst));
+      } else if (overlap) {
+        HIPCHK(hipEventRecord(evDone[seg % nsets], A));
       }
-      if (overlap) HIPCHK(hipEventRecord(evDone[seg % 2], A));
+    }
+    if (defer && overlap) {  // the last sensitivity pass, before the sweep's end
+      HIPCHK(hipEventRecord(evFin[0], Sst));
+      HIPCHK(hipStreamWaitEvent(A, evFin[0], 0));
     }
     const double* muFinal = ring(i + R - 1);
     HIPCHK(launch_mass(h->p, K, h->B.d(), muFinal, h->tab.d(), h->off, tmp.d(), A));
