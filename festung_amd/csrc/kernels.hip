@@ -1128,14 +1128,17 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
   const int fl = a.m.flags[k];
 #pragma unroll(UNR ? 3 : 1)
   for (int n = 0; n < (ADJ || SENS_V ? 3 : 0); ++n) {
-    const double area = a.m.areaE[n * K + k];
+    // the transposed stage alone reads the edge-flux factors |E| w_r vn
+    // (the same products as area * v * w_r below)
+    constexpr bool FW = W == ADJ_STAGE;
+    const double area = FW ? 1.0 : a.m.areaE[n * K + k];
     const int nb = a.m.nbr[n * K + k];
     const int np = flag_np(fl, n);  // neighbour's local edge
     const bool outOn = flag_out(fl, n);
     // own edge points: symmetric outflow block, and the sensitivities
 #pragma unroll
     for (int r = 0; r < Re; ++r) {
-      const double v = a.vn[(r * 3 + n) * K + k];
+      const double v = FW ? a.vnw[(r * 3 + n) * K + k] : a.vn[(r * 3 + n) * K + k];
       const bool up = v > 0.0 && outOn;
       const bool in = v < 0.0 && nb >= 0;
       if ((ADJ && up) || (SENS_V && (up || in))) {
@@ -1144,7 +1147,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
         for (int i = 0; i < N; ++i) t += mu[i] * T[a.off.edgeQ + QIdx<P>::ephi(n, r, i)];
         const double wr = T[a.off.edgeQ + QIdx<P>::ew(n, r)];
         if (ADJ && up) {
-          const double fx = area * v * wr * t;
+          const double fx = FW ? v * t : area * v * wr * t;
 #pragma unroll
           for (int j = 0; j < N; ++j) acc[j] -= T[a.off.edgeQ + QIdx<P>::ephi(n, r, j)] * fx;
         }
@@ -1169,12 +1172,13 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
       for (int i = 0; i < N; ++i) mn[i] = a.mu[i * K + nb];
 #pragma unroll
       for (int r = 0; r < Re; ++r) {
-        const double v = a.vn[(r * 3 + np) * K + nb];
+        // the neighbour's factor carries the same edge length
+        const double v = FW ? a.vnw[(r * 3 + np) * K + nb] : a.vn[(r * 3 + np) * K + nb];
         if (v < 0.0) {
           double t = 0.0;
 #pragma unroll
           for (int i = 0; i < N; ++i) t += mn[i] * sPhi1D[r + i * Re + np * Re * N];
-          const double fx = area * v * T[a.off.edgeQ + QIdx<P>::ew(n, r)] * t;
+          const double fx = FW ? v * t : area * v * T[a.off.edgeQ + QIdx<P>::ew(n, r)] * t;
           const double* th = sTheta + r + np * Re * N + n * Re * N * 3;
 #pragma unroll
           for (int j = 0; j < N; ++j) acc[j] -= th[j * Re] * fx;

@@ -176,6 +176,7 @@ struct AdjArgs {
   const double* u1;
   const double* u2;
   const double* vn;
+  const double* vnw;     // the stage kernel's edge-flux factors |E| w_r vn (read by ADJ_STAGE)
   const double* tab;
   TabOff off;
   double tau;

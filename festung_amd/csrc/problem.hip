@@ -1063,6 +1063,7 @@ int festung_adj_rhs(festung_problem* h, const double* mu, double* out) {
     a.u1 = h->u1.d();
     a.u2 = h->u2.d();
     a.vn = h->vn.d();
+    a.vnw = h->vnw.d();
     a.tab = h->tab.d();
     a.off = h->off;
     a.mode = 4;
@@ -1271,6 +1272,7 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
           a.u1 = h->u1.d();
           a.u2 = h->u2.d();
           a.vn = h->vn.d();
+          a.vnw = h->vnw.d();
           a.tab = h->tab.d();
           a.off = h->off;
           a.tau = tau;
