@@ -1112,7 +1112,11 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
     if (overlap) {
       h->ensure_aux_streams();
       Cst = h->aux[0];
-      if (defer) Sst = h->aux[1];
+    }
+    // (with a single segment -- all states stored -- the passes overlap its sweep)
+    if (defer && !(h->flags & FESTUNG_FLAG_ADJ_SERIAL) && (overlap || nseg == 1)) {
+      h->ensure_aux_streams();
+      Sst = h->aux[1];
     }
     const int nsets = overlap ? (defer ? 3 : 2) : 1;
     std::vector<std::unique_ptr<DBuf>> ck(nseg + 1), sb(size_t(nsets) * C * S);
@@ -1264,6 +1268,34 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
         run_segment(seg - 1, Cst, SEG_RECOMPUTE);
         HIPCHK(hipEventRecord(evSeg[(seg - 1) % nsets], Cst));
       }
+      // deferred sensitivities: a pass over every nChunk swept steps (in
+      // sweep order), on Ss once those steps are swept
+      auto sens_pass = [&](long first, int nsteps_) {
+        if (Sst != A) {
+          HIPCHK(hipEventRecord(evSwept[seg % nsets], A));
+          HIPCHK(hipStreamWaitEvent(Sst, evSwept[seg % nsets], 0));
+        }
+        SensArgs sa{};
+        sa.m = h->mesh();
+        sa.vn = h->vn.d();
+        sa.tab = h->tab.d();
+        sa.off = h->off;
+        sa.tau = tau;
+        sa.tauK = h->hasTauK ? h->tauK.d() : nullptr;
+        sa.mus = static_cast<const double* const*>(dMus.p);
+        sa.cs = static_cast<const double* const*>(dCs.p);
+        sa.omegas = dOm.d();
+        sa.part = dPart.d();
+        sa.Gu1 = Gu1.d();
+        sa.Gu2 = Gu2.d();
+        sa.Gvn = Gvn.d();
+        sa.first = int(first);
+        sa.count = nsteps_ * S;
+        sa.nchunk = nsteps_;
+        HIPCHK(launch_sens_segment(h->p, sa, Sst));
+      };
+      long passFirst = segFirst;
+      int passSteps = 0;
       for (int l = len - 1; l >= 0; --l) {
         for (int st = S - 1; st >= 0; --st, ++i) {
           AdjArgs a{};
@@ -1288,41 +1320,15 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
           a.what = defer ? ADJ_STAGE : what;
           HIPCHK(launch_adj_stage(h->p, a, A));
         }
-      }
-      if (defer) {
-        hipStream_t sm = A;
-        if (overlap) {
-          HIPCHK(hipEventRecord(evSwept[seg % nsets], A));
-          HIPCHK(hipStreamWaitEvent(Sst, evSwept[seg % nsets], 0));
-          sm = Sst;
+        if (defer && (++passSteps == nChunk || l == 0)) {
+          sens_pass(passFirst, passSteps);
+          passFirst = i;
+          passSteps = 0;
         }
-        SensArgs sa{};
-        sa.m = h->mesh();
-        sa.vn = h->vn.d();
-        sa.tab = h->tab.d();
-        sa.off = h->off;
-        sa.tau = tau;
-        sa.tauK = h->hasTauK ? h->tauK.d() : nullptr;
-        sa.mus = static_cast<const double* const*>(dMus.p);
-        sa.cs = static_cast<const double* const*>(dCs.p);
-        sa.omegas = dOm.d();
-        sa.part = dPart.d();
-        sa.Gu1 = Gu1.d();
-        sa.Gu2 = Gu2.d();
-        sa.Gvn = Gvn.d();
-        for (int s0 = 0; s0 < len; s0 += nChunk) {  // steps s0 .. of the segment, in sweep order
-          const int ns = std::min(nChunk, len - s0);
-          sa.first = int(segFirst) + s0 * S;
-          sa.count = ns * S;
-          sa.nchunk = ns;
-          HIPCHK(launch_sens_segment(h->p, sa, sm));
-        }
-        if (overlap) HIPCHK(hipEventRecord(evDone[seg % nsets], Sst));
-      } else if (overlap) {
-        HIPCHK(hipEventRecord(evDone[seg % nsets], A));
       }
+      if (overlap) HIPCHK(hipEventRecord(evDone[seg % nsets], defer ? Sst : A));
     }
-    if (defer && overlap) {  // the last sensitivity pass, before the sweep's end
+    if (Sst != A) {  // the last sensitivity pass, before the sweep's end
       HIPCHK(hipEventRecord(evFin[0], Sst));
       HIPCHK(hipStreamWaitEvent(A, evFin[0], 0));
     }
