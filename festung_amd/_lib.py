@@ -75,6 +75,7 @@ SIGNATURES = {
     'festung_stage': (C.c_int, [vp, C.c_int, C.c_double]),
     'festung_step': (C.c_int, [vp, C.c_int, C.c_double]),
     'festung_limit': (C.c_int, [vp]),
+    'festung_limit_disc_linear': (C.c_int, [vp]),
     'festung_rhs': (C.c_int, [vp, dp]),
     'festung_limiter_info': (C.c_int, [vp, dp, dp, dp]),
     'festung_l2_error': (C.c_int, [vp, C.c_int, dp, dp]),

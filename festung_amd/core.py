@@ -303,6 +303,12 @@ class Problem:
     def limit(self):
         check(lib().festung_limit(self._h))
 
+    def limit_disc_linear(self):
+        """Limit the current state with the DG-basis linear limiter
+        (core/applySlopeLimiterDiscLinear.m:109-123); limiter_info() then
+        reports its alpha and bounds."""
+        check(lib().festung_limit_disc_linear(self._h))
+
     def rhs(self):
         out = np.empty((self.K, self.N), order='F')
         check(lib().festung_rhs(self._h, ptr(out)))

@@ -20,6 +20,8 @@
 //                      dc/dt mass-lumping correction (advection/solveSubStep.m:75-78),
 //                      change back to the DG basis, and for dc/dt the RK combination
 //   to_taylor_kernel   DG -> Taylor for a stand-alone limiter call
+//   centroid_kernel,   the limiter of the DG basis' linear part
+//   limit_disc_linear  (core/applySlopeLimiterDiscLinear.m)
 //   project / source / vnormal / l2 kernels for setup and checks.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
@@ -627,6 +629,20 @@ __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
 template <int P>
 constexpr int limit_min_waves() { return P == 3 ? 3 : P == 4 ? 2 : 1; }
 
+// Limiting factor of one vertex, core/computeVertexBasedLimiter.m:89-105:
+// the deviation d = val - cen of the vertex value is scaled back into
+// [lo - cen, hi - cen], with the reference's tolerance 1e-8 on both sides.
+__device__ __forceinline__ double vertex_alpha(double val, double cen, double lo, double hi) {
+  const double tol = 1.e-8;
+  const double d = val - cen;
+  const double dMin = lo - cen;
+  const double dMax = hi - cen;
+  double al = 1.0;
+  if (d < dMin + tol) al = fmax(0.0, fmin(1.0, dMin / (d - tol)));
+  if (d > dMax - tol) al = fmax(0.0, fmin(1.0, dMax / (d + tol)));
+  return al;
+}
+
 template <int P, int MODE, int LT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min_waves<P>()))) void limit_kernel(
     LimitArgs a) {
@@ -649,7 +665,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
   const double px[3] = {-ex, g.B11 - ex, g.B12 - ex};
   const double py[3] = {-ey, g.B21 - ey, g.B22 - ey};
   const double xc = x[0];
-  const double tol = 1.e-8;
   // patch bounds with the reference's global shift: min over the patch of
   // (c - sNeg), then + sNeg (and max with sPos); the Dirichlet vertex value
   // joins NaN-ignoring afterwards (core/computeMinMaxV0TElementPatch.m:75-82,107-110)
@@ -671,13 +686,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
         lo = fmin(lo, dv);
         hi = fmax(hi, dv);
       }
-      const double d = val - xc;
-      const double dMin = lo - xc;
-      const double dMax = hi - xc;
-      double al = 1.0;
-      if (d < dMin + tol) al = fmax(0.0, fmin(1.0, dMin / (d - tol)));
-      if (d > dMax - tol) al = fmax(0.0, fmin(1.0, dMax / (d + tol)));
-      alpha = fmin(alpha, al);
+      alpha = fmin(alpha, vertex_alpha(val, xc, lo, hi));
     }
     lim[0] = x[0];
     lim[1] = alpha * x[1];
@@ -778,13 +787,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
             lo = fmin(lo, dv);
             hi = fmax(hi, dv);
           }
-          const double d = val - cen;
-          const double dMin = lo - cen;
-          const double dMax = hi - cen;
-          double al = 1.0;
-          if (d < dMin + tol) al = fmax(0.0, fmin(1.0, dMin / (d - tol)));
-          if (d > dMax - tol) al = fmax(0.0, fmin(1.0, dMax / (d + tol)));
-          alphaTmp = fmin(alphaTmp, al);
+          alphaTmp = fmin(alphaTmp, vertex_alpha(val, cen, lo, hi));
         }
         alphaOrd = fmin(alphaOrd, alphaTmp);
       }
@@ -893,6 +896,63 @@ __global__ __launch_bounds__(BLOCK) void to_taylor_kernel(ToTaylorArgs a) {
     for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
   }
   sink_taylor<P>(Tc, active, a.sink);
+}
+
+// ---------------------------------------------------- DG-linear limit --
+// core/applySlopeLimiterDiscLinear.m:109-123.  Pass 1 (centroid_kernel): the
+// centroid values c_1 phi_1(1/3, 1/3) and their min / max partials; pass 2:
+// vertex_minmax_kernel on that one field; pass 3 (limit_disc_linear_kernel):
+// the vertex values of the DG data's linear part (core/projectDataDisc2DataLagr.m
+// at the reference vertices) against the patch bounds, the linear
+// coefficients scaled by alpha and the higher ones kept only where alpha = 1.
+
+__global__ __launch_bounds__(BLOCK) void centroid_kernel(int K, const double* __restrict__ c, double phi1c,
+                                                         double* __restrict__ cen, MinMaxSink sink) {
+  const int k = gtid();
+  const bool active = k < K;
+  const double v[1] = {active ? c[k] * phi1c : 0.0};
+  if (active) cen[k] = v[0];
+  sink_taylor<0>(v, active, sink);
+}
+
+template <int P>
+__global__ __launch_bounds__(BLOCK) void limit_disc_linear_kernel(DiscLinArgs a) {
+  constexpr int N = Dims<P>::N;
+  const int K = a.m.K;
+  const int k = gtid();
+  if (k >= K) return;
+  double x[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] = a.c[i * K + k];
+  const double xc = a.cen[k];
+  const double sNeg = fabs(a.gmm[1]) + 1.0;
+  const double sPos = fabs(a.gmm[0]) + 1.0;
+  double alpha = 1.0;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    double val;
+    {
+      // separately rounded products summed left to right, as
+      // core/projectDataDisc2DataLagr.m:53-56 accumulates them
+#pragma clang fp contract(off)
+      val = x[0] * a.phiV[n][0] + x[1] * a.phiV[n][1] + x[2] * a.phiV[n][2];
+    }
+    const int v = a.m.V0T[n * K + k];
+    double lo = (a.vmin[v] - sNeg) + sNeg;
+    double hi = (a.vmax[v] + sPos) - sPos;
+    if (a.vertexD) {
+      const double dv = a.vertexD[v];
+      lo = fmin(lo, dv);
+      hi = fmax(hi, dv);
+    }
+    alpha = fmin(alpha, vertex_alpha(val, xc, lo, hi));
+  }
+  a.c[K + k] = alpha * x[1];
+  a.c[2 * K + k] = alpha * x[2];
+  const double keep = alpha == 1.0 ? 1.0 : 0.0;
+#pragma unroll
+  for (int i = 3; i < N; ++i) a.c[i * K + k] = keep * x[i];
+  if (a.alphaOut) a.alphaOut[k] = alpha;
 }
 
 // ------------------------------------------------------ setup kernels --
@@ -1628,6 +1688,21 @@ hipError_t launch_to_taylor(int p, const ToTaylorArgs& a, hipStream_t s) {
   if (!valid_elem_block(BS)) return hipErrorInvalidValue;
   dim3 gr = grid_for(a.K, BS);
   FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((to_taylor_kernel<PP>), gr, dim3(BS), 0, s, a));
+  return hipGetLastError();
+}
+
+hipError_t launch_centroid(int K, int bs, const double* c, double phi1c, double* cen, const MinMaxSink& sink,
+                           hipStream_t s) {
+  if (!valid_elem_block(bs)) return hipErrorInvalidValue;
+  FESTUNG_LAUNCH(centroid_kernel, grid_for(K, bs), dim3(bs), 0, s, K, c, phi1c, cen, sink);
+  return hipGetLastError();
+}
+
+hipError_t launch_limit_disc_linear(int p, const DiscLinArgs& a, hipStream_t s) {
+  const int BS = a.m.bs;
+  if (!valid_elem_block(BS)) return hipErrorInvalidValue;
+  dim3 gr = grid_for(a.m.K, BS);
+  FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((limit_disc_linear_kernel<PP>), gr, dim3(BS), 0, s, a));
   return hipGetLastError();
 }
 

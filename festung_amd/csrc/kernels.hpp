@@ -165,6 +165,22 @@ struct ToTaylorArgs {
   int bs;
 };
 
+// Vertex-based limiter of the DG coefficients' linear part,
+// core/applySlopeLimiterDiscLinear.m: the vertex values of
+// sum_{j<3} c_j phi_j are limited against the patch bounds of the centroid
+// values c_1 phi_1(1/3, 1/3).
+struct DiscLinArgs {
+  Mesh m;
+  const double* cen;     // centroid values (K), the field the vertex pass reduced
+  const double* vmin;    // raw patch min / max of cen per vertex (numV)
+  const double* vmax;
+  const double* gmm;     // global min, max of cen
+  const double* vertexD; // Dirichlet vertex values (NaN = none) or null
+  double phiV[3][3];     // phi_(j+1) at reference vertex n: [n][j]
+  double* c;             // N x K DG coefficients, limited in place
+  double* alphaOut;      // K or null
+};
+
 // Adjoint stage in mu = M^-1 lambda space:
 //   y = (1 - omega)(mu + tau_k M^-1 A^T mu)        (mode 0..3)
 //   mode 0: out = y, L0 = omega mu;  mode 1: out = y, L0 += omega mu;
@@ -236,6 +252,10 @@ hipError_t launch_fill(size_t n, double v, double* x, hipStream_t s);
 hipError_t launch_vertex(const VertexArgs& a, hipStream_t s);
 hipError_t launch_limit(int p, int mode, int type, const LimitArgs& a, hipStream_t s);
 hipError_t launch_to_taylor(int p, const ToTaylorArgs& a, hipStream_t s);
+// centroid values c_1 phi1c -> cen (K) and their min / max partials
+hipError_t launch_centroid(int K, int bs, const double* c, double phi1c, double* cen, const MinMaxSink& sink,
+                           hipStream_t s);
+hipError_t launch_limit_disc_linear(int p, const DiscLinArgs& a, hipStream_t s);
 hipError_t launch_project(int p, bool ordF, int K, const double* fq, const double* tab, const TabOff& off,
                           double* out, hipStream_t s);
 hipError_t launch_vnormal(int K, int Re, const double* nuE0T, const double* u1e, const double* u2e, double* vn,

@@ -276,8 +276,8 @@ struct festung_problem {
     HIPCHK(launch_edge_flux_weights(K, tabs.Re, area.d(), vn.d(), tab.d() + off.wEdge, vnw.d(), stream));
   }
 
-  void enqueue_vertex(const double* fields) {
-    VertexArgs va{numV, maxDeg, v2tPad.i(), fields, K, nF, partials.d(), nblocks, gmm.d(), vmin.d(), vmax.d()};
+  void enqueue_vertex(const double* fields, int F = -1) {
+    VertexArgs va{numV, maxDeg, v2tPad.i(), fields, K, F < 0 ? nF : F, partials.d(), nblocks, gmm.d(), vmin.d(), vmax.d()};
     timed(FESTUNG_KERNEL_VERTEX, [&] { HIPCHK(launch_vertex(va, stream)); });
   }
 
@@ -873,6 +873,36 @@ int festung_limit(festung_problem* h) {
     l.sink = h->sink();
     l.alphaOut = h->alpha.d();
     HIPCHK(launch_limit(h->p, LIM_SOLUTION, h->limType, l, h->stream));
+    sync(h);
+    h->cur = 0;
+  });
+}
+
+int festung_limit_disc_linear(festung_problem* h) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    if (!(h->flags & FESTUNG_FLAG_LIMITER)) fail(FESTUNG_ERR_STATE, "problem created without the limiter");
+    h->finish_stage_sequence();
+    // centroid values c_1 phi_1(1/3, 1/3) -> T2 (field 0) and their partials
+    HIPCHK(launch_centroid(h->K, h->bs, h->X.d(), phi(1, 1.0 / 3.0, 1.0 / 3.0), h->T2.d(), MinMaxSink{h->partials.d(), 1},
+                           h->stream));
+    h->enqueue_vertex(h->T2.d(), 1);
+    DiscLinArgs l{};
+    l.m = h->mesh();
+    l.cen = h->T2.d();
+    l.vmin = h->vmin.d();
+    l.vmax = h->vmax.d();
+    l.gmm = h->gmm.d();
+    l.vertexD = h->hasVertexD ? h->vertexD.d() : nullptr;
+    // the reference element's vertices (0,0), (1,0), (0,1),
+    // core/projectDataDisc2DataLagr.m:50
+    const double L1[3] = {0.0, 1.0, 0.0}, L2[3] = {0.0, 0.0, 1.0};
+    for (int n = 0; n < 3; ++n)
+      for (int j = 0; j < 3; ++j) l.phiV[n][j] = phi(j + 1, L1[n], L2[n]);
+    l.c = h->X.d();
+    l.alphaOut = h->alpha.d();
+    HIPCHK(launch_limit_disc_linear(h->p, l, h->stream));
     sync(h);
     h->cur = 0;
   });

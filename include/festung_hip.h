@@ -258,8 +258,19 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
 int festung_adj_rhs(festung_problem* h, const double* mu, double* out);
 
 /* Apply the slope limiter to the current state.
- * Replaces core/applySlopeLimiterDisc.m:77-86 ('linear'). */
+ * Replaces core/applySlopeLimiterDisc.m:77-86 (the limiter type chosen by
+ * the FESTUNG_FLAG_LIM_* flags at creation: 'linear', 'hierarch_vert' or
+ * 'strict'). */
 int festung_limit(festung_problem* h);
+
+/* Apply the vertex-based limiter of the DG basis' linear part to the
+ * current state: vertex values of sum_{j<=3} c_j phi_j against the patch
+ * bounds of the centroid values c_1 phi_1(1/3, 1/3) (with the Dirichlet
+ * vertex values, if set, as for festung_limit), then
+ * [c_1, alpha c_2:3, (alpha == 1) c_4:N].  Needs p >= 1 and a problem
+ * created with FESTUNG_FLAG_LIMITER; festung_limiter_info then reports its
+ * alpha and bounds.  Replaces core/applySlopeLimiterDiscLinear.m:109-123. */
+int festung_limit_disc_linear(festung_problem* h);
 
 /* Evaluate the DG right-hand side cDiscDot = M \ (sysV - sysA c) for the
  * current state without limiting (K x N) (advection/solveSubStep.m:62-66). */

@@ -6,7 +6,8 @@ core/computeMinMaxV0TElementPatch.m, core/computeVertexBasedLimiter.m,
 core/applySlopeLimiterTaylorLinear.m,
 core/applySlopeLimiterTaylorHierarchicalVertex.m,
 core/applySlopeLimiterTaylorStrict.m, core/multiindex.m, core/mult2ind.m,
-core/applySlopeLimiterTaylor.m and core/applySlopeLimiterDisc.m.
+core/applySlopeLimiterTaylor.m, core/applySlopeLimiterDisc.m,
+core/projectDataDisc2DataLagr.m and core/applySlopeLimiterDiscLinear.m.
 
 Pinned by: brute-force patch min/max on small meshes (dense markV0TV0T path),
 the invariants of the limiter (cell means unchanged; alpha in [0, 1]; linear
@@ -17,13 +18,18 @@ and the mean-free property of the Taylor basis.  The hierarchical and strict
 types are pinned by their reduction to the linear limiter at p = 1, by a
 loop-based evaluation on small meshes that forms every patch, vertex value
 and Taylor basis value directly from coordinates, and by their invariants
-(cell means unchanged, data inside its bounds untouched).
+(cell means unchanged, data inside its bounds untouched).  The limiter of
+the DG basis' linear part is pinned by its vertex values reproducing a
+linear function at the mesh vertices from coordinates, its centroid value
+reproducing the function at the centroid, its agreement with the Taylor
+'linear' limiter at p = 1 (where both limit the same linear polynomial) and
+its invariants (c_1 kept, higher coefficients dropped exactly where alpha < 1).
 """
 import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from .basis import p_of_N
+from .basis import p_of_N, phi
 from .grid import mapRef2Phy
 from .quadrature import quadRule2D
 from .assembly import computeFuncDiscAtPoints
@@ -266,3 +272,32 @@ def applySlopeLimiterDisc(g, dataDisc, markV0TbdrD, dataV0T, globM, globMDiscTay
     dataTaylor = projectDataDisc2DataTaylor(dataDisc, globM, globMDiscTaylor)
     dataTaylor, minMax = applySlopeLimiterTaylor(g, dataTaylor, markV0TbdrD, dataV0T, basesOnQuad, type_)
     return projectDataTaylor2DataDisc(dataTaylor, globM, globMDiscTaylor), minMax
+
+
+def projectDataDisc2DataLagr(dataDisc, p=None):
+    """DG data -> values at the Lagrange nodes of degree min(p, 2) on every
+    element, products summed in basis order; core/projectDataDisc2DataLagr.m:43-57."""
+    K, N = dataDisc.shape
+    if p is None:
+        p = int(round((np.sqrt(8 * N + 1) - 3) / 2))
+    if p == 0:
+        L1, L2 = [1 / 3], [1 / 3]
+    elif p == 1:
+        L1, L2 = [0, 1, 0], [0, 0, 1]
+    else:
+        L1, L2 = [0, 1, 0, 1 / 2, 0, 1 / 2], [0, 0, 1, 1 / 2, 1 / 2, 0]
+    out = np.zeros((K, len(L1)))
+    for i in range(N):
+        out = out + dataDisc[:, i:i + 1] * phi(i + 1, L1, L2)[None, :]
+    return out
+
+
+def applySlopeLimiterDiscLinear(g, dataDisc, markV0TbdrD, dataV0T):
+    """Vertex-based limiter of the DG basis' linear part;
+    core/applySlopeLimiterDiscLinear.m:109-123."""
+    assert dataDisc.shape[1] >= 3, 'needs p >= 1'
+    valV0T = projectDataDisc2DataLagr(dataDisc[:, 0:3])
+    alphaE, minMax = computeVertexBasedLimiter(g, dataDisc[:, 0] * phi(1, 1 / 3, 1 / 3), valV0T, markV0TbdrD, dataV0T)
+    lim = np.column_stack([dataDisc[:, 0], alphaE[:, None] * dataDisc[:, 1:3],
+                           (alphaE == 1)[:, None] * dataDisc[:, 3:]])
+    return lim, minMax, alphaE

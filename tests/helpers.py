@@ -48,3 +48,9 @@ def oracle_limit(pd, c, t):
     cDV0T = OAs.computeFuncContV0T(pd.g, lambda x1, x2: pd.cDCont(t, x1, x2))
     return OL.applySlopeLimiterDisc(pd.g, c, pd.g.markV0TbdrD, cDV0T, pd.globM, pd.globMDiscTaylor,
                                     pd.basesOnQuad, pd.typeSlopeLim)
+
+
+def oracle_limit_disc_linear(pd, c, t):
+    """core/applySlopeLimiterDiscLinear.m with the problem's Dirichlet vertex data."""
+    cDV0T = OAs.computeFuncContV0T(pd.g, lambda x1, x2: pd.cDCont(t, x1, x2))
+    return OL.applySlopeLimiterDiscLinear(pd.g, c, pd.g.markV0TbdrD, cDV0T)

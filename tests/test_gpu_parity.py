@@ -8,7 +8,8 @@ import pytest
 
 from festung_amd.advection import steps as S
 from festung_amd.advection import testcases as TC
-from helpers import oracle_problem, oracle_stage_operators, oracle_step, oracle_limit, rel_l2
+from helpers import (oracle_problem, oracle_stage_operators, oracle_step, oracle_limit,
+                     oracle_limit_disc_linear, rel_l2)
 from oracle import advection as OA
 
 pytestmark = pytest.mark.gpu
@@ -143,6 +144,34 @@ def test_limiter_types_alone(gpu, type_, p):
     assert rel_l2(got, ref) < TOL
     assert np.max(np.abs(gmn - mn)) < 1e-14 and np.max(np.abs(gmx - mx)) < 1e-14
     assert np.all((alpha >= 0) & (alpha <= 1))
+
+
+@pytest.mark.parametrize('p', [1, 2, 3, 4])
+def test_limiter_disc_linear_alone(gpu, p):
+    """core/applySlopeLimiterDiscLinear.m on a rough state with the
+    problem's Dirichlet vertex data: bit-identical result, alpha and bounds
+    (the same IEEE operations in the same order as the oracle)."""
+    tc = TC.rotating_humps()
+    o = oracle_problem(tc, p, 1 / 8, min(p + 1, 3), True)
+    d = product_problem(tc, p, 1 / 8, min(p + 1, 3), True)
+    rng = np.random.default_rng(61 + p)
+    c = o.cDisc + 0.05 * rng.standard_normal(o.cDisc.shape)
+    ref, (mn, mx), aref = oracle_limit_disc_linear(o, c, 0.0)
+    assert 0 < np.sum(aref < 1) < o.K
+    d.problem.set_state(c)
+    d.problem.limit_disc_linear()
+    got = d.problem.get_state()
+    alpha, gmn, gmx = d.problem.limiter_info()
+    assert np.array_equal(alpha, aref)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(gmn, mn) and np.array_equal(gmx, mx)
+
+
+def test_limiter_disc_linear_needs_limiter(gpu):
+    from festung_amd._lib import FestungError
+    d = product_problem(TC.rotating_humps(), 2, 1 / 4, 3, False)
+    with pytest.raises(FestungError, match='without the limiter'):
+        d.problem.limit_disc_linear()
 
 
 @pytest.mark.parametrize('type_', ['hierarch_vert', 'strict'])

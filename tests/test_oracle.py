@@ -479,6 +479,77 @@ def test_limiter_keeps_linear_data_inside_bounds():
     assert np.all(alpha[inside] == 1)
 
 
+@pytest.mark.parametrize('p', [1, 3])
+def test_disc_linear_vertex_and_centroid_values(p):
+    """DG data of a linear function: the Lagrange values at the vertices are
+    the function at the mesh vertices (from coordinates), the centroid value
+    c_1 phi_1(1/3, 1/3) the function at the barycentre."""
+    pd = _setup(p, h=1 / 4, lim=True)
+    f = lambda x1, x2: 0.7 - 1.3 * x1 + 0.4 * x2
+    c = A.projectFuncCont2DataDisc(pd.g, f, 2 * p + 1, pd.hatM, pd.basesOnQuad)
+    g = pd.g
+    vals = L.projectDataDisc2DataLagr(c[:, :3])
+    want = f(g.coordV0T[:, :, 0], g.coordV0T[:, :, 1])
+    assert np.max(np.abs(vals - want)) < 1e-13
+    cen = c[:, 0] * B.phi(1, 1 / 3, 1 / 3)
+    assert np.max(np.abs(cen - f(g.baryT[:, 0], g.baryT[:, 1]))) < 1e-13
+    # at p >= 2 the quadratic nodes follow the vertices: the edge midpoints
+    # 23, 31, 12 (opposite vertex 1, 2, 3)
+    f2 = lambda x1, x2: f(x1, x2) + 0.9 * x1 * x2 - 0.5 * x2 ** 2
+    c2 = A.projectFuncCont2DataDisc(pd.g, f2, 2 * p + 1, pd.hatM, pd.basesOnQuad)[:, :6]
+    v2 = L.projectDataDisc2DataLagr(c2, 2)
+    X = g.coordV0T
+    mids = [(X[:, 1] + X[:, 2]) / 2, (X[:, 2] + X[:, 0]) / 2, (X[:, 0] + X[:, 1]) / 2]
+    want2 = np.column_stack([f2(X[:, n, 0], X[:, n, 1]) for n in range(3)]
+                            + [f2(m[:, 0], m[:, 1]) for m in mids])
+    if p >= 2:
+        assert np.max(np.abs(v2 - want2)) < 1e-12
+
+
+def test_disc_linear_equals_taylor_linear_at_p1():
+    """At p = 1 the DG and Taylor data describe the same linear polynomial,
+    so the two limiters agree to rounding (alpha, bounds and result)."""
+    pd = _setup(1, h=1 / 5, lim=True)
+    rng = np.random.default_rng(31)
+    c = rng.standard_normal((pd.K, pd.N))
+    cD = np.where(pd.g.markV0TbdrD, rng.standard_normal((pd.K, 3)), np.nan)
+    a, (mna, mxa), alpha = L.applySlopeLimiterDiscLinear(pd.g, c, pd.g.markV0TbdrD, cD)
+    b, (mnb, mxb) = L.applySlopeLimiterDisc(pd.g, c, pd.g.markV0TbdrD, cD, pd.globM, pd.globMDiscTaylor,
+                                            pd.basesOnQuad, 'linear')
+    assert 0 < np.sum(alpha < 1) < pd.K
+    assert np.max(np.abs(a - b)) < 1e-12
+    assert np.max(np.abs(mna - mnb)) < 1e-13 and np.max(np.abs(mxa - mxb)) < 1e-13
+
+
+def test_disc_linear_bruteforce_and_invariants():
+    """p = 3: patch bounds by loops over the elements, c_1 kept, c_2:3 scaled
+    by alpha in [0, 1], c_4: kept exactly where alpha = 1 and dropped
+    elsewhere, limited vertex values inside the bounds (up to the tolerance)."""
+    pd = _setup(3, h=1 / 4, lim=True)
+    rng = np.random.default_rng(33)
+    g = pd.g
+    c = A.projectFuncCont2DataDisc(g, pd.c0Cont, 7, pd.hatM, pd.basesOnQuad)
+    c = c + 0.05 * rng.standard_normal(c.shape)
+    cD = np.where(g.markV0TbdrD, 0.25, np.nan)
+    lim, (mn, mx), alpha = L.applySlopeLimiterDiscLinear(g, c, g.markV0TbdrD, cD)
+    cen = np.sqrt(2) * c[:, 0]
+    for k in range(g.numT):
+        for n in range(3):
+            v = g.V0T[k, n]
+            patch = [kk for kk in range(g.numT) if v in g.V0T[kk]]
+            lo, hi = cen[patch].min(), cen[patch].max()
+            if g.markV0TbdrD[k, n]:
+                lo, hi = min(lo, 0.25), max(hi, 0.25)
+            assert abs(mn[k, n] - lo) < 1e-13 and abs(mx[k, n] - hi) < 1e-13
+    assert np.all((alpha >= 0) & (alpha <= 1)) and 0 < np.sum(alpha < 1) < pd.K
+    assert np.array_equal(lim[:, 0], c[:, 0])
+    assert np.array_equal(lim[:, 1:3], alpha[:, None] * c[:, 1:3])
+    one = alpha == 1
+    assert np.array_equal(lim[one, 3:], c[one, 3:]) and np.all(lim[~one, 3:] == 0)
+    vals = L.projectDataDisc2DataLagr(lim[:, :3])
+    assert np.all(vals <= mx + 1e-8 + 1e-12) and np.all(vals >= mn - 1e-8 - 1e-12)
+
+
 def test_multiindex_and_mult2ind():
     """core/multiindex.m order (degree, then the second exponent) and its
     inverse core/mult2ind.m."""
