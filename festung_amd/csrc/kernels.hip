@@ -123,7 +123,8 @@ __device__ __forceinline__ void dg_to_taylor(const double (&x)[Dims<P>::N], cons
   for (int r = 0; r < N; ++r) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) s += F[r * N + j] * x[j];
+    for (int j = 0; j < N; ++j)
+      if (taylor_nz(false, P, r, j)) s += F[r * N + j] * x[j];
     y[r] = s;
   }
   const double s = 1.0 / (g.det * g.dX);
@@ -150,7 +151,8 @@ __device__ __forceinline__ void taylor_to_dg(const double (&T)[Dims<P>::N], cons
   for (int i = 0; i < N; ++i) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) s += Finv[i * N + j] * z[j];
+    for (int j = 0; j < N; ++j)
+      if (taylor_nz(true, P, i, j)) s += Finv[i * N + j] * z[j];
     x[i] = s;
   }
 }

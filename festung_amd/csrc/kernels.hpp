@@ -15,6 +15,32 @@ struct Dims {
   static constexpr int Re = P + 1;  // points of quadRule1D(2P + 1)
 };
 
+// Nonzero pattern of the change to the Taylor basis (bit j of row r set when
+// entry (r, j) may be nonzero), for F (DG -> Taylor: derivatives of the DG
+// functions at the centroid) and its inverse.  A Taylor row of degree d only
+// involves DG functions of degree >= d, and several derivatives of the
+// Dubiner functions vanish at the centroid (1/3, 1/3), so 137 of the 225
+// entries of F are zero at p = 4.  The kernels skip the zero products (the
+// sums are unchanged: s + 0 x = s); the host checks each table against the
+// pattern when it builds it.
+inline constexpr uint16_t kTaylorMaskF[5][15] = {
+    {0x0001},
+    {0x0001, 0x0006, 0x0004},
+    {0x0001, 0x003e, 0x0034, 0x0038, 0x0020, 0x0030},
+    {0x0001, 0x03fe, 0x03b4, 0x03f8, 0x03a0, 0x0330, 0x03c0, 0x0380, 0x0300, 0x0200},
+    {0x0001, 0x7ffe, 0x7bb4, 0x7ff8, 0x7ba0, 0x7330, 0x77c0, 0x7380, 0x7300, 0x2200, 0x7c00, 0x7800, 0x7000,
+     0x6000, 0x4000}};
+inline constexpr uint16_t kTaylorMaskFinv[5][15] = {
+    {0x0001},
+    {0x0001, 0x0006, 0x0004},
+    {0x0001, 0x003e, 0x0034, 0x0038, 0x0030, 0x0010},
+    {0x0001, 0x03fe, 0x03b4, 0x03f8, 0x03b0, 0x0390, 0x03c0, 0x0380, 0x0300, 0x0200},
+    {0x0001, 0x7ffe, 0x7bb4, 0x7ff8, 0x7bb0, 0x7b90, 0x7fc0, 0x6380, 0x7300, 0x6200, 0x7c00, 0x7800, 0x7000,
+     0x6000, 0x4000}};
+__host__ __device__ constexpr bool taylor_nz(bool inv, int p, int r, int j) {
+  return ((inv ? kTaylorMaskFinv[p][r] : kTaylorMaskF[p][r]) >> j) & 1;
+}
+
 // The stage kernel's element term on the matrix cores (FP64 16x16x4 MFMA)
 // when the element rule splits into 16-point blocks and the basis fits 16
 // rows: p = 4 (64 points, N = 15).  Lower degrees stay on the vector ALU,

@@ -444,6 +444,10 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
           mi[size_t(i) * N + j] = t.hatMinv[i + size_t(j) * N];
           fr[size_t(i) * N + j] = t.F[i + size_t(j) * N];
           fir[size_t(i) * N + j] = t.Finv[i + size_t(j) * N];
+          // the kernels skip the products outside the compile-time pattern
+          if ((fr[size_t(i) * N + j] != 0 && !taylor_nz(false, p, i, j)) ||
+              (fir[size_t(i) * N + j] != 0 && !taylor_nz(true, p, i, j)))
+            fail(FESTUNG_ERR_INTERNAL, "Taylor change of basis (%d, %d) outside its nonzero pattern", i, j);
         }
       o.elemQ = put(eq);
       o.edgeQ = put(ed);
