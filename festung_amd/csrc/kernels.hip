@@ -40,11 +40,11 @@ constexpr int BLOCK = 256;
 // on a compact part of the mesh and neighbour / vertex-patch gathers across
 // block boundaries hit its own L2.
 constexpr int kXCDs = 8;
-__device__ __forceinline__ int logical_block() {
-  const int nb = gridDim.x, b = blockIdx.x;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
   const int full = nb - nb % kXCDs;
   return b < full ? (b % kXCDs) * (full / kXCDs) + b / kXCDs : b;
 }
+__device__ __forceinline__ int logical_block() { return xcd_block(blockIdx.x, gridDim.x); }
 __device__ __forceinline__ int gtid() { return logical_block() * int(blockDim.x) + threadIdx.x; }
 
 // ---------------------------------------------------------------- Taylor --
@@ -599,8 +599,10 @@ __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
       }
       __syncthreads();  // smn / smx are reused for the next coefficient
     }
+    return;
   }
-  const int v = gtid();
+  // workgroups 1 .. numV / BLOCK: one vertex per thread
+  const int v = xcd_block(blockIdx.x - 1, gridDim.x - 1) * int(blockDim.x) + threadIdx.x;
   if (v >= a.numV) return;
   for (int j0 = 0; j0 < a.maxDeg; j0 += 8) {
     int id[8];
@@ -1657,7 +1659,8 @@ hipError_t launch_edge_flux_weights(int K, int Re, const double* areaE, const do
 }
 
 hipError_t launch_vertex(const VertexArgs& a, hipStream_t s) {
-  FESTUNG_LAUNCH(vertex_minmax_kernel, grid_for(a.numV), dim3(BLOCK), 0, s, a);
+  // workgroup 0 folds the partials, the others take the vertices
+  FESTUNG_LAUNCH(vertex_minmax_kernel, dim3(grid_for(a.numV).x + 1), dim3(BLOCK), 0, s, a);
   return hipGetLastError();
 }
 
