@@ -163,6 +163,18 @@ def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6):
             'adjoint_forward_ms': r['forward_ms'], 'adjoint_reverse_ms': r['backward_ms'],
             'cost_vs_forward': (r['forward_ms'] + r['backward_ms']) / fwd_ms}
         del mf
+    # binomial checkpointing with as many state snapshots as the uniform
+    # schedule keeps segment checkpoints (nsteps / ckpt + 1), and with a few
+    from festung_amd.core import binomial_schedule
+    for snaps in (nsteps // max(ckpt, 1) + 1, 4):
+        mb = AD.StreamModel(hmax, p, modes=modes, c0=TC.gaussian_humps(51), adj_kernel='fused', adj_binomial=True)
+        mb.gradient(mb.c0, a, 2, tau, cObs, ckpt=2)  # warm-up
+        r = mb.gradient(mb.c0, a, nsteps, tau, cObs, ckpt=snaps)[3]
+        adv = binomial_schedule(nsteps, snaps)[0]
+        out['binomial_%d_snapshots' % snaps] = {
+            'forward_steps': nsteps + adv, 'adjoint_forward_ms': r['forward_ms'],
+            'adjoint_reverse_ms': r['backward_ms'], 'cost_vs_forward': (r['forward_ms'] + r['backward_ms']) / fwd_ms}
+        del mb
     rng = np.random.default_rng(54)
     errs = []
     for _ in range(ndir):

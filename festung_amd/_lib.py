@@ -21,6 +21,7 @@ FLAG_ADJ_SPLIT = 16
 FLAG_ADJ_DEFER = 32
 FLAG_LIM_HIERARCH = 64
 FLAG_LIM_STRICT = 128
+FLAG_ADJ_BINOMIAL = 256
 LIMITER_TYPES = {'linear': 0, 'hierarch_vert': FLAG_LIM_HIERARCH, 'strict': FLAG_LIM_STRICT}
 
 
@@ -88,6 +89,7 @@ SIGNATURES = {
     'festung_batch_sync': (C.c_int, [vp]),
     'festung_batch_destroy': (None, [vp]),
     'festung_adjoint': (C.c_int, [vp, C.c_int32, C.c_double, dp, C.c_int32, dp, dp, dp, dp, dp, dp]),
+    'festung_binomial_schedule': (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'festung_adj_rhs': (C.c_int, [vp, dp, dp]),
     'festung_profile': (C.c_int, [vp, C.c_int]),
     'festung_profile_read': (C.c_int, [vp, dp, C.POINTER(C.c_int64)]),

@@ -45,10 +45,12 @@ def rotation(x1, x2):
 class StreamModel:
     """Device problem (limiter off) whose velocity is u0 + sum a_m u_m."""
 
-    def __init__(self, hmax, p, ordRK=3, modes=None, c0=None, adj_serial=False, adj_kernel=None):
+    def __init__(self, hmax, p, ordRK=3, modes=None, c0=None, adj_serial=False, adj_kernel=None,
+                 adj_binomial=False):
         self.g = Grid.domainSquare(hmax)
         self.p = p
-        self.prob = Problem(self.g, p, ordRK, adj_serial=adj_serial, adj_kernel=adj_kernel)
+        self.prob = Problem(self.g, p, ordRK, adj_serial=adj_serial, adj_kernel=adj_kernel,
+                            adj_binomial=adj_binomial)
         self.modes = modes if modes is not None else stream_modes()
         pr = self.prob
         Q1, Q2, _ = quadRule2D(pr.ordE)

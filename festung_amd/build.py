@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, 'libfestung_hip.so')
 ROCM = os.environ.get('ROCM_PATH', '/opt/rocm')
 ARCH = 'gfx950'
 
-HOST_SRCS = ['tables.cpp', 'grid.cpp', 'capi.cpp']
+HOST_SRCS = ['tables.cpp', 'grid.cpp', 'checkpoint.cpp', 'capi.cpp']
 HIP_SRCS = ['kernels.hip', 'problem.hip']
 
 CXXFLAGS = ['-O2', '-std=c++17', '-fPIC', '-Wall', '-Wno-unused-function', '-I' + INCLUDE, '-I' + CSRC]

@@ -145,23 +145,26 @@ class Problem:
     """Device-resident advection problem (festung_problem_* of the C-ABI)."""
 
     def __init__(self, grid, p, ordRK, limiter=False, graph=False, adj_serial=False, adj_kernel=None,
-                 limiter_type='linear'):
+                 limiter_type='linear', adj_binomial=False):
         """limiter_type: 'linear', 'hierarch_vert' or 'strict'
         (core/applySlopeLimiterTaylor.m:69-85); adj_serial: recompute
         checkpointed segments on the problem's own stream; adj_kernel: None (by
         mesh size), 'fused', 'split' or 'deferred' -- where the adjoint forms
         the velocity sensitivities (in the transposed stage kernel's thread, on
         two more threads per element, or once per checkpoint segment after its
-        sweep)."""
+        sweep); adj_binomial: binomial checkpointing in adjoint() (its ckpt
+        argument is then the number of state snapshots)."""
         if adj_kernel not in (None, 'fused', 'split', 'deferred'):
             raise ValueError('adj_kernel must be None, "fused", "split" or "deferred"')
+        if adj_binomial and adj_kernel == 'deferred':
+            raise ValueError('binomial checkpointing forms the sensitivities fused or split')
         if limiter_type not in L.LIMITER_TYPES:
             raise ValueError('Unknown limiter type %r' % (limiter_type,))
         flags = ((L.FLAG_LIMITER if limiter else 0) | (L.FLAG_GRAPH if graph else 0)
                  | (L.FLAG_ADJ_SERIAL if adj_serial else 0)
                  | {None: 0, 'fused': L.FLAG_ADJ_FUSED, 'split': L.FLAG_ADJ_SPLIT,
                     'deferred': L.FLAG_ADJ_DEFER}[adj_kernel]
-                 | L.LIMITER_TYPES[limiter_type])
+                 | L.LIMITER_TYPES[limiter_type] | (L.FLAG_ADJ_BINOMIAL if adj_binomial else 0))
         h = C.c_void_p()
         check(lib().festung_problem_create(grid._h, int(p), int(ordRK), flags, C.byref(h)))
         self._h = h
@@ -328,9 +331,11 @@ class Problem:
         return e.value
 
     def adjoint(self, nsteps, tau, cObs, ckpt=0):
-        """Forward nsteps from the current state, loss and reverse sweep.
-        Returns dict(J, gradC0 (K x N), gradU1, gradU2 (K x N), gradVn
-        (K x 3 x Re), forward_ms, backward_ms)."""
+        """Forward nsteps from the current state, loss and reverse sweep;
+        ckpt: steps per checkpoint segment (0: all stored), or the number of
+        state snapshots with binomial checkpointing.  Returns dict(J, gradC0
+        (K x N), gradU1, gradU2 (K x N), gradVn (K x 3 x Re), forward_ms,
+        backward_ms)."""
         K, N, Re = self.K, self.N, self.Re
         o = fortran(cObs, (K, N))
         J = np.zeros(1)
@@ -389,3 +394,12 @@ class Batch:
     def __del__(self):
         if L._lib is not None and getattr(self, '_h', None) is not None:
             self.close()
+
+
+def binomial_schedule(nsteps, snapshots):
+    """(forward steps besides the per-step recomputations, actions) of the
+    binomial checkpointing schedule for nsteps steps and the given number of
+    state snapshots (festung_binomial_schedule)."""
+    adv = C.c_int64(); act = C.c_int64()
+    check(lib().festung_binomial_schedule(int(nsteps), int(snapshots), C.byref(adv), C.byref(act)))
+    return adv.value, act.value

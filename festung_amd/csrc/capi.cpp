@@ -1,8 +1,10 @@
 // Host-only part of the C-ABI: error reporting, grids and reference tables.
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 
+#include "checkpoint.hpp"
 #include "common.hpp"
 #include "grid.hpp"
 #include "tables.hpp"
@@ -265,3 +267,20 @@ int festung_ref_get(int p, const char* name, int ord, double* out) {
 }
 
 }  // extern "C"
+
+int festung_binomial_schedule(int32_t nsteps, int32_t snapshots, int64_t* advances, int64_t* actions) {
+  return guard([&] {
+    if (nsteps < 1 || snapshots < 1 || !advances || !actions) fail(FESTUNG_ERR_ARG, "invalid argument");
+    std::vector<CkAction> plan;
+    try {
+      plan = binomial_schedule(nsteps, snapshots);
+    } catch (const std::exception& e) {
+      fail(FESTUNG_ERR_INTERNAL, "%s", e.what());
+    }
+    int64_t adv = 0;
+    for (const CkAction& x : plan)
+      if (x.kind == CkAction::ADVANCE) adv += x.steps;
+    *advances = adv;
+    *actions = int64_t(plan.size());
+  });
+}

@@ -10,6 +10,7 @@
 #include <memory>
 #include <vector>
 
+#include "checkpoint.hpp"
 #include "common.hpp"
 #include "grid.hpp"
 #include "kernels.hpp"
@@ -1108,6 +1109,153 @@ int festung_adj_rhs(festung_problem* h, const double* mu, double* out) {
   });
 }
 
+// Discrete adjoint with binomial checkpointing: only state snapshots are
+// kept (`snaps` of them, the initial state included), placed by the binomial
+// schedule; each step is recomputed with its stage states right before it
+// is reversed.  Velocity sensitivities fused into the transposed stage (or
+// split over three threads per element); one stream.  The stage kernels and
+// their order are those of the uniform schedule, so the results are the same
+// bits as festung_adjoint's with fused (split) sensitivities.
+static void adjoint_binomial(festung_problem* h, int nsteps, double tau, const double* cObs, int snaps, double* J,
+                             double* gradC0, double* gradU1, double* gradU2, double* gradVn, double* timing_ms) {
+  if (snaps < 1) fail(FESTUNG_ERR_ARG, "binomial checkpointing needs at least one snapshot");
+  if (h->flags & FESTUNG_FLAG_ADJ_DEFER)
+    fail(FESTUNG_ERR_ARG, "binomial checkpointing forms the sensitivities fused or split, not deferred");
+  std::vector<CkAction> plan;
+  try {
+    plan = binomial_schedule(nsteps, snaps);
+  } catch (const std::exception& e) {
+    fail(FESTUNG_ERR_INTERNAL, "%s", e.what());
+  }
+  const int K = h->K, N = h->N, Re = h->tabs.Re, S = int(h->plan.size());
+  const size_t nk = size_t(N) * K * sizeof(double);
+  const int what = (h->flags & FESTUNG_FLAG_ADJ_SPLIT) ? ADJ_SPLIT : ADJ_FUSED;
+  hipStream_t A = h->stream;
+  const int c = std::min(snaps, nsteps);
+  std::vector<std::unique_ptr<DBuf>> snap(c);
+  for (auto& b : snap) {
+    b.reset(new DBuf);
+    b->alloc(nk);
+  }
+  // rot: advance ping-pong; rotOut: the schedule's scratch state; tmp: an
+  // advance's stage outputs; stg: the stage states of the step being
+  // reversed; fin: the final state; junk: other recomputed step ends
+  DBuf rot[2], rotOut, fin, junk, dObs, part, L0, Gu1, Gu2, Gvn, mass;
+  std::vector<DBuf> tmp(std::max(S - 1, 1)), stg(std::max(S - 1, 1));
+  for (DBuf* b : {&rot[0], &rot[1], &rotOut, &fin, &junk, &L0, &Gu1, &Gu2, &mass}) b->alloc(nk);
+  for (auto& b : tmp) b.alloc(nk);
+  for (auto& b : stg) b.alloc(nk);
+  Gvn.alloc(size_t(3) * Re * K * sizeof(double));
+  part.alloc(size_t(h->nred) * sizeof(double));
+  h->upload_elem(dObs, cObs, N);
+  constexpr int R = 4;  // ring of adjoint stage states
+  DBuf ringb[R];
+  for (auto& b : ringb) b.alloc(nk);
+  auto ring = [&](long j) -> double* { return ringb[j % R].d(); };
+  std::vector<hipEvent_t> ev(3, nullptr);
+  struct EvGuard {
+    std::vector<hipEvent_t>& v;
+    ~EvGuard() {
+      for (auto e : v)
+        if (e) (void)hipEventDestroy(e);
+    }
+  } evg{ev};
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+
+  // one forward step in -> out; stage outputs into outs[0 .. S-2]
+  auto fwd_step = [&](const double* in, double* out, std::vector<DBuf>& outs) {
+    for (int st = 0; st < S; ++st) {
+      StageArgs a{};
+      a.m = h->mesh();
+      a.c = st == 0 ? in : outs[st - 1].d();
+      a.c0 = in;
+      a.out = st + 1 < S ? outs[st].d() : out;
+      a.u1 = h->u1.d();
+      a.u2 = h->u2.d();
+      a.vnw = h->vnw.d();
+      a.tab = h->tab.d();
+      a.off = h->off;
+      a.tau = tau;
+      a.tauK = h->hasTauK ? h->tauK.d() : nullptr;
+      a.rk = h->plan[st].rk;
+      a.sink = h->sink();
+      HIPCHK(launch_stage(h->p, OUT_RK_DG, a, A));
+    }
+  };
+  auto buf = [&](int32_t id) -> double* { return id == kRot ? rotOut.d() : snap[id]->d(); };
+  HIPCHK(hipEventRecord(ev[0], A));
+  HIPCHK(hipMemcpyAsync(snap[0]->p, h->buf(h->cur), nk, hipMemcpyDeviceToDevice, A));
+  long i = 0;  // adjoint stage counter
+  for (const CkAction& x : plan) {
+    if (x.kind == CkAction::ADVANCE) {
+      const double* cur = buf(x.src);
+      for (int l = 0; l < x.steps; ++l) {
+        double* next = l + 1 == x.steps ? buf(x.dst) : rot[l % 2].d();
+        fwd_step(cur, next, tmp);
+        cur = next;
+      }
+      continue;
+    }
+    // REVERSE step j: recompute it with its stage states, then its
+    // transposed stages in reverse order
+    const int j = x.from;
+    const double* in = buf(x.src);
+    fwd_step(in, j + 1 == nsteps ? fin.d() : junk.d(), stg);
+    if (j + 1 == nsteps) {
+      // loss and terminal adjoint (into ring[R-1], read by the first stage)
+      HIPCHK(hipEventRecord(ev[1], A));
+      HIPCHK(launch_loss(h->p, K, h->B.d(), fin.d(), dObs.d(), h->tab.d(), h->off, part.d(), ring(R - 1), A));
+      HIPCHK(launch_fill(size_t(N) * K, 0.0, Gu1.d(), A));
+      HIPCHK(launch_fill(size_t(N) * K, 0.0, Gu2.d(), A));
+      HIPCHK(launch_fill(size_t(3) * Re * K, 0.0, Gvn.d(), A));
+    }
+    for (int st = S - 1; st >= 0; --st, ++i) {
+      AdjArgs a{};
+      a.m = h->mesh();
+      a.mu = ring(i + R - 1);
+      a.u1 = h->u1.d();
+      a.u2 = h->u2.d();
+      a.vn = h->vn.d();
+      a.vnw = h->vnw.d();
+      a.tab = h->tab.d();
+      a.off = h->off;
+      a.tau = tau;
+      a.tauK = h->hasTauK ? h->tauK.d() : nullptr;
+      a.omega = h->plan[st].rk.a0;
+      a.L0 = L0.d();
+      a.mode = S == 1 ? 3 : (st == S - 1 ? 0 : (st > 0 ? 1 : 2));
+      a.out = ring(i);
+      a.c = st == 0 ? in : stg[st - 1].d();
+      a.Gu1 = Gu1.d();
+      a.Gu2 = Gu2.d();
+      a.Gvn = Gvn.d();
+      a.what = what;
+      HIPCHK(launch_adj_stage(h->p, a, A));
+    }
+  }
+  HIPCHK(launch_mass(h->p, K, h->B.d(), ring(i + R - 1), h->tab.d(), h->off, mass.d(), A));
+  HIPCHK(hipEventRecord(ev[2], A));
+  HIPCHK(hipMemcpyAsync(h->X.p, fin.p, nk, hipMemcpyDeviceToDevice, A));
+  h->cur = 0;
+  sync(h);
+  std::vector<double> hp(h->nred);
+  HIPCHK(hipMemcpy(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double jsum = 0;
+  for (double v : hp) jsum += v;
+  if (J) *J = jsum;
+  if (gradC0) h->download_elem(mass, mass.d(), N, gradC0);
+  if (gradU1) h->download_elem(Gu1, Gu1.d(), N, gradU1);
+  if (gradU2) h->download_elem(Gu2, Gu2.d(), N, gradU2);
+  if (gradVn) h->download_elem(Gvn, Gvn.d(), 3 * Re, gradVn);
+  if (timing_ms) {
+    float f = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&f, ev[0], ev[1]));
+    HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+    timing_ms[0] = f;
+    timing_ms[1] = b;
+  }
+}
+
 int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double* cObs, int32_t ckpt, double* J,
                     double* gradC0, double* gradU1, double* gradU2, double* gradVn, double* timing_ms) {
   return guard([&] {
@@ -1122,6 +1270,10 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
     if (h->hasRhs) fail(FESTUNG_ERR_UNSUPPORTED, "the adjoint assumes f = 0 and c_D = 0");
     const int K = h->K, N = h->N, Re = h->tabs.Re, S = int(h->plan.size());
     const size_t nk = size_t(N) * K * sizeof(double);
+    if (h->flags & FESTUNG_FLAG_ADJ_BINOMIAL) {
+      adjoint_binomial(h, nsteps, tau, cObs, ckpt, J, gradC0, gradU1, gradU2, gradVn, timing_ms);
+      return;
+    }
     const int C = (ckpt <= 0 || ckpt > nsteps) ? nsteps : ckpt;
     const int nseg = (nsteps + C - 1) / C;
     // Velocity sensitivities: fused into the transposed stage kernel (one

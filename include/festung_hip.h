@@ -129,7 +129,10 @@ enum {
                                     sensitivities of each checkpoint segment in one pass after it
                                     (default on small meshes) */
   FESTUNG_FLAG_LIM_HIERARCH = 64, /* limiter type 'hierarch_vert' (core/applySlopeLimiterTaylorHierarchicalVertex.m) */
-  FESTUNG_FLAG_LIM_STRICT = 128   /* limiter type 'strict' (core/applySlopeLimiterTaylorStrict.m); default 'linear' */
+  FESTUNG_FLAG_LIM_STRICT = 128,  /* limiter type 'strict' (core/applySlopeLimiterTaylorStrict.m); default 'linear' */
+  FESTUNG_FLAG_ADJ_BINOMIAL = 256 /* festung_adjoint: binomial checkpointing -- ckptSteps is the number
+                                    of state snapshots; every step is recomputed with its stage states
+                                    just before it is reversed (fused or split sensitivities) */
 };
 
 /* Upload the grid (its connectivity and marks as they are now) and build the
@@ -248,11 +251,20 @@ void festung_batch_destroy(festung_batch* b);
  * these fields, so a caller's parameter gradient is their contraction with
  * the fields' derivatives.  Stage states are kept in HBM: checkpoints every
  * ckptSteps steps (<= 0: all steps stored, no recomputation), each segment
- * recomputed once in the reverse sweep.  timing_ms (NULL or 2 values):
- * forward and reverse device time.  No counterpart in the reference. */
+ * recomputed once in the reverse sweep; under FESTUNG_FLAG_ADJ_BINOMIAL,
+ * ckptSteps state snapshots placed by the binomial schedule
+ * (festung_binomial_schedule).  timing_ms (NULL or 2 values): forward (up
+ * to the loss) and reverse device time.  No counterpart in the reference. */
 int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double* cObs, int32_t ckptSteps,
                     double* J, double* gradC0, double* gradU1, double* gradU2, double* gradVn,
                     double* timing_ms);
+/* The binomial checkpointing schedule festung_adjoint runs under
+ * FESTUNG_FLAG_ADJ_BINOMIAL: reversing nsteps steps with `snapshots` state
+ * snapshots (the initial state included) takes *advances forward steps
+ * besides the nsteps recomputations with stored stage states, the least
+ * possible (Griewank's binomial bound), in *actions advance / reverse
+ * actions.  Host only.  No counterpart in the reference. */
+int festung_binomial_schedule(int32_t nsteps, int32_t snapshots, int64_t* advances, int64_t* actions);
 /* out = M^-1 A^T mu for the current velocity (K x N): the transposed
  * right-hand-side operator, for dot-product tests of the adjoint. */
 int festung_adj_rhs(festung_problem* h, const double* mu, double* out);

@@ -172,3 +172,28 @@ def test_adjoint_variants_agree(gpu, nsteps, ckpt):
     assert np.allclose(d[2], f[2], rtol=0, atol=1e-13 * np.abs(f[2]).max())
     for key in ('gradU1', 'gradU2', 'gradVn'):
         assert np.allclose(d[3][key], f[3][key], rtol=0, atol=1e-13 * np.abs(f[3][key]).max())
+
+
+@pytest.mark.parametrize('kern', ['fused', 'split'])
+def test_adjoint_binomial_checkpointing(gpu, kern):
+    """Binomial checkpointing (state snapshots only, every step recomputed
+    with its stage states before it is reversed) runs the same kernels in the
+    same order as the uniform checkpoints: the same bits for every number of
+    snapshots, from one (walk from the start) to one per step."""
+    nsteps = 13
+    modes = AD.stream_modes()[:4]
+    a = 0.05 * np.random.default_rng(52).standard_normal(len(modes))
+    tau = 0.1 / (16 * 0.8 * 5)
+
+    def run(binomial, ckpt):
+        m = AD.StreamModel(1 / 16, 2, modes=modes, c0=TC.gaussian_humps(51), adj_kernel=kern,
+                           adj_binomial=binomial)
+        cObs = m.run(m.c0, 1.1 * a, nsteps, tau)
+        return m.gradient(m.c0, a, nsteps, tau, cObs, ckpt=ckpt)
+    ref = run(False, 5)
+    for snaps in (1, 2, 3, 6, nsteps):
+        r = run(True, snaps)
+        assert r[0] == ref[0]
+        assert np.array_equal(r[1], ref[1]) and np.array_equal(r[2], ref[2])
+        for key in ('gradU1', 'gradU2', 'gradVn'):
+            assert np.array_equal(r[3][key], ref[3][key])

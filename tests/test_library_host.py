@@ -128,6 +128,13 @@ def test_taylor_tables(p):
     Fi = ref_table(p, 'taylorFinv').reshape(N, N, order='F')
     assert np.allclose(F @ Fi, np.eye(N), atol=1e-10)
     assert np.allclose(F[0], np.eye(N)[0] * np.sqrt(2))
+    # a Taylor row of degree d involves only DG functions of degree >= d
+    # (exact zeros, which the kernels skip)
+    deg = [d for d in range(p + 1) for _ in range(d + 1)]
+    for r in range(N):
+        for j in range(N):
+            if deg[r] > deg[j]:
+                assert F[r, j] == 0 and Fi[r, j] == 0
     c = 1 / 3
     for j in range(N):
         assert abs(F[1, j] - OB.gradPhi(j + 1, 1, c, c)) < 1e-12
@@ -235,3 +242,47 @@ def test_hotstart_roundtrip(tmp_path):
     assert fn.endswith('state.mat')
     d = O.readHotstart(fn)
     assert np.array_equal(d['cDisc'], c) and d['t'] == 0.25 and d['nStep'] == 7
+
+
+def _least_advances(n, c):
+    """Least number of forward steps (besides one recomputation per reversed
+    step) that reverses n steps keeping at most c states, the start state
+    included, by exhaustive recursion over the position of the next
+    snapshot: reversing [0, n) either walks from the start to every step
+    (no free snapshot) or advances k steps, stores that state, reverses
+    [k, n) with one snapshot fewer and then [0, k) with the same number."""
+    from functools import lru_cache
+
+    @lru_cache(None)
+    def best(n, s):
+        if n <= 1:
+            return 0
+        if s == 0:
+            return n * (n - 1) // 2
+        return min(k + best(n - k, s - 1) + best(k, s) for k in range(1, n))
+    return best(n, min(c, n) - 1)
+
+
+def test_binomial_schedule_is_optimal():
+    """The schedule reaches the least forward-step count on every n <= 60,
+    c <= 7, and Griewank's closed form r n - C(c + r, c + 1) (r the least
+    integer with C(c + r, c) >= n) beyond; every step is reversed once, so
+    the action count is at least n."""
+    from math import comb
+    from festung_amd.core import binomial_schedule
+    for c in range(1, 8):
+        for n in range(1, 61):
+            adv, act = binomial_schedule(n, c)
+            assert adv == _least_advances(n, c), (n, c)
+            assert act >= n
+    for n, c in ((400, 21), (400, 6), (1000, 10), (777, 2)):
+        r = 0
+        while comb(c + r, c) < n:
+            r += 1
+        assert binomial_schedule(n, c)[0] == r * n - comb(c + r, c + 1)
+    assert binomial_schedule(10, 1)[0] == 45          # no snapshot: walk from the start
+    assert binomial_schedule(10, 10)[0] == 9          # every state stored
+    with pytest.raises(_lib.FestungError):
+        binomial_schedule(0, 3)
+    with pytest.raises(_lib.FestungError):
+        binomial_schedule(5, 0)
