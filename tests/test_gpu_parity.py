@@ -413,3 +413,39 @@ def test_unstructured_mesh_steps(gpu, p, ordRK, lim):
         c = oracle_step(o, c, n)
         got = product_step(d, n)
         assert rel_l2(got, c) < TOL * n, n
+
+
+def test_mid_size_step_parity(gpu):
+    """One SSP-RK3 step with the limiter at K = 32 768 (hmax 2^-7: hundreds
+    of workgroups over the eight XCD chunks, Hilbert element order, patch
+    bounds across workgroups) against the oracle."""
+    tc = TC.rotating_humps()
+    o = oracle_problem(tc, 2, 1 / 128, 3, True, numSteps=20, tEnd=0.01)
+    d = product_problem(tc, 2, 1 / 128, 3, True, numSteps=20, tEnd=0.01)
+    assert d.problem.K == 32768
+    c = o.cDisc.copy()
+    d.problem.set_state(c)
+    got = product_step(d, 1)
+    ref = oracle_step(o, c, 1)
+    assert rel_l2(got, ref) < TOL
+
+
+def test_full_size_free_stream(gpu):
+    """At the p = 2 benchmark size (K = 524 288, hmax 2^-9; SSP-RK3, limiter,
+    captured graph) c = 1 with inflow c_D = 1 under the rotation
+    u = (0.5 - y, x - 0.5) -- linear and divergence-free, so represented
+    exactly -- stays 1 to rounding over three steps: every element's flux
+    balance closes, whatever workgroup, XCD chunk or patch it lies in."""
+    import festung_amd
+    zero = lambda t, x1, x2: np.zeros_like(x1)
+    tc = TC.Testcase(isAnalytical=False, c0Cont=lambda x1, x2: np.ones_like(x1),
+                     u1Cont=lambda t, x1, x2: 0.5 - x2, u2Cont=lambda t, x1, x2: x1 - 0.5,
+                     fCont=zero, cDCont=lambda t, x1, x2: np.ones_like(x1), gNCont=zero,
+                     fZero=True, cDZero=False, isStationary=True)
+    h = 2.0 ** -9
+    d = festung_amd.main('advection', dict(testcase='custom', p=2, hmax=h, ordRK=3, isSlopeLim=True, numSteps=3,
+                                           tEnd=3 * 0.4 * h / (0.8 * 5), useGraph=True, **tc))
+    c = d.cDisc
+    assert c.shape == (524288, 6)
+    assert np.abs(c[:, 1:]).max() < 1e-12
+    assert np.abs(c[:, 0] / np.sqrt(0.5) - 1).max() < 1e-12
