@@ -430,8 +430,10 @@ def test_mid_size_step_parity(gpu):
     assert rel_l2(got, ref) < TOL
 
 
-def test_full_size_free_stream(gpu):
-    """At the p = 2 benchmark size (K = 524 288, hmax 2^-9; SSP-RK3, limiter,
+@pytest.mark.parametrize('p,ordRK,hexp,K', [(2, 3, 9, 524288), (4, 4, 8, 131072)])
+def test_full_size_free_stream(gpu, p, ordRK, hexp, K):
+    """At the benchmark sizes (p = 2: K = 524 288, hmax 2^-9, SSP-RK3; p = 4:
+    K = 131 072, hmax 2^-8, RK4 with the MFMA element term; limiter on,
     captured graph) c = 1 with inflow c_D = 1 under the rotation
     u = (0.5 - y, x - 0.5) -- linear and divergence-free, so represented
     exactly -- stays 1 to rounding over three steps: every element's flux
@@ -442,10 +444,10 @@ def test_full_size_free_stream(gpu):
                      u1Cont=lambda t, x1, x2: 0.5 - x2, u2Cont=lambda t, x1, x2: x1 - 0.5,
                      fCont=zero, cDCont=lambda t, x1, x2: np.ones_like(x1), gNCont=zero,
                      fZero=True, cDZero=False, isStationary=True)
-    h = 2.0 ** -9
-    d = festung_amd.main('advection', dict(testcase='custom', p=2, hmax=h, ordRK=3, isSlopeLim=True, numSteps=3,
-                                           tEnd=3 * 0.4 * h / (0.8 * 5), useGraph=True, **tc))
+    h = 2.0 ** -hexp
+    d = festung_amd.main('advection', dict(testcase='custom', p=p, hmax=h, ordRK=ordRK, isSlopeLim=True, numSteps=3,
+                                           tEnd=3 * 0.4 * h / (0.8 * (2 * p + 1)), useGraph=True, **tc))
     c = d.cDisc
-    assert c.shape == (524288, 6)
+    assert c.shape == (K, (p + 1) * (p + 2) // 2)
     assert np.abs(c[:, 1:]).max() < 1e-12
     assert np.abs(c[:, 0] / np.sqrt(0.5) - 1).max() < 1e-12
