@@ -124,6 +124,17 @@ def test_limiter_alone(gpu, p):
     assert rel_l2(got, ref) < TOL
     assert np.max(np.abs(gmn - mn)) < 1e-14 and np.max(np.abs(gmx - mx)) < 1e-14
     assert np.all((alpha >= 0) & (alpha <= 1))
+    # the limiter's decisions: alpha of every element as the oracle forms it
+    # (core/computeVertexBasedLimiter.m:78-106 on the Taylor data); a flip at a
+    # tol threshold would show here as a difference of order one
+    from oracle import limiter as OL
+    from oracle import assembly as OAs
+    t = OL.projectDataDisc2DataTaylor(c, o.globM, o.globMDiscTaylor)
+    valV0T = OAs.computeFuncDiscAtPoints(t[:, :3], o.basesOnQuad['phiTaylorV0T'][:, :, :3])
+    cD = OAs.computeFuncContV0T(o.g, lambda x1, x2: o.cDCont(0.0, x1, x2))
+    aref, _ = OL.computeVertexBasedLimiter(o.g, t[:, 0], valV0T, o.g.markV0TbdrD, cD)
+    assert 0 < np.sum(aref < 1) < o.K
+    assert np.max(np.abs(alpha - aref)) < 1e-12
 
 
 @pytest.mark.parametrize('type_', ['hierarch_vert', 'strict'])
