@@ -59,16 +59,15 @@ struct MfmaElem {
 
 // Offsets (in doubles) of the reference tables inside one device buffer.
 struct TabOff {
-  int wE, phiE, gradE;      // quadRule2D(max(2p,1)): weights, phi (RE x N), gradPhi (RE x N x 2)
+  int wE, phiE;             // quadRule2D(max(2p,1)): weights, phi (RE x N)
   int wF, phiF;             // quadRule2D(2p+1)
   int wEdge, phi1D, theta;  // quadRule1D(2p+1): weights, phi1D (Re x N x 3), thetaPhi1D (Re x N x 3 x 3)
   int hatM, hatMinv;        // N x N
-  int F, Finv;              // Taylor change of basis, N x N
   // the same data laid out for the element kernels' loop order, so that the
   // values one loop iteration needs are contiguous (wide uniform loads):
   int elemQ;                // per point q of quadRule2D(max(2p,1)): w_q, phi_j(q), d1phi_j(q), d2phi_j(q)
   int edgeQ;                // per edge n, point r: w_r, phi_j(gamma_n(s_r))
-  int hatMinvR, FR, FinvR;  // N x N row-major
+  int hatMinvR, FR, FinvR;  // N x N row-major (FR, FinvR: the Taylor change of basis and its inverse)
   // operands of the MFMA element term (MfmaElem), per lane l (r = l % 16,
   // k = l / 16): A[qb][s] = phi_(4s+k)(q = 16qb + r); D1/D2[qb][t] =
   // w_q d1phi_r(q) / w_q d2phi_r(q) at q = 16qb + 4t + k (zero past N); -1 if unused

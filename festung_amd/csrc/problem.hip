@@ -415,7 +415,6 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     TabOff& o = h->off;
     o.wE = put(t.ruleE.w);
     o.phiE = put(t.phiE);
-    o.gradE = put(t.gradE);
     o.wF = put(t.ruleF.w);
     o.phiF = put(t.phiF);
     o.wEdge = put(t.ruleEdge.w);
@@ -423,8 +422,6 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     o.theta = put(t.thetaPhi1D);
     o.hatM = put(t.hatM);
     o.hatMinv = put(t.hatMinv);
-    o.F = put(t.F);
-    o.Finv = put(t.Finv);
     {
       // loop-ordered copies (see TabOff)
       const int RE = t.RE, Re = t.Re;
