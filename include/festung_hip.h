@@ -40,9 +40,11 @@ enum {
 typedef struct festung_grid festung_grid;
 typedef struct festung_problem festung_problem;
 
-/* Message of the last failing call on this thread ("" after a success). */
+/* Message of the last failing call on this thread ("" after a success);
+ * the text a gateway hands to Matlab's error() (the reference reports
+ * failures by error()/assert).  No counterpart in the reference. */
 const char* festung_last_error(void);
-/* Library version string. */
+/* Library version string.  No counterpart in the reference. */
 const char* festung_version(void);
 
 /* ---------------------------------------------------------------- grid --
@@ -73,6 +75,7 @@ int festung_grid_set_boundary(festung_grid* g, const int32_t* idE, const uint8_t
  * solves one problem per run. */
 int festung_grid_concat(const festung_grid* const* grids, int32_t n, festung_grid** out);
 
+/* g.numT, g.numV, g.numE (core/generateGridData.m:130-198). */
 int festung_grid_sizes(const festung_grid* g, int32_t* numT, int32_t* numV, int32_t* numE);
 
 /* Copy a named grid field out.  int32 fields: "V0T" (numT x 3), "V0E",
@@ -92,6 +95,7 @@ int festung_grid_get_f64(const festung_grid* g, const char* field, double* out);
 int festung_grid_map_ref_to_phy(const festung_grid* g, int32_t nq, const double* q1, const double* q2,
                                 double* X1, double* X2);
 
+/* Free a grid (the reference's struct g is freed by Matlab). */
 void festung_grid_destroy(festung_grid* g);
 
 /* ------------------------------------------------------ reference tables --
@@ -190,7 +194,9 @@ int festung_set_vertex_dirichlet_dev(festung_problem* h, const double* vertexD);
  * If set_state != 0 the result also becomes the current state. */
 int festung_project(festung_problem* h, int ord, const double* fq, double* cDisc_out, int set_state);
 
-/* Current solution cDisc (K x N). */
+/* Current solution problemData.cDisc (K x N; advection/initializeProblem.m:53,
+ * advection/postprocessStep.m:59); the _dev variants take [N][K] device
+ * memory on the problem's device. */
 int festung_set_state(festung_problem* h, const double* cDisc);
 int festung_get_state(festung_problem* h, double* cDisc);
 int festung_set_state_dev(festung_problem* h, const double* d_cDisc);
@@ -313,7 +319,8 @@ enum {
 int festung_profile(festung_problem* h, int enable);
 int festung_profile_read(festung_problem* h, double* ms, int64_t* count);
 
-/* Synchronise the problem's stream. */
+/* Synchronise the problem's stream.  No counterpart in the reference
+ * (whose calls are synchronous). */
 int festung_sync(festung_problem* h);
 
 #ifdef __cplusplus
