@@ -147,8 +147,11 @@ struct festung_problem {
   bool hasRhs = false, hasVertexD = false, hasVelocity = false;
   int cur = 0;           // buffer holding the current state (0 = X)
   int nextStage = -1;    // next sub-step of a step begun with festung_begin_step
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graphExec = nullptr;
+  // captured step sequences: graph[0] one step, graph[1] kGraphSteps steps
+  // (fewer graph launches: each boundary between launches costs a few us)
+  static constexpr int kGraphSteps = 8;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graphExec[2] = {nullptr, nullptr};
   double graphTau = std::numeric_limits<double>::quiet_NaN();
   int bs = 256;      // elements per workgroup of the element kernels
   int nblocks = 0;   // workgroups of the element kernels = min/max partials
@@ -204,10 +207,12 @@ struct festung_problem {
   }
   void set_device() const { HIPCHK(hipSetDevice(device)); }
   void drop_graph() {
-    if (graphExec) (void)hipGraphExecDestroy(graphExec);
-    if (graph) (void)hipGraphDestroy(graph);
-    graphExec = nullptr;
-    graph = nullptr;
+    for (int i = 0; i < 2; ++i) {
+      if (graphExec[i]) (void)hipGraphExecDestroy(graphExec[i]);
+      if (graph[i]) (void)hipGraphDestroy(graph[i]);
+      graphExec[i] = nullptr;
+      graph[i] = nullptr;
+    }
     graphTau = std::numeric_limits<double>::quiet_NaN();
   }
   hipStream_t aux[2] = {nullptr, nullptr};  // side streams of the adjoint sweep
@@ -830,22 +835,35 @@ int festung_step(festung_problem* h, int nsteps, double tau) {
     if (nsteps < 0) fail(FESTUNG_ERR_ARG, "negative step count");
     if (nsteps == 0) return;
     if ((h->flags & FESTUNG_FLAG_GRAPH) && !h->prof) {
-      if (!h->graphExec || !(h->graphTau == tau)) {
-        h->drop_graph();
-        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        try {
-          h->enqueue_step(tau);
-        } catch (...) {
-          hipGraph_t g = nullptr;
-          (void)hipStreamEndCapture(h->stream, &g);
-          if (g) (void)hipGraphDestroy(g);
-          throw;
+      if (!(h->graphTau == tau)) h->drop_graph();
+      // nsteps = q G + r: the G-step sequence q times, the one-step one r times
+      constexpr int G = festung_problem::kGraphSteps;
+      const int q = nsteps / G, r = nsteps % G;
+      auto captured = [&](int i) -> hipGraphExec_t {
+        if (!h->graphExec[i]) {
+          HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+          try {
+            for (int j = 0; j < (i ? G : 1); ++j) h->enqueue_step(tau);
+          } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(h->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+          }
+          HIPCHK(hipStreamEndCapture(h->stream, &h->graph[i]));
+          HIPCHK(hipGraphInstantiate(&h->graphExec[i], h->graph[i], nullptr, nullptr, 0));
+          h->graphTau = tau;
         }
-        HIPCHK(hipStreamEndCapture(h->stream, &h->graph));
-        HIPCHK(hipGraphInstantiate(&h->graphExec, h->graph, nullptr, nullptr, 0));
-        h->graphTau = tau;
+        return h->graphExec[i];
+      };
+      if (q > 0) {
+        hipGraphExec_t e = captured(1);
+        for (int s = 0; s < q; ++s) HIPCHK(hipGraphLaunch(e, h->stream));
       }
-      for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(h->graphExec, h->stream));
+      if (r > 0) {
+        hipGraphExec_t e = captured(0);
+        for (int s = 0; s < r; ++s) HIPCHK(hipGraphLaunch(e, h->stream));
+      }
     } else {
       for (int s = 0; s < nsteps; ++s) h->enqueue_step(tau);
     }
