@@ -118,8 +118,8 @@ def run_batched(nprob, nsteps, seed0=4000):
 
 def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6):
     """Gradient of J = ||c(T) - c_obs||^2_M w.r.t. c0 and nmodes stream-function
-    coefficients: cost against the plain forward (graph) run, and central
-    differences (step h) along ndir random directions."""
+    coefficients: cost against the faster plain forward run (eager or captured
+    graphs), and central differences (step h) along ndir random directions."""
     import numpy as np
     import torch
     from festung_amd import adjoint as AD
@@ -131,20 +131,28 @@ def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6):
     dim = int(np.ceil(1 / hmax))
     tau = 0.1 / (dim * 0.8 * (2 * p + 1))
     cObs = m.run(m.c0, a_true, nsteps, tau)
-    # plain forward (captured graph), timed with events on the problem's stream
-    stream = torch.cuda.ExternalStream(m.prob.stream)
-    m.set_velocity(a)
-    m.prob.set_state(m.c0)
-    m.prob.step(nsteps, tau)
-    m.prob.sync()
-    m.prob.set_state(m.c0)
-    e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    m.prob.step(nsteps, tau)
-    e1.record(stream)
-    m.prob.sync()
-    fwd_ms = e0.elapsed_time(e1)
-    out = {'forward_ms': fwd_ms}
+
+    # plain forward, eager and as captured graphs, timed with events on the
+    # problem's stream; the faster one is the cost unit
+    def time_forward(model):
+        stream = torch.cuda.ExternalStream(model.prob.stream)
+        model.set_velocity(a)
+        model.prob.set_state(model.c0)
+        model.prob.step(nsteps, tau)
+        model.prob.sync()
+        model.prob.set_state(model.c0)
+        e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        model.prob.step(nsteps, tau)
+        e1.record(stream)
+        model.prob.sync()
+        return e0.elapsed_time(e1)
+    eager_ms = time_forward(m)
+    mg = AD.StreamModel(hmax, p, modes=modes, c0=TC.gaussian_humps(51), graph=True)
+    graph_ms = time_forward(mg)
+    del mg
+    fwd_ms = min(eager_ms, graph_ms)
+    out = {'forward_ms': fwd_ms, 'forward_eager_ms': eager_ms, 'forward_graph_ms': graph_ms}
     for c in (ckpt, 0):
         t0 = time.perf_counter()
         J, g0, ga, r = m.gradient(m.c0, a, nsteps, tau, cObs, ckpt=c)

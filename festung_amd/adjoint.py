@@ -46,11 +46,11 @@ class StreamModel:
     """Device problem (limiter off) whose velocity is u0 + sum a_m u_m."""
 
     def __init__(self, hmax, p, ordRK=3, modes=None, c0=None, adj_serial=False, adj_kernel=None,
-                 adj_binomial=False):
+                 adj_binomial=False, graph=False):
         self.g = Grid.domainSquare(hmax)
         self.p = p
         self.prob = Problem(self.g, p, ordRK, adj_serial=adj_serial, adj_kernel=adj_kernel,
-                            adj_binomial=adj_binomial)
+                            adj_binomial=adj_binomial, graph=graph)
         self.modes = modes if modes is not None else stream_modes()
         pr = self.prob
         Q1, Q2, _ = quadRule2D(pr.ordE)
