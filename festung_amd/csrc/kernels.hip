@@ -1190,7 +1190,8 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
     }
   }
   const int fl = a.m.flags[k];
-#pragma unroll(UNR ? 3 : 1)
+  constexpr int kEdgeUnroll = UNR ? 3 : 1;
+#pragma unroll kEdgeUnroll
   for (int n = 0; n < (ADJ || SENS_V ? 3 : 0); ++n) {
     // the transposed stage alone reads the edge-flux factors |E| w_r vn
     // (the same products as area * v * w_r below)
