@@ -22,6 +22,7 @@ FLAG_ADJ_DEFER = 32
 FLAG_LIM_HIERARCH = 64
 FLAG_LIM_STRICT = 128
 FLAG_ADJ_BINOMIAL = 256
+OP_G, OP_G1, OP_G2, OP_R, OP_M, OP_MINV = range(6)
 LIMITER_TYPES = {'linear': 0, 'hierarch_vert': FLAG_LIM_HIERARCH, 'strict': FLAG_LIM_STRICT}
 
 
@@ -62,10 +63,10 @@ SIGNATURES = {
     'festung_problem_stream': (C.c_int, [vp, C.POINTER(vp)]),
     'festung_set_velocity_disc': (C.c_int, [vp, dp, dp, dp]),
     'festung_set_velocity_points': (C.c_int, [vp, dp, dp, dp, dp]),
-    'festung_set_source_points': (C.c_int, [vp, dp, dp]),
+    'festung_set_source_points': (C.c_int, [vp, dp, dp, dp]),
     'festung_set_vertex_dirichlet': (C.c_int, [vp, dp]),
     'festung_set_velocity_points_dev': (C.c_int, [vp, vp, vp, vp, vp]),
-    'festung_set_source_points_dev': (C.c_int, [vp, vp, vp]),
+    'festung_set_source_points_dev': (C.c_int, [vp, vp, vp, vp]),
     'festung_set_vertex_dirichlet_dev': (C.c_int, [vp, vp]),
     'festung_project': (C.c_int, [vp, C.c_int, dp, dp, C.c_int]),
     'festung_set_state': (C.c_int, [vp, dp]),
@@ -91,6 +92,13 @@ SIGNATURES = {
     'festung_adjoint': (C.c_int, [vp, C.c_int32, C.c_double, dp, C.c_int32, dp, dp, dp, dp, dp, dp]),
     'festung_binomial_schedule': (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'festung_adj_rhs': (C.c_int, [vp, dp, dp]),
+    'festung_adjoint_dev': (C.c_int, [vp, C.c_int32, C.c_double, vp, C.c_int32, dp, vp, vp, vp, vp, dp]),
+    'festung_apply_operator': (C.c_int, [vp, C.c_int, dp, dp]),
+    'festung_apply_operator_dev': (C.c_int, [vp, C.c_int, vp, vp]),
+    'festung_assemble_vectors': (C.c_int, [vp, dp, dp, dp, dp, dp, dp]),
+    'festung_vertex_based_limiter': (C.c_int, [vp, dp, dp, C.POINTER(C.c_uint8), dp, dp, dp, dp]),
+    'festung_apply_slope_limiter_disc': (C.c_int, [vp, dp, C.POINTER(C.c_uint8), dp, dp, dp, dp]),
+    'festung_runge_kutta_explicit': (C.c_int, [C.c_int, C.c_double, C.c_double, dp, dp, ip]),
     'festung_profile': (C.c_int, [vp, C.c_int]),
     'festung_profile_read': (C.c_int, [vp, dp, C.POINTER(C.c_int64)]),
     'festung_sync': (C.c_int, [vp]),

@@ -93,6 +93,9 @@ def configureProblem(pd):
         raise ValueError('Invalid testcase "%s".' % pd.testcase)
     pd.setdefault('isVelocityStationary', pd.isStationary)
     pd.generateGridData = Grid.domainSquare
+    # Neumann edges, advection/configureProblem.m:149 (the reference's
+    # default: none); a callable g -> K x 3 bool marks when given
+    pd.setdefault('generateMarkE0TbdrN', None)
     return pd
 
 
@@ -100,6 +103,9 @@ def preprocessProblem(pd):
     """advection/preprocessProblem.m:48-112: grid, sizes, device problem
     (reference blocks and mass matrices are built inside the library)."""
     pd.g = pd.generateGridData(pd.hmax)
+    if pd.get('generateMarkE0TbdrN') is not None:
+        # advection/preprocessProblem.m:72-76: markE0TbdrD = ~(int | N), markV0TbdrD
+        pd.g.set_boundary(pd.g.idE, np.asarray(pd.generateMarkE0TbdrN(pd.g), dtype=bool))
     pd.K = pd.g.numT
     pd.N = (pd.p + 1) * (pd.p + 2) // 2
     pd.tau = pd.tEnd / pd.numSteps
@@ -118,6 +124,9 @@ def preprocessProblem(pd):
         X1[:, n, :], X2[:, n, :] = pd.g.mapRef2Phy(A1, A2)
     pd.XEdge = (X1, X2)
     pd.dirichletVertices = np.unique(pd.g.V0T[pd.g.markV0TbdrD])
+    # K_N is formed only when the grid has Neumann edges (markE0TbdrN, set
+    # through Grid.set_boundary); it is identically zero otherwise
+    pd.hasNeumann = bool(np.any(pd.g.markE0TbdrN))
     pd.velocitySet = False
     pd.deviceEval = _device_points(pd) if pd.get('deviceEval', True) else None
     return pd
@@ -150,6 +159,8 @@ def _device_points(pd):
         need.append(pd.fCont)
     if not pd.cDZero:
         need.append(pd.cDCont)
+    if pd.hasNeumann:
+        need.append(pd.gNCont)
     try:
         for f in need:
             _as_device(f(0.0, *probe), probe[0])
@@ -228,8 +239,8 @@ def iterateSubSteps(pd, nStep):
 
 def preprocessSubStep(pd, nStep, nSubStep):
     """advection/preprocessSubStep.m:56-99: velocity (projected with order 2p
-    and as u.nu on the edge points), globL - globKD.  Time-independent or
-    identically-zero data are uploaded once / skipped."""
+    and as u.nu on the edge points), globL - globKD - globKN.
+    Time-independent or identically-zero data are uploaded once / skipped."""
     prob = pd.problem
     ts = pd.t[nSubStep - 1]
     D = pd.get('deviceEval')
@@ -243,16 +254,18 @@ def preprocessSubStep(pd, nStep, nSubStep):
             prob.set_velocity_points(pd.u1Cont(ts, *pd.XE), pd.u2Cont(ts, *pd.XE),
                                      pd.u1Cont(ts, X1, X2), pd.u2Cont(ts, X1, X2))
         pd.velocitySet = True
-    if not (pd.fZero and pd.cDZero):
+    if not (pd.fZero and pd.cDZero and not pd.hasNeumann):
         if D is not None:
             E, Q = D['XEdge'], D['XE']
             fq = None if pd.fZero else _as_device(pd.fCont(ts, *Q), Q[0])
             cDe = None if pd.cDZero else _as_device(pd.cDCont(ts, *E), E[0])
-            prob.set_source_points_dev(fq, cDe)
+            gNe = _as_device(pd.gNCont(ts, *E), E[0]) if pd.hasNeumann else None
+            prob.set_source_points_dev(fq, cDe, gNe)
         else:
             fq = None if pd.fZero else pd.fCont(ts, *pd.XE)
             cDe = None if pd.cDZero else pd.cDCont(ts, *pd.XEdge)
-            prob.set_source_points(fq, cDe)
+            gNe = pd.gNCont(ts, *pd.XEdge) if pd.hasNeumann else None
+            prob.set_source_points(fq, cDe, gNe)
     return pd
 
 

@@ -203,28 +203,42 @@ class Problem:
         c = fortran(u1e, (K, 3, self.Re)); d = fortran(u2e, (K, 3, self.Re))
         check(lib().festung_set_velocity_points(self._h, ptr(a), ptr(b), ptr(c), ptr(d)))
 
-    def set_source_points(self, fq=None, cDe=None):
+    def set_source_points(self, fq=None, cDe=None, gNe=None):
+        """globL - globKD - globKN from f at the element points (K x RE), c_D
+        and g_N at the edge points (K x 3 x Re); None = zero data."""
         f = None if fq is None else fortran(fq, (self.K, self.RE))
         c = None if cDe is None else fortran(cDe, (self.K, 3, self.Re))
-        check(lib().festung_set_source_points(self._h, ptr(f), ptr(c)))
+        n = None if gNe is None else fortran(gNe, (self.K, 3, self.Re))
+        check(lib().festung_set_source_points(self._h, ptr(f), ptr(c), ptr(n)))
 
     def _dev_args(self, tensors, shapes):
         """Device pointers of float64 CUDA tensors in [cols][K] layout, made
         ready for the problem's stream (it waits for the producing torch
-        stream; the caching allocator is told the problem's stream uses them)."""
+        stream; the caching allocator is told the problem's stream uses them).
+        Every tensor is checked before anything is enqueued."""
         import torch
+        for t, shape in zip(tensors, shapes):
+            if t is None:
+                continue
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+                    and t.numel() == int(np.prod(shape))):
+                raise ValueError('expected a contiguous float64 CUDA tensor with %d values' % int(np.prod(shape)))
         ours = torch.cuda.ExternalStream(self.stream)
         ours.wait_stream(torch.cuda.current_stream())
         out = []
-        for t, shape in zip(tensors, shapes):
+        for t in tensors:
             if t is None:
                 out.append(None)
                 continue
-            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == int(np.prod(shape))):
-                raise ValueError('expected a contiguous float64 CUDA tensor with %d values' % int(np.prod(shape)))
             t.record_stream(ours)
             out.append(C.c_void_p(t.data_ptr()))
         return out
+
+    def _after_problem_stream(self):
+        """Make torch's current stream wait for the work enqueued so far on the
+        problem's stream (e.g. a copy into a tensor torch reads next)."""
+        import torch
+        torch.cuda.current_stream().wait_stream(torch.cuda.ExternalStream(self.stream))
 
     def set_velocity_points_dev(self, u1q, u2q, u1e, u2e):
         """set_velocity_points from CUDA tensors laid out [RE][K] and [Re][3][K]
@@ -233,8 +247,8 @@ class Problem:
         a = self._dev_args([u1q, u2q, u1e, u2e], [(self.RE, K)] * 2 + [(self.Re, 3, K)] * 2)
         check(lib().festung_set_velocity_points_dev(self._h, *a))
 
-    def set_source_points_dev(self, fq=None, cDe=None):
-        a = self._dev_args([fq, cDe], [(self.RE, self.K), (self.Re, 3, self.K)])
+    def set_source_points_dev(self, fq=None, cDe=None, gNe=None):
+        a = self._dev_args([fq, cDe, gNe], [(self.RE, self.K), (self.Re, 3, self.K), (self.Re, 3, self.K)])
         check(lib().festung_set_source_points_dev(self._h, *a))
 
     def set_vertex_dirichlet_dev(self, vertexD):
@@ -286,13 +300,17 @@ class Problem:
         return out
 
     def set_state_dev(self, t):
-        """State from a device tensor laid out [N][K] (contiguous)."""
-        assert t.is_contiguous() and t.numel() == self.K * self.N
-        check(lib().festung_set_state_dev(self._h, C.c_void_p(t.data_ptr())))
+        """State from a contiguous float64 CUDA tensor laid out [N][K]
+        (reference element order); ordered after torch's pending work on it."""
+        a = self._dev_args([t], [(self.N, self.K)])
+        check(lib().festung_set_state_dev(self._h, *a))
 
     def get_state_dev(self, t):
-        assert t.is_contiguous() and t.numel() == self.K * self.N
-        check(lib().festung_get_state_dev(self._h, C.c_void_p(t.data_ptr())))
+        """Copy the state into a contiguous float64 CUDA tensor laid out
+        [N][K]; torch's current stream waits for the copy."""
+        a = self._dev_args([t], [(self.N, self.K)])
+        check(lib().festung_get_state_dev(self._h, *a))
+        self._after_problem_stream()
 
     def begin_step(self):
         check(lib().festung_begin_step(self._h))
@@ -352,6 +370,68 @@ class Problem:
         check(lib().festung_adj_rhs(self._h, ptr(m), ptr(out)))
         return out
 
+    def adjoint_dev(self, nsteps, tau, cObs, ckpt=0, gradC0=None, gradU1=None, gradU2=None, gradVn=None):
+        """adjoint() with cObs and the gradients as float64 CUDA tensors
+        ([N][K], [N][K], [N][K], [Re][3][K]; any output may be None).
+        Returns dict(J, forward_ms, backward_ms); torch's current stream
+        waits for the outputs."""
+        K, N, Re = self.K, self.N, self.Re
+        a = self._dev_args([cObs, gradC0, gradU1, gradU2, gradVn], [(N, K)] * 4 + [(Re, 3, K)])
+        J = np.zeros(1)
+        tm = np.zeros(2)
+        check(lib().festung_adjoint_dev(self._h, int(nsteps), float(tau), a[0], int(ckpt), ptr(J), *a[1:], ptr(tm)))
+        self._after_problem_stream()
+        return dict(J=float(J[0]), forward_ms=tm[0], backward_ms=tm[1])
+
+    OPERATORS = {'G': L.OP_G, 'G1': L.OP_G1, 'G2': L.OP_G2, 'R': L.OP_R, 'M': L.OP_M, 'Minv': L.OP_MINV}
+
+    def apply_operator(self, op, x):
+        """y = A x (K x N) for A = 'G1', 'G2', 'G' (globG{1}, globG{2}, their
+        sum), 'R' (globR), 'M' (globM) or 'Minv' (globM \\ x), at the current
+        velocity."""
+        xx = fortran(x, (self.K, self.N))
+        y = np.empty((self.K, self.N), order='F')
+        check(lib().festung_apply_operator(self._h, self.OPERATORS[op], ptr(xx), ptr(y)))
+        return y
+
+    def apply_operator_dev(self, op, x, y):
+        """apply_operator on float64 CUDA tensors laid out [N][K]."""
+        a = self._dev_args([x, y], [(self.N, self.K)] * 2)
+        check(lib().festung_apply_operator_dev(self._h, self.OPERATORS[op], *a))
+        self._after_problem_stream()
+
+    def assemble_vectors(self, fq=None, cDe=None, gNe=None):
+        """(globL, globKD, globKN) as K x N arrays from point values (None
+        where the input is None)."""
+        K, N = self.K, self.N
+        f = None if fq is None else fortran(fq, (K, self.RE))
+        c = None if cDe is None else fortran(cDe, (K, 3, self.Re))
+        n = None if gNe is None else fortran(gNe, (K, 3, self.Re))
+        outs = [None if v is None else np.empty((K, N), order='F') for v in (f, c, n)]
+        check(lib().festung_assemble_vectors(self._h, ptr(f), ptr(c), ptr(n), *[ptr(o) for o in outs]))
+        return tuple(outs)
+
+    def vertex_based_limiter(self, valCentroid, valV0T, markV0TbdrD=None, dataV0T=None):
+        """core/computeVertexBasedLimiter.m: (alphaE (K), [minV0T, maxV0T] (K x 3))."""
+        K = self.K
+        vc = fortran(valCentroid, (K,))
+        vv = fortran(valV0T, (K, 3))
+        mk = None if markV0TbdrD is None else fortran(markV0TbdrD, (K, 3), np.uint8)
+        dv = None if dataV0T is None else fortran(dataV0T, (K, 3))
+        a = np.empty(K); mn = np.empty((K, 3), order='F'); mx = np.empty((K, 3), order='F')
+        check(lib().festung_vertex_based_limiter(self._h, ptr(vc), ptr(vv), ptr(mk), ptr(dv), ptr(a), ptr(mn), ptr(mx)))
+        return a, [mn, mx]
+
+    def apply_slope_limiter_disc(self, dataDisc, markV0TbdrD=None, dataV0T=None):
+        """core/applySlopeLimiterDisc.m: (dataDisc limited, [minV0T, maxV0T])."""
+        K, N = self.K, self.N
+        c = fortran(dataDisc, (K, N))
+        mk = None if markV0TbdrD is None else fortran(markV0TbdrD, (K, 3), np.uint8)
+        dv = None if dataV0T is None else fortran(dataV0T, (K, 3))
+        out = np.empty((K, N), order='F'); mn = np.empty((K, 3), order='F'); mx = np.empty((K, 3), order='F')
+        check(lib().festung_apply_slope_limiter_disc(self._h, ptr(c), ptr(mk), ptr(dv), ptr(out), ptr(mn), ptr(mx)))
+        return out, [mn, mx]
+
     KERNEL_KINDS = ('stage', 'vertex', 'limit_dot', 'limit', 'copy')
 
     def profile(self, enable):
@@ -394,6 +474,14 @@ class Batch:
     def __del__(self):
         if L._lib is not None and getattr(self, '_h', None) is not None:
             self.close()
+
+
+def rungeKuttaExplicit(ord_, tau, t0):
+    """core/rungeKuttaExplicit.m:64-78 through festung_runge_kutta_explicit:
+    (stage times, omega)."""
+    t = np.zeros(4); w = np.zeros(4); n = C.c_int32()
+    check(lib().festung_runge_kutta_explicit(int(ord_), float(tau), float(t0), ptr(t), ptr(w), C.byref(n)))
+    return t[:n.value].copy(), w[:n.value].copy()
 
 
 def binomial_schedule(nsteps, snapshots):

@@ -1018,11 +1018,15 @@ __global__ __launch_bounds__(BLOCK) void edge_flux_weights_kernel(int K, int Re,
   }
 }
 
-// globL - globKD: detJ hatM fDisc minus the Dirichlet inflow term,
-// advection/preprocessSubStep.m:88-98, core/assembleVecEdgePhiIntFuncContVal.m:173-184
+// globL - globKD - globKN: detJ hatM fDisc minus the Dirichlet inflow term
+// and the Neumann term, advection/preprocessSubStep.m:88-98,
+// core/assembleVecEdgePhiIntFuncContVal.m:173-184,
+// core/assembleVecEdgePhiIntFuncCont.m:126-135 (with the upwind g_N of
+// advection/preprocessSubStep.m:92: g_N where g_N <= 0, else 0)
 template <int P>
 __global__ __launch_bounds__(BLOCK) void source_kernel(Mesh m, const double* __restrict__ fq,
-                                                       const double* __restrict__ cDe, const double* __restrict__ vn,
+                                                       const double* __restrict__ cDe, const double* __restrict__ gNe,
+                                                       const double* __restrict__ vn,
                                                        const double* __restrict__ tab, TabOff off,
                                                        double* __restrict__ rhs) {
   constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
@@ -1070,6 +1074,26 @@ __global__ __launch_bounds__(BLOCK) void source_kernel(Mesh m, const double* __r
 #pragma unroll
         for (int i = 0; i < N; ++i) r[i] -= fx * tab[off.phi1D + r2 + i * Re + n * Re * N];
       }
+    }
+  }
+  if (gNe) {
+    // K_N(k, i) = sum_n [markE0TbdrN] |E_kn| sum_r gN_up(x_knr) w_r phi_i(gamma_n(s_r)),
+    // the r-sum first (funcQ0E * (W' .* phi1D), then the edge factor)
+    const int fl = m.flags[k];
+    for (int n = 0; n < 3; ++n) {
+      if (!flag_neu(fl, n)) continue;
+      const double area = m.areaE[n * K + k];
+      double e[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) e[i] = 0.0;
+      for (int r2 = 0; r2 < Re; ++r2) {
+        const double gn = gNe[(r2 * 3 + n) * K + k];
+        const double gu = (gn <= 0.0) ? gn : 0.0 * gn;
+#pragma unroll
+        for (int i = 0; i < N; ++i) e[i] += gu * (tab[off.wEdge + r2] * tab[off.phi1D + r2 + i * Re + n * Re * N]);
+      }
+#pragma unroll
+      for (int i = 0; i < N; ++i) r[i] -= area * e[i];
     }
   }
 #pragma unroll
@@ -1586,6 +1610,145 @@ __global__ __launch_bounds__(BLOCK) void mass_kernel(int K, const double* __rest
   }
 }
 
+// ------------------------------------------------ operator-level applies --
+// y = A x for one of the operators the reference assembles as sparse
+// matrices, each applied on its own (the stage kernel fuses them):
+//   OP_G1, OP_G2, OP_G  the element operators globG{1}, globG{2} and their sum
+//                       (core/assembleMatElemDphiPhiFuncDisc.m:181-202) at the
+//                       velocity u1Disc, u2Disc, in quadrature form on the 2p rule:
+//                       (G1 x)_i = sum_q w_q x(q) u1(q) (B22 d1phi_i - B21 d2phi_i)(q),
+//                       (G2 x)_i = sum_q w_q x(q) u2(q) (B11 d2phi_i - B12 d1phi_i)(q)
+//   OP_R                the upwind edge operator globR
+//                       (core/assembleMatEdgePhiPhiValUpwind.m:261-299): the own
+//                       trace where u.nu > 0 on non-Neumann edges, the neighbour's
+//                       where u.nu < 0 across interior non-Neumann edges
+//   OP_M, OP_MINV       globM = kron(diag(detJ), hatM) (core/assembleMatElemPhiPhi.m:98)
+//                       and its block inverse (globM \ x, advection/solveSubStep.m:66)
+template <int P>
+__global__ __launch_bounds__(BLOCK) void apply_kernel(Mesh m, int op, const double* __restrict__ x,
+                                                      const double* __restrict__ u1, const double* __restrict__ u2,
+                                                      const double* __restrict__ vnw, const double* __restrict__ tab,
+                                                      TabOff off, double* __restrict__ y) {
+  constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
+  const int K = m.K;
+  const int k = gtid();
+  if (k >= K) return;
+  double c[N], acc[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    c[i] = x[i * K + k];
+    acc[i] = 0.0;
+  }
+  Geo<P> g;
+  g.load(m.B, k, K);
+  const double* __restrict__ EQ = tab + off.elemQ;
+  if (op == OP_G || op == OP_G1 || op == OP_G2) {
+    const bool w1 = op != OP_G2, w2 = op != OP_G1;
+    for (int q = 0; q < RE; ++q) {
+      double cq = 0.0, u1q = 0.0, u2q = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const double ph = EQ[QIdx<P>::phi(q, j)];
+        cq += c[j] * ph;
+        if (w1) u1q += u1[j * K + k] * ph;
+        if (w2) u2q += u2[j * K + k] * ph;
+      }
+      const double wc = EQ[QIdx<P>::w(q)] * cq;
+      double f1, f2;
+      if (op == OP_G) {
+        f1 = wc * (g.B22 * u1q - g.B12 * u2q);
+        f2 = wc * (g.B11 * u2q - g.B21 * u1q);
+      } else if (op == OP_G1) {
+        f1 = wc * (g.B22 * u1q);
+        f2 = wc * (-g.B21 * u1q);
+      } else {
+        f1 = wc * (-g.B12 * u2q);
+        f2 = wc * (g.B11 * u2q);
+      }
+#pragma unroll
+      for (int i = 1; i < N; ++i) acc[i] += EQ[QIdx<P>::d1(q, i)] * f1 + EQ[QIdx<P>::d2(q, i)] * f2;
+    }
+  } else if (op == OP_R) {
+    const int fl = m.flags[k];
+    for (int n = 0; n < 3; ++n) {
+      const int nb = m.nbr[n * K + k];
+      const int np = flag_np(fl, n);
+      const bool outOn = flag_out(fl, n);
+      for (int r = 0; r < Re; ++r) {
+        const double v = vnw[(r * 3 + n) * K + k];
+        double tr = 0.0;
+        if (v > 0.0 && outOn) {
+#pragma unroll
+          for (int j = 0; j < N; ++j) tr += c[j] * tab[off.edgeQ + QIdx<P>::ephi(n, r, j)];
+        } else if (v < 0.0 && nb >= 0) {
+          const double* th = tab + off.theta + r + n * Re * N + np * Re * N * 3;
+#pragma unroll
+          for (int j = 0; j < N; ++j) tr += x[j * K + nb] * th[j * Re];
+        }
+        const double fx = v * tr;
+#pragma unroll
+        for (int i = 0; i < N; ++i) acc[i] += tab[off.edgeQ + QIdx<P>::ephi(n, r, i)] * fx;
+      }
+    }
+  } else if (op == OP_M) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s += tab[off.hatM + i + j * N] * c[j];
+      acc[i] = g.det * s;
+    }
+  } else {  // OP_MINV
+    const double idet = 1.0 / g.det;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s += tab[off.hatMinvR + i * N + j] * c[j];
+      acc[i] = s * idet;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) y[i * K + k] = acc[i];
+}
+
+// core/computeVertexBasedLimiter.m:78-106 on caller data: per element the
+// centroid value cen (K), the vertex values valV0T (3 x K) and the Dirichlet
+// values dataV0T (3 x K, NaN where markV0TbdrD is false; null: none); the
+// raw patch min / max of cen per vertex and their global min / max come from
+// vertex_minmax_kernel.  Out: alpha (K), minV0T, maxV0T (3 x K) with the
+// reference's global shift (core/computeMinMaxV0TElementPatch.m:75-78,107-110).
+__global__ __launch_bounds__(BLOCK) void vertex_limiter_kernel(Mesh m, const double* __restrict__ cen,
+                                                               const double* __restrict__ valV0T,
+                                                               const double* __restrict__ dataV0T,
+                                                               const double* __restrict__ vmin,
+                                                               const double* __restrict__ vmax,
+                                                               const double* __restrict__ gmm, double* __restrict__ alpha,
+                                                               double* __restrict__ minV0T, double* __restrict__ maxV0T) {
+  const int K = m.K;
+  const int k = gtid();
+  if (k >= K) return;
+  const double sNeg = fabs(gmm[1]) + 1.0;
+  const double sPos = fabs(gmm[0]) + 1.0;
+  const double xc = cen[k];
+  double al = 1.0;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const int v = m.V0T[n * K + k];
+    double lo = (vmin[v] - sNeg) + sNeg;
+    double hi = (vmax[v] + sPos) - sPos;
+    if (dataV0T) {
+      const double dv = dataV0T[n * K + k];
+      lo = fmin(lo, dv);
+      hi = fmax(hi, dv);
+    }
+    minV0T[n * K + k] = lo;
+    maxV0T[n * K + k] = hi;
+    al = fmin(al, vertex_alpha(valV0T[n * K + k], xc, lo, hi));
+  }
+  alpha[k] = al;
+}
+
 __global__ __launch_bounds__(BLOCK) void fill_kernel(size_t n, double v, double* __restrict__ x) {
   const size_t i = size_t(blockIdx.x) * BLOCK + threadIdx.x;
   if (i < n) x[i] = v;
@@ -1731,12 +1894,12 @@ hipError_t launch_vnormal(int K, int Re, const double* nuE0T, const double* u1e,
   return hipGetLastError();
 }
 
-hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* cDe, const double* vn,
-                         const double* tab, const TabOff& off, double* rhs, hipStream_t s) {
+hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* cDe, const double* gNe,
+                         const double* vn, const double* tab, const TabOff& off, double* rhs, hipStream_t s) {
   const int BS = m.bs;
   if (!valid_elem_block(BS)) return hipErrorInvalidValue;
   dim3 gr = grid_for(m.K, BS);
-  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((source_kernel<PP>), gr, dim3(BS), 0, s, m, fq, cDe, vn, tab, off, rhs));
+  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((source_kernel<PP>), gr, dim3(BS), 0, s, m, fq, cDe, gNe, vn, tab, off, rhs));
   return hipGetLastError();
 }
 
@@ -1807,6 +1970,22 @@ hipError_t launch_mass(int p, int K, const double* B, const double* x, const dou
                        double* out, hipStream_t s) {
   dim3 gr = grid_for(K);
   FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((mass_kernel<PP>), gr, dim3(BLOCK), 0, s, K, B, x, tab, off, out));
+  return hipGetLastError();
+}
+
+hipError_t launch_apply(int p, int op, const Mesh& m, const double* x, const double* u1, const double* u2,
+                        const double* vnw, const double* tab, const TabOff& off, double* y, hipStream_t s) {
+  if (op < OP_G || op > OP_MINV) return hipErrorInvalidValue;
+  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((apply_kernel<PP>), grid_for(m.K), dim3(BLOCK), 0, s, m, op, x, u1, u2, vnw, tab,
+                                       off, y));
+  return hipGetLastError();
+}
+
+hipError_t launch_vertex_limiter(const Mesh& m, const double* cen, const double* valV0T, const double* dataV0T,
+                                 const double* vmin, const double* vmax, const double* gmm, double* alpha,
+                                 double* minV0T, double* maxV0T, hipStream_t s) {
+  FESTUNG_LAUNCH(vertex_limiter_kernel, grid_for(m.K), dim3(BLOCK), 0, s, m, cen, valV0T, dataV0T, vmin, vmax, gmm,
+                 alpha, minV0T, maxV0T);
   return hipGetLastError();
 }
 

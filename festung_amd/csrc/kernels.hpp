@@ -78,11 +78,12 @@ struct TabOff {
 // Per-element flag word: bits 2n..2n+1 = neighbour's local edge across
 // local edge n; bit 6+n = outflow (diagonal) term active on edge n; bit 9+n =
 // Dirichlet inflow on edge n; bit 12+n = local vertex n lies on a Dirichlet
-// edge (markV0TbdrD).
+// edge (markV0TbdrD); bit 15+n = edge n is a Neumann edge (markE0TbdrN).
 __host__ __device__ inline int flag_np(int f, int n) { return (f >> (2 * n)) & 3; }
 __host__ __device__ inline bool flag_out(int f, int n) { return (f >> (6 + n)) & 1; }
 __host__ __device__ inline bool flag_dir(int f, int n) { return (f >> (9 + n)) & 1; }
 __host__ __device__ inline bool flag_vdir(int f, int n) { return (f >> (12 + n)) & 1; }
+__host__ __device__ inline bool flag_neu(int f, int n) { return (f >> (15 + n)) & 1; }
 
 // Coefficients of one explicit RK stage:
 //   out = a0*c0 + a1*c + b*tau*cdot + g*acc
@@ -113,6 +114,8 @@ struct Mesh {
   int bs;                // elements per workgroup of the element kernels (64, 128 or 256)
 };
 
+// Operators of festung_apply_operator (values of FESTUNG_OP_* in the C-ABI)
+enum OpKind { OP_G = 0, OP_G1 = 1, OP_G2 = 2, OP_R = 3, OP_M = 4, OP_MINV = 5 };
 enum StageOut { OUT_RK_DG = 0, OUT_RK_TAYLOR = 1, OUT_DOT_TAYLOR = 2, OUT_DOT_DG = 3 };
 enum AdjWhat { ADJ_STAGE = 0, ADJ_FUSED = 1, ADJ_SENS = 2, ADJ_SPLIT = 3, ADJ_SENS_U = 4, ADJ_SENS_V = 5 };
 // festung_adjoint's sweep with the sensitivities deferred to one pass per
@@ -273,6 +276,13 @@ hipError_t launch_loss(int p, int K, const double* B, const double* c, const dou
 // out = det_k hatM x_k (the block mass matrix)
 hipError_t launch_mass(int p, int K, const double* B, const double* x, const double* tab, const TabOff& off,
                        double* out, hipStream_t s);
+// y = A x for one reference operator applied alone (OpKind)
+hipError_t launch_apply(int p, int op, const Mesh& m, const double* x, const double* u1, const double* u2,
+                        const double* vnw, const double* tab, const TabOff& off, double* y, hipStream_t s);
+// alpha and the shifted patch bounds of computeVertexBasedLimiter on caller data
+hipError_t launch_vertex_limiter(const Mesh& m, const double* cen, const double* valV0T, const double* dataV0T,
+                                 const double* vmin, const double* vmax, const double* gmm, double* alpha,
+                                 double* minV0T, double* maxV0T, hipStream_t s);
 hipError_t launch_fill(size_t n, double v, double* x, hipStream_t s);
 hipError_t launch_vertex(const VertexArgs& a, hipStream_t s);
 hipError_t launch_limit(int p, int mode, int type, const LimitArgs& a, hipStream_t s);
@@ -285,8 +295,10 @@ hipError_t launch_project(int p, bool ordF, int K, const double* fq, const doubl
                           double* out, hipStream_t s);
 hipError_t launch_vnormal(int K, int Re, const double* nuE0T, const double* u1e, const double* u2e, double* vn,
                           hipStream_t s);
-hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* cDe, const double* vn,
-                         const double* tab, const TabOff& off, double* rhs, hipStream_t s);
+// rhs = globL - globKD - globKN from f at the element points (fq), c_D and
+// g_N at the edge points (cDe, gNe); any of the three may be null (zero)
+hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* cDe, const double* gNe,
+                         const double* vn, const double* tab, const TabOff& off, double* rhs, hipStream_t s);
 hipError_t launch_l2(int p, bool ordF, int K, const double* B, const double* c, const double* cq, const double* tab,
                      const TabOff& off, double* partial, hipStream_t s);
 hipError_t launch_permute(int K, int cols, const double* src, const int* idx, double* dst, hipStream_t s);

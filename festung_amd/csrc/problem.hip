@@ -264,6 +264,26 @@ struct festung_problem {
     HIPCHK(hipMemcpy(v.data(), devp, v.size() * sizeof(double), hipMemcpyDeviceToHost));
     from_dev(v.data(), cols, dst);
   }
+  // Caller arrays of the entry point running now: host memory (reference
+  // order, copied) or, while ioDev is set (the _dev entry points), device
+  // memory in the same layout, permuted on the device.
+  bool ioDev = false;
+  void load_in(DBuf& b, const double* src, int cols) {
+    if (!ioDev) {
+      upload_elem(b, src, cols);
+      return;
+    }
+    b.alloc(size_t(K) * cols * sizeof(double));
+    HIPCHK(launch_permute(K, cols, src, dEperm.i(), b.d(), stream));
+  }
+  void store_out(const DBuf& b, int cols, double* dst) {
+    if (!ioDev) {
+      download_elem(b, b.d(), cols, dst);
+      return;
+    }
+    HIPCHK(launch_permute(K, cols, b.d(), dEinv.i(), dst, stream));
+    HIPCHK(hipStreamSynchronize(stream));  // b may be freed when the entry point returns
+  }
 
   // raw vertex-patch min / max of the first nF Taylor coefficients just
   // written to `fields` (the centroid values first), and their global min /
@@ -276,6 +296,20 @@ struct festung_problem {
     drop_graph();
     ++version;
     hasVertexD = v;
+  }
+  // The same for whether a stage source (globL - globKD - globKN) is added.
+  void set_has_rhs(bool v) {
+    if (v == hasRhs) return;
+    HIPCHK(hipStreamSynchronize(stream));
+    drop_graph();
+    ++version;
+    hasRhs = v;
+  }
+  // Captured sequences also bake in the scalar time step: drop them (after
+  // the launches in flight, which may still run them) when it changes.
+  void drop_graph_synced() {
+    HIPCHK(hipStreamSynchronize(stream));
+    drop_graph();
   }
 
   void refresh_vnw() {
@@ -532,6 +566,7 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
         if (notN && markElemEdges) fl[d] |= 1 << (6 + n);
         if (g.markE0TbdrD[kn]) fl[d] |= 1 << (9 + n);
         if (g.markV0TbdrD[kn]) fl[d] |= 1 << (12 + n);
+        if (g.markE0TbdrN[kn]) fl[d] |= 1 << (15 + n);
       }
     h->upload(h->nbr, nb.data(), nb.size());
     h->upload(h->flg, fl.data(), fl.size());
@@ -656,17 +691,21 @@ int festung_set_velocity_points_dev(festung_problem* h, const double* u1q, const
   });
 }
 
-int festung_set_source_points_dev(festung_problem* h, const double* fq, const double* cDe) {
+int festung_set_source_points_dev(festung_problem* h, const double* fq, const double* cDe, const double* gNe) {
   return guard([&] {
     check_handle(h);
     h->set_device();
-    if (!fq && !cDe) {
-      h->hasRhs = false;
+    if (!fq && !cDe && !gNe) {
+      h->set_has_rhs(false);
       return;
     }
     if (cDe) h->require_velocity();
     const int K = h->K, RE = h->tabs.RE, Re3 = 3 * h->tabs.Re;
-    if (!h->rhs.p) h->rhs.alloc(size_t(h->N) * K * sizeof(double));
+    if (!h->rhs.p) {
+      sync(h);
+      h->rhs.alloc(size_t(h->N) * K * sizeof(double));
+    }
+    h->set_has_rhs(true);
     if (fq) {
       if (!h->devQ2.p) h->devQ2.alloc(size_t(RE) * K * sizeof(double));
       HIPCHK(launch_permute(K, RE, fq, h->dEperm.i(), h->devQ2.d(), h->stream));
@@ -675,9 +714,12 @@ int festung_set_source_points_dev(festung_problem* h, const double* fq, const do
       if (!h->devE1.p) h->devE1.alloc(size_t(Re3) * K * sizeof(double));
       HIPCHK(launch_permute(K, Re3, cDe, h->dEperm.i(), h->devE1.d(), h->stream));
     }
-    HIPCHK(launch_source(h->p, h->mesh(), fq ? h->devQ2.d() : nullptr, cDe ? h->devE1.d() : nullptr, h->vn.d(),
-                         h->tab.d(), h->off, h->rhs.d(), h->stream));
-    h->hasRhs = true;
+    if (gNe) {
+      if (!h->devE2.p) h->devE2.alloc(size_t(Re3) * K * sizeof(double));
+      HIPCHK(launch_permute(K, Re3, gNe, h->dEperm.i(), h->devE2.d(), h->stream));
+    }
+    HIPCHK(launch_source(h->p, h->mesh(), fq ? h->devQ2.d() : nullptr, cDe ? h->devE1.d() : nullptr,
+                         gNe ? h->devE2.d() : nullptr, h->vn.d(), h->tab.d(), h->off, h->rhs.d(), h->stream));
   });
 }
 
@@ -692,23 +734,28 @@ int festung_set_vertex_dirichlet_dev(festung_problem* h, const double* vertexD) 
   });
 }
 
-int festung_set_source_points(festung_problem* h, const double* fq, const double* cDe) {
+int festung_set_source_points(festung_problem* h, const double* fq, const double* cDe, const double* gNe) {
   return guard([&] {
     check_handle(h);
     h->set_device();
-    if (!fq && !cDe) {
-      h->hasRhs = false;
+    if (!fq && !cDe && !gNe) {
+      h->set_has_rhs(false);
       return;
     }
     if (cDe) h->require_velocity();
     const int K = h->K;
-    if (!h->rhs.p) h->rhs.alloc(size_t(h->N) * K * sizeof(double));
-    DBuf f, c;
+    if (!h->rhs.p) {
+      sync(h);
+      h->rhs.alloc(size_t(h->N) * K * sizeof(double));
+    }
+    h->set_has_rhs(true);
+    DBuf f, c, gn;
     if (fq) h->upload_elem(f, fq, h->tabs.RE);
     if (cDe) h->upload_elem(c, cDe, 3 * h->tabs.Re);
-    HIPCHK(launch_source(h->p, h->mesh(), f.d(), c.d(), h->vn.d(), h->tab.d(), h->off, h->rhs.d(), h->stream));
+    if (gNe) h->upload_elem(gn, gNe, 3 * h->tabs.Re);
+    HIPCHK(launch_source(h->p, h->mesh(), f.d(), c.d(), gn.d(), h->vn.d(), h->tab.d(), h->off, h->rhs.d(),
+                         h->stream));
     sync(h);
-    h->hasRhs = true;
   });
 }
 
@@ -835,7 +882,7 @@ int festung_step(festung_problem* h, int nsteps, double tau) {
     if (nsteps < 0) fail(FESTUNG_ERR_ARG, "negative step count");
     if (nsteps == 0) return;
     if ((h->flags & FESTUNG_FLAG_GRAPH) && !h->prof) {
-      if (!(h->graphTau == tau)) h->drop_graph();
+      if (!(h->graphTau == tau) && (h->graphExec[0] || h->graphExec[1])) h->drop_graph_synced();
       // nsteps = q G + r: the G-step sequence q times, the one-step one r times
       constexpr int G = festung_problem::kGraphSteps;
       const int q = nsteps / G, r = nsteps % G;
@@ -1013,7 +1060,7 @@ int festung_set_tau_field(festung_problem* h, const double* tauK) {
   return guard([&] {
     check_handle(h);
     h->set_device();
-    h->drop_graph();
+    h->drop_graph_synced();
     ++h->version;
     if (!tauK) {
       h->hasTauK = false;
@@ -1162,7 +1209,7 @@ static void adjoint_binomial(festung_problem* h, int nsteps, double tau, const d
   for (auto& b : stg) b.alloc(nk);
   Gvn.alloc(size_t(3) * Re * K * sizeof(double));
   part.alloc(size_t(h->nred) * sizeof(double));
-  h->upload_elem(dObs, cObs, N);
+  h->load_in(dObs, cObs, N);
   constexpr int R = 4;  // ring of adjoint stage states
   DBuf ringb[R];
   for (auto& b : ringb) b.alloc(nk);
@@ -1258,10 +1305,10 @@ static void adjoint_binomial(festung_problem* h, int nsteps, double tau, const d
   double jsum = 0;
   for (double v : hp) jsum += v;
   if (J) *J = jsum;
-  if (gradC0) h->download_elem(mass, mass.d(), N, gradC0);
-  if (gradU1) h->download_elem(Gu1, Gu1.d(), N, gradU1);
-  if (gradU2) h->download_elem(Gu2, Gu2.d(), N, gradU2);
-  if (gradVn) h->download_elem(Gvn, Gvn.d(), 3 * Re, gradVn);
+  if (gradC0) h->store_out(mass, N, gradC0);
+  if (gradU1) h->store_out(Gu1, N, gradU1);
+  if (gradU2) h->store_out(Gu2, N, gradU2);
+  if (gradVn) h->store_out(Gvn, 3 * Re, gradVn);
   if (timing_ms) {
     float f = 0, b = 0;
     HIPCHK(hipEventElapsedTime(&f, ev[0], ev[1]));
@@ -1342,7 +1389,7 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
     }
     auto ring = [&](long j) -> double* { return ringb[size_t(j % R)]->d(); };
     DBuf dObs, part, L0, Gu1, Gu2, Gvn, tmp;
-    h->upload_elem(dObs, cObs, N);
+    h->load_in(dObs, cObs, N);
     part.alloc(size_t(h->nred) * sizeof(double));
     for (DBuf* b : {&L0, &Gu1, &Gu2, &tmp}) b->alloc(nk);
     Gvn.alloc(size_t(3) * Re * K * sizeof(double));
@@ -1545,10 +1592,10 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
     double j = 0;
     for (double v : hp) j += v;
     if (J) *J = j;
-    if (gradC0) h->download_elem(tmp, tmp.d(), N, gradC0);
-    if (gradU1) h->download_elem(Gu1, Gu1.d(), N, gradU1);
-    if (gradU2) h->download_elem(Gu2, Gu2.d(), N, gradU2);
-    if (gradVn) h->download_elem(Gvn, Gvn.d(), 3 * Re, gradVn);
+    if (gradC0) h->store_out(tmp, N, gradC0);
+    if (gradU1) h->store_out(Gu1, N, gradU1);
+    if (gradU2) h->store_out(Gu2, N, gradU2);
+    if (gradVn) h->store_out(Gvn, 3 * Re, gradVn);
     if (timing_ms) {
       float f = 0, b = 0;
       HIPCHK(hipEventElapsedTime(&f, ev[0], ev[1]));
@@ -1556,6 +1603,180 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
       timing_ms[0] = f;
       timing_ms[1] = b;
     }
+  });
+}
+
+int festung_adjoint_dev(festung_problem* h, int32_t nsteps, double tau, const double* cObs, int32_t ckpt,
+                        double* J, double* gradC0, double* gradU1, double* gradU2, double* gradVn,
+                        double* timing_ms) {
+  return guard([&] {
+    check_handle(h);
+    struct Reset {
+      festung_problem* h;
+      ~Reset() { h->ioDev = false; }
+    } reset{h};
+    h->ioDev = true;
+    check_call(festung_adjoint(h, nsteps, tau, cObs, ckpt, J, gradC0, gradU1, gradU2, gradVn, timing_ms));
+  });
+}
+
+// One operator of the reference's assembled set applied alone (OpKind).
+static void apply_operator(festung_problem* h, int op, const double* x, double* y) {
+  if (op < FESTUNG_OP_G || op > FESTUNG_OP_MINV) fail(FESTUNG_ERR_ARG, "unknown operator %d", op);
+  if (!x || !y) fail(FESTUNG_ERR_ARG, "null pointer argument");
+  h->set_device();
+  if (op <= FESTUNG_OP_R) h->require_velocity();
+  DBuf dx, dy;
+  h->load_in(dx, x, h->N);
+  dy.alloc(size_t(h->N) * h->K * sizeof(double));
+  HIPCHK(launch_apply(h->p, op, h->mesh(), dx.d(), h->u1.d(), h->u2.d(), h->vnw.d(), h->tab.d(), h->off, dy.d(),
+                      h->stream));
+  h->store_out(dy, h->N, y);
+}
+
+int festung_apply_operator(festung_problem* h, int op, const double* x, double* y) {
+  return guard([&] {
+    check_handle(h);
+    apply_operator(h, op, x, y);
+  });
+}
+
+int festung_apply_operator_dev(festung_problem* h, int op, const double* x, double* y) {
+  return guard([&] {
+    check_handle(h);
+    struct Reset {
+      festung_problem* h;
+      ~Reset() { h->ioDev = false; }
+    } reset{h};
+    h->ioDev = true;
+    apply_operator(h, op, x, y);
+  });
+}
+
+int festung_assemble_vectors(festung_problem* h, const double* fq, const double* cDe, const double* gNe, double* L,
+                             double* KD, double* KN) {
+  return guard([&] {
+    check_handle(h);
+    h->set_device();
+    if ((L && !fq) || (KD && !cDe) || (KN && !gNe)) fail(FESTUNG_ERR_ARG, "an output needs its point values");
+    if (KD) h->require_velocity();
+    const int K = h->K, N = h->N;
+    DBuf pts, out;
+    out.alloc(size_t(N) * K * sizeof(double));
+    // one source launch per vector, from that vector's data alone: globL
+    // with the sign it has, globKD and globKN negated back
+    auto one = [&](int which, const double* src, int cols, double* dst) {
+      h->upload_elem(pts, src, cols);
+      HIPCHK(launch_source(h->p, h->mesh(), which == 0 ? pts.d() : nullptr, which == 1 ? pts.d() : nullptr,
+                           which == 2 ? pts.d() : nullptr, h->vn.d(), h->tab.d(), h->off, out.d(), h->stream));
+      h->download_elem(out, out.d(), N, dst);
+      if (which != 0)
+        for (size_t i = 0; i < size_t(N) * K; ++i) dst[i] = -dst[i];
+    };
+    if (L) one(0, fq, h->tabs.RE, L);
+    if (KD) one(1, cDe, 3 * h->tabs.Re, KD);
+    if (KN) one(2, gNe, 3 * h->tabs.Re, KN);
+  });
+}
+
+int festung_vertex_based_limiter(festung_problem* h, const double* valCentroid, const double* valV0T,
+                                 const uint8_t* markV0TbdrD, const double* dataV0T, double* alphaE, double* minV0T,
+                                 double* maxV0T) {
+  return guard([&] {
+    check_handle(h);
+    if (!valCentroid || !valV0T) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    h->set_device();
+    const int K = h->K, numV = h->numV;
+    // valD(markV0TbdrD) = dataV0T(markV0TbdrD), NaN elsewhere
+    // (core/computeMinMaxV0TElementPatch.m:81-82)
+    const bool hasD = markV0TbdrD && dataV0T;
+    std::vector<double> valD;
+    if (hasD) {
+      valD.assign(3 * size_t(K), std::numeric_limits<double>::quiet_NaN());
+      for (size_t i = 0; i < valD.size(); ++i)
+        if (markV0TbdrD[i]) valD[i] = dataV0T[i];
+    }
+    DBuf cenIn, cen, val, dD, part, gm, vmn, vmx, al, mn, mx;
+    h->upload_elem(cenIn, valCentroid, 1);
+    h->upload_elem(val, valV0T, 3);
+    if (hasD) h->upload_elem(dD, valD.data(), 3);
+    cen.alloc(size_t(K) * sizeof(double));
+    part.alloc(2 * size_t(h->nblocks) * sizeof(double));
+    gm.alloc(2 * sizeof(double));
+    vmn.alloc(size_t(numV) * sizeof(double));
+    vmx.alloc(size_t(numV) * sizeof(double));
+    al.alloc(size_t(K) * sizeof(double));
+    mn.alloc(3 * size_t(K) * sizeof(double));
+    mx.alloc(3 * size_t(K) * sizeof(double));
+    HIPCHK(launch_centroid(K, h->bs, cenIn.d(), 1.0, cen.d(), MinMaxSink{part.d(), 1}, h->stream));
+    VertexArgs va{numV, h->maxDeg, h->v2tPad.i(), cen.d(), K, 1, part.d(), h->nblocks, gm.d(), vmn.d(), vmx.d()};
+    HIPCHK(launch_vertex(va, h->stream));
+    HIPCHK(launch_vertex_limiter(h->mesh(), cen.d(), val.d(), hasD ? dD.d() : nullptr, vmn.d(), vmx.d(), gm.d(),
+                                 al.d(), mn.d(), mx.d(), h->stream));
+    if (alphaE) h->download_elem(al, al.d(), 1, alphaE);
+    if (minV0T) h->download_elem(mn, mn.d(), 3, minV0T);
+    if (maxV0T) h->download_elem(mx, mx.d(), 3, maxV0T);
+    sync(h);
+  });
+}
+
+int festung_apply_slope_limiter_disc(festung_problem* h, const double* dataDisc, const uint8_t* markV0TbdrD,
+                                     const double* dataV0T, double* dataDiscLim, double* minV0T, double* maxV0T) {
+  return guard([&] {
+    check_handle(h);
+    if (!dataDisc || !dataDiscLim) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    if (!(h->flags & FESTUNG_FLAG_LIMITER)) fail(FESTUNG_ERR_STATE, "problem created without the limiter");
+    const int K = h->K, numV = h->numV;
+    // the per-(element, vertex) Dirichlet data as per-vertex values: they
+    // must agree between the elements that share a vertex
+    std::vector<double> vD;
+    if (markV0TbdrD && dataV0T) {
+      vD.assign(numV, std::numeric_limits<double>::quiet_NaN());
+      std::vector<int8_t> seen(numV, -1);
+      for (size_t i = 0; i < 3 * size_t(K); ++i) {
+        const int32_t v = h->V0Tref[i];
+        const int8_t m = markV0TbdrD[i] ? 1 : 0;
+        if (seen[v] >= 0 && seen[v] != m)
+          fail(FESTUNG_ERR_ARG, "markV0TbdrD differs between the elements sharing vertex %d", v);
+        seen[v] = m;
+        if (!m) continue;
+        if (!std::isnan(vD[v]) && !(vD[v] == dataV0T[i]))
+          fail(FESTUNG_ERR_ARG, "dataV0T differs between the elements sharing vertex %d", v);
+        vD[v] = dataV0T[i];
+      }
+    }
+    check_call(festung_set_state(h, dataDisc));
+    check_call(festung_set_vertex_dirichlet(h, vD.empty() ? nullptr : vD.data()));
+    check_call(festung_limit(h));
+    check_call(festung_get_state(h, dataDiscLim));
+    if (minV0T || maxV0T) check_call(festung_limiter_info(h, nullptr, minV0T, maxV0T));
+  });
+}
+
+int festung_runge_kutta_explicit(int ord, double tau, double t0, double* t, double* omega, int32_t* nStages) {
+  return guard([&] {
+    if (!t || !omega) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    // core/rungeKuttaExplicit.m:64-78 (SSP orders 1..3); 4 = classic RK4,
+    // whose stage times are returned with omega = NaN (not a convex form)
+    static const double c1[1] = {0.0}, w1[1] = {0.0};
+    static const double c2[2] = {0.0, 1.0}, w2[2] = {0.0, 0.5};
+    static const double c3[3] = {0.0, 1.0, 0.5}, w3[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
+    static const double c4[4] = {0.0, 0.5, 0.5, 1.0};
+    const double* c = nullptr;
+    const double* w = nullptr;
+    int S = 0;
+    switch (ord) {
+      case 1: c = c1; w = w1; S = 1; break;
+      case 2: c = c2; w = w2; S = 2; break;
+      case 3: c = c3; w = w3; S = 3; break;
+      case 4: c = c4; S = 4; break;
+      default: fail(FESTUNG_ERR_UNSUPPORTED, "Runge-Kutta order %d not supported (1..4)", ord);
+    }
+    for (int i = 0; i < S; ++i) {
+      t[i] = t0 + c[i] * tau;
+      omega[i] = w ? w[i] : std::numeric_limits<double>::quiet_NaN();
+    }
+    if (nStages) *nStages = S;
   });
 }
 
@@ -1608,6 +1829,7 @@ struct festung_batch {
   std::vector<double> taus;
   std::vector<int> versions;
   std::vector<hipEvent_t> events;
+  hipEvent_t done = nullptr;  // recorded after each launch; every member's stream waits on it
   void drop() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1618,6 +1840,7 @@ struct festung_batch {
   ~festung_batch() {
     drop();
     for (auto e : events) (void)hipEventDestroy(e);
+    if (done) (void)hipEventDestroy(done);
   }
 };
 
@@ -1639,6 +1862,7 @@ int festung_batch_create(festung_problem* const* hs, int32_t n, festung_batch** 
     b->device = hs[0]->device;
     b->events.resize(size_t(n) + 1);
     for (auto& e : b->events) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
     *out = b.release();
   });
 }
@@ -1666,6 +1890,7 @@ int festung_batch_step(festung_batch* b, int32_t nsteps, const double* tau) {
     std::vector<int> versions;
     for (auto* h : b->hs) versions.push_back(h->version);
     if (!b->exec || b->nsteps != nsteps || b->taus != taus || b->versions != versions) {
+      if (b->exec) HIPCHK(hipStreamSynchronize(h0->stream));  // the last launch may still run
       b->drop();
       HIPCHK(hipStreamBeginCapture(h0->stream, hipStreamCaptureModeThreadLocal));
       try {
@@ -1690,6 +1915,10 @@ int festung_batch_step(festung_batch* b, int32_t nsteps, const double* tau) {
       b->versions = versions;
     }
     HIPCHK(hipGraphLaunch(b->exec, h0->stream));
+    // later work on any member's own stream (its per-problem entries) runs
+    // after the batch, without a host synchronisation
+    HIPCHK(hipEventRecord(b->done, h0->stream));
+    for (size_t i = 1; i < n; ++i) HIPCHK(hipStreamWaitEvent(b->hs[i]->stream, b->done, 0));
     for (auto* h : b->hs) h->cur = 0;
   });
 }

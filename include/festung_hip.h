@@ -164,13 +164,16 @@ int festung_set_velocity_points(festung_problem* h, const double* u1q, const dou
                                 const double* u2e);
 
 /* Right-hand side of the stage: fq = f at the element points of
- * quadRule2D(2p) (K x RE, may be NULL for f = 0) and cDe = c_D at the edge
- * points of quadRule1D(2p+1) (K x 3 x Re, may be NULL for c_D = 0).  The
- * device forms globL - globKD (advection/preprocessSubStep.m:69-98:
- * core/projectFuncCont2DataDisc.m, core/assembleMatElemPhiPhi.m,
- * core/assembleVecEdgePhiIntFuncContVal.m).  With both NULL the source is
- * cleared.  Takes effect for the following stages. */
-int festung_set_source_points(festung_problem* h, const double* fq, const double* cDe);
+ * quadRule2D(2p) (K x RE, may be NULL for f = 0), cDe = c_D and gNe = g_N at
+ * the edge points of quadRule1D(2p+1) (K x 3 x Re each, may be NULL for
+ * zero data).  The device forms globL - globKD - globKN
+ * (advection/preprocessSubStep.m:69-98: core/projectFuncCont2DataDisc.m,
+ * core/assembleMatElemPhiPhi.m, core/assembleVecEdgePhiIntFuncContVal.m on
+ * the Dirichlet edges, core/assembleVecEdgePhiIntFuncCont.m with the upwind
+ * part of g_N, (g_N <= 0) g_N, on the Neumann edges of
+ * festung_grid_set_boundary).  With all three NULL the source is cleared.
+ * Takes effect for the following stages. */
+int festung_set_source_points(festung_problem* h, const double* fq, const double* cDe, const double* gNe);
 
 /* Dirichlet data at the vertices for the limiter: vertexD (numV values, NaN
  * where not given), used for vertices with markV0TbdrD
@@ -186,7 +189,7 @@ int festung_set_vertex_dirichlet(festung_problem* h, const double* vertexD);
  * advection/preprocessSubStep.m:61-98 and core/computeFuncContV0T.m as above. */
 int festung_set_velocity_points_dev(festung_problem* h, const double* u1q, const double* u2q, const double* u1e,
                                     const double* u2e);
-int festung_set_source_points_dev(festung_problem* h, const double* fq, const double* cDe);
+int festung_set_source_points_dev(festung_problem* h, const double* fq, const double* cDe, const double* gNe);
 int festung_set_vertex_dirichlet_dev(festung_problem* h, const double* vertexD);
 
 /* L2 projection of point values fq (K x R of quadRule2D(ord), ord in
@@ -264,6 +267,14 @@ void festung_batch_destroy(festung_batch* b);
 int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double* cObs, int32_t ckptSteps,
                     double* J, double* gradC0, double* gradU1, double* gradU2, double* gradVn,
                     double* timing_ms);
+/* festung_adjoint with device pointers (the problem's device, reference
+ * element order, layouts [N][K] and [Re][3][K]) for cObs and the gradient
+ * outputs; J and timing_ms stay host values.  Lets a torch caller keep the
+ * observation and the gradients on the GPU.  No counterpart in the
+ * reference. */
+int festung_adjoint_dev(festung_problem* h, int32_t nsteps, double tau, const double* cObs, int32_t ckptSteps,
+                        double* J, double* gradC0, double* gradU1, double* gradU2, double* gradVn,
+                        double* timing_ms);
 /* The binomial checkpointing schedule festung_adjoint runs under
  * FESTUNG_FLAG_ADJ_BINOMIAL: reversing nsteps steps with `snapshots` state
  * snapshots (the initial state included) takes *advances forward steps
@@ -274,6 +285,63 @@ int festung_binomial_schedule(int32_t nsteps, int32_t snapshots, int64_t* advanc
 /* out = M^-1 A^T mu for the current velocity (K x N): the transposed
  * right-hand-side operator, for dot-product tests of the adjoint. */
 int festung_adj_rhs(festung_problem* h, const double* mu, double* out);
+
+/* ------------------------------------------- operator-level entry points --
+ * The reference's assembled objects one at a time, for callers that keep
+ * advection/solveSubStep.m:62-66 as written (operator objects whose mtimes /
+ * mldivide call these).  x, y are K x N column-major (reference element
+ * order); the _dev variant takes device memory in the same layout (enqueued
+ * on the problem's stream, returns when the result is written). */
+enum {
+  FESTUNG_OP_G = 0,    /* globG{1} + globG{2} */
+  FESTUNG_OP_G1 = 1,   /* globG{1}: core/assembleMatElemDphiPhiFuncDisc.m:181-202 with u1Disc */
+  FESTUNG_OP_G2 = 2,   /* globG{2}: the same with u2Disc */
+  FESTUNG_OP_R = 3,    /* globR: core/assembleMatEdgePhiPhiValUpwind.m:261-299 with markE0Tbdr =
+                          ~markE0TbdrN and vNormalOnQuadEdge (advection/preprocessSubStep.m:83-85) */
+  FESTUNG_OP_M = 4,    /* globM: core/assembleMatElemPhiPhi.m:88-100 */
+  FESTUNG_OP_MINV = 5  /* globM \ x: the block solve of advection/solveSubStep.m:66 */
+};
+/* y = A x for A = FESTUNG_OP_* at the problem's current velocity (set by
+ * festung_set_velocity_*; G and R need it). */
+int festung_apply_operator(festung_problem* h, int op, const double* x, double* y);
+int festung_apply_operator_dev(festung_problem* h, int op, const double* x, double* y);
+
+/* The stage vectors of advection/preprocessSubStep.m:87-98 from point
+ * values (as festung_set_source_points takes them), each formed alone and
+ * returned K x N: L = globM fDisc from fq (core/projectFuncCont2DataDisc.m,
+ * :98), KD = core/assembleVecEdgePhiIntFuncContVal.m:162-185 from cDe on the
+ * Dirichlet edges at the current velocity, KN =
+ * core/assembleVecEdgePhiIntFuncCont.m:126-135 from the upwind part of gNe
+ * on the Neumann edges.  An output may be NULL; a non-NULL output needs its
+ * input.  Does not change the stage source. */
+int festung_assemble_vectors(festung_problem* h, const double* fq, const double* cDe, const double* gNe, double* L,
+                             double* KD, double* KN);
+
+/* core/computeVertexBasedLimiter.m:78-106 with its own inputs: valCentroid
+ * (K), valV0T (K x 3), markV0TbdrD (K x 3 bytes) and dataV0T (K x 3; both
+ * NULL: no Dirichlet data).  Outputs (any may be NULL): alphaE (K) and
+ * minMaxV0T as minV0T, maxV0T (K x 3), with the global shift of
+ * core/computeMinMaxV0TElementPatch.m:66-112.  The handle supplies the mesh
+ * (vertex patches); its state is not touched. */
+int festung_vertex_based_limiter(festung_problem* h, const double* valCentroid, const double* valV0T,
+                                 const uint8_t* markV0TbdrD, const double* dataV0T, double* alphaE, double* minV0T,
+                                 double* maxV0T);
+
+/* core/applySlopeLimiterDisc.m:77-86: dataDisc (K x N) limited with the
+ * handle's limiter type -> dataDiscLim (K x N), minMaxV0T -> minV0T, maxV0T
+ * (K x 3, may be NULL).  markV0TbdrD (K x 3 bytes) and dataV0T (K x 3) give
+ * the Dirichlet vertex data (both NULL: none); the values of one vertex must
+ * agree between its elements (FESTUNG_ERR_ARG otherwise), as
+ * core/computeFuncContV0T.m produces them.  The limited data become the
+ * handle's state.  Needs FESTUNG_FLAG_LIMITER. */
+int festung_apply_slope_limiter_disc(festung_problem* h, const double* dataDisc, const uint8_t* markV0TbdrD,
+                                     const double* dataV0T, double* dataDiscLim, double* minV0T, double* maxV0T);
+
+/* core/rungeKuttaExplicit.m:64-78: stage times t[0..S-1] = t0 + c tau and the
+ * SSP weights omega for ord 1..3; ord 4 (classic RK4, an extension) gives its
+ * stage times and omega = NaN.  t, omega hold at least 4 values; *nStages = S.
+ * Host only. */
+int festung_runge_kutta_explicit(int ord, double tau, double t0, double* t, double* omega, int32_t* nStages);
 
 /* Apply the slope limiter to the current state.
  * Replaces core/applySlopeLimiterDisc.m:77-86 (the limiter type chosen by

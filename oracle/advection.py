@@ -112,7 +112,7 @@ def preprocessProblem(pd):
     pd.K = pd.g.numT
     pd.N = (pd.p + 1) * (pd.p + 2) // 2
     pd.tau = pd.tEnd / pd.numSteps
-    G.markBoundaries(pd.g)
+    G.markBoundaries(pd.g, pd.get('generateMarkE0TbdrN'))
     G.computeDerivedGridData(pd.g)
     pd.basesOnQuad = Bm.computeBasesOnQuad(pd.N)
     if pd.isSlopeLim:
@@ -225,6 +225,8 @@ def solveSubStep(pd, nStep, nSubStep):
         pd.cDiscRK[s + 1] = w * pd.cDiscRK[0] + (1 - w) * (pd.cDiscRK[s] + pd.tau * cDiscDot)
     if pd.isSlopeLim:
         tBC = pd.t[nSubStep] if nSubStep < len(pd.t) else pd.t[0] + pd.tau
+        # kept for checks of the limiter's decisions (alpha of this application)
+        pd.lastLimiterInput = (pd.cDiscRK[s + 1].reshape(K, N).copy(), tBC)
         cDV0T = A.computeFuncContV0T(pd.g, lambda x1, x2: pd.cDCont(tBC, x1, x2))
         lim, _ = L.applySlopeLimiterDisc(pd.g, pd.cDiscRK[s + 1].reshape(K, N), pd.g.markV0TbdrD, cDV0T,
                                          pd.globM, pd.globMDiscTaylor, pd.basesOnQuad, pd.typeSlopeLim)

@@ -173,11 +173,13 @@ def mapRef2Phy(g, i, X1, X2):
             + np.outer(g.coordV0T[:, 0, i], np.ones_like(X1)))
 
 
-def markBoundaries(g):
+def markBoundaries(g, generateMarkE0TbdrN=None):
     """Boundary marks of advection/configureProblem.m:148-150 and
-    advection/preprocessProblem.m:72-76."""
+    advection/preprocessProblem.m:72-76; generateMarkE0TbdrN(g) gives the
+    Neumann marks (the reference's default: none)."""
     g.markE0Tint = g.idE0T == 0
-    g.markE0TbdrN = np.zeros((g.numT, 3), dtype=bool)
+    g.markE0TbdrN = (np.zeros((g.numT, 3), dtype=bool) if generateMarkE0TbdrN is None
+                     else np.asarray(generateMarkE0TbdrN(g), dtype=bool))
     g.markE0TbdrD = ~(g.markE0Tint | g.markE0TbdrN)
     bdrV = np.unique(g.V0E[g.E0T[g.markE0TbdrD], :].ravel())
     g.markV0TbdrD = np.isin(g.V0T, bdrV)

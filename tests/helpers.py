@@ -54,3 +54,14 @@ def oracle_limit_disc_linear(pd, c, t):
     """core/applySlopeLimiterDiscLinear.m with the problem's Dirichlet vertex data."""
     cDV0T = OAs.computeFuncContV0T(pd.g, lambda x1, x2: pd.cDCont(t, x1, x2))
     return OL.applySlopeLimiterDiscLinear(pd.g, c, pd.g.markV0TbdrD, cDV0T)
+
+
+def oracle_alpha(pd, c_pre, t):
+    """alpha (per element) of core/computeVertexBasedLimiter.m for the
+    solution limiter applied to c_pre at time t with the problem's Dirichlet
+    vertex data: the decisions of core/applySlopeLimiterTaylorLinear.m."""
+    tt = OL.projectDataDisc2DataTaylor(c_pre, pd.globM, pd.globMDiscTaylor)
+    valV0T = OAs.computeFuncDiscAtPoints(tt[:, :3], pd.basesOnQuad['phiTaylorV0T'][:, :, :3])
+    cD = OAs.computeFuncContV0T(pd.g, lambda x1, x2: pd.cDCont(t, x1, x2))
+    a, _ = OL.computeVertexBasedLimiter(pd.g, tt[:, 0], valV0T, pd.g.markV0TbdrD, cD)
+    return a

@@ -127,6 +127,29 @@ def test_autograd_wrapper(gpu):
     assert np.allclose(at.grad.numpy(), ga, rtol=0, atol=0)
 
 
+def test_autograd_wrapper_on_device(gpu):
+    """c0, c_obs and the gradients as CUDA tensors ([N][K]) through
+    festung_adjoint_dev: the same bits as the host entry point (the velocity
+    is formed on the device in the host's order of operations); with no
+    gradient requested the forward run alone gives J."""
+    import torch
+    m, a_true, a, tau = setup(1, 1 / 8, 4)
+    cObs = m.run(m.c0, a_true, 5, tau)
+    F = AD.misfit_function()
+    c0 = torch.tensor(np.ascontiguousarray(m.c0.T), device='cuda', requires_grad=True)
+    at = torch.tensor(a, device='cuda', requires_grad=True)
+    cO = torch.tensor(np.ascontiguousarray(cObs.T), device='cuda')
+    J = F.apply(c0, at, m, 5, tau, cO, 0)
+    J.backward()
+    Jh, g0, ga, _ = m.gradient(m.c0, a, 5, tau, cObs)
+    assert J.is_cuda and c0.grad.is_cuda and at.grad.is_cuda
+    assert J.item() == Jh
+    assert np.array_equal(c0.grad.cpu().numpy().T, g0)
+    assert np.allclose(at.grad.cpu().numpy(), ga, rtol=1e-13, atol=0)
+    J2 = F.apply(c0.detach(), at.detach(), m, 5, tau, cO, 0)
+    assert J2.is_cuda and abs(J2.item() - Jh) <= 1e-12 * abs(Jh)
+
+
 def test_adjoint_elem_block_invariance(gpu):
     """The transposed stage kernels, with their fused velocity sensitivities,
     give the same gradient bit for bit for every workgroup size."""

@@ -1,0 +1,139 @@
+"""Operator-level entry points against the oracle's assembled objects: the
+element operators globG{1}, globG{2} (core/assembleMatElemDphiPhiFuncDisc.m),
+the upwind edge operator globR (core/assembleMatEdgePhiPhiValUpwind.m, with
+Neumann edges), globM and its block solve, the stage vectors globL, globKD,
+globKN, computeVertexBasedLimiter and applySlopeLimiterDisc on their own
+inputs, and the composition of advection/solveSubStep.m:62-66 from them.
+Tolerance 1e-12 relative (FP64); limiter decisions exactly."""
+import numpy as np
+import pytest
+
+from festung_amd.advection import steps as S
+from festung_amd.advection import testcases as TC
+from helpers import oracle_stage_operators, rel_l2
+from oracle import advection as OA
+from oracle import assembly as OAs
+from oracle import limiter as OL
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _both(p, lim=False, neumann=False, hmax=1 / 8):
+    tc = dict(TC.rotating_humps())
+    tc['cDCont'] = lambda t, x1, x2: 0.2 + 0.1 * x1 - 0.3 * x2 + 0 * t
+    tc['gNCont'] = lambda t, x1, x2: (x2 - 0.5) + 0 * x1 + 0 * t
+    tc['fCont'] = lambda t, x1, x2: 0.5 * x1 * x2 + 0 * t
+    tc.update(fZero=False, cDZero=False)
+    marks = (lambda g: g.idE0T == 2) if neumann else None
+    kw = dict(p=p, hmax=hmax, ordRK=min(p + 1, 3), isSlopeLim=lim, numSteps=10, tEnd=0.1)
+    o = OA.Struct(testcase='custom', generateMarkE0TbdrN=marks, **kw)
+    for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):
+        o[k] = tc[k]
+    for f in (OA.configureProblem, OA.preprocessProblem, OA.initializeProblem):
+        o = f(o)
+    o = oracle_stage_operators(o, 0.0)
+    d = S.Struct(testcase='custom', useGraph=False, generateMarkE0TbdrN=marks, **kw)
+    d.update(tc)
+    for f in (S.configureProblem, S.preprocessProblem, S.initializeProblem):
+        d = f(d)
+    d.t = np.array([0.0])
+    S.preprocessSubStep(d, 1, 1)
+    return o, d
+
+
+@pytest.mark.parametrize('p,neumann', [(0, False), (1, False), (2, True), (3, False), (4, True)])
+def test_operators_match_assembled(gpu, p, neumann):
+    o, d = _both(p, neumann=neumann)
+    rng = np.random.default_rng(200 + p)
+    x = rng.standard_normal((o.K, o.N))
+    xv = x.reshape(-1)                    # element-major KN vector
+    ref = {'G1': o.globG[0] @ xv, 'G2': o.globG[1] @ xv, 'G': o.globG[0] @ xv + o.globG[1] @ xv,
+           'R': o.globR @ xv, 'M': o.globM @ xv}
+    import scipy.sparse.linalg as spla
+    ref['Minv'] = spla.spsolve(o.globM.tocsc(), xv)
+    for op, r in ref.items():
+        got = d.problem.apply_operator(op, x)
+        r = r.reshape(o.K, o.N)
+        if p == 0 and op in ('G1', 'G2', 'G'):
+            assert np.abs(got).max() == 0.0 and np.abs(r).max() == 0.0
+            continue
+        assert rel_l2(got, r) < TOL, op
+
+
+def test_operator_dev_equals_host(gpu):
+    import torch
+    o, d = _both(2, neumann=True)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((o.K, o.N))
+    xt = torch.tensor(np.ascontiguousarray(x.T), device='cuda')
+    for op in ('G1', 'G2', 'G', 'R', 'M', 'Minv'):
+        yt = torch.empty_like(xt)
+        d.problem.apply_operator_dev(op, xt, yt)
+        assert np.array_equal(yt.cpu().numpy().T, d.problem.apply_operator(op, x)), op
+
+
+@pytest.mark.parametrize('p', [1, 2, 4])
+def test_stage_vectors_and_solve_substep_composition(gpu, p):
+    """globL, globKD, globKN from the point values, and
+    cdot = globM \\ (L - KD - KN - (-G1 - G2 + R) c) composed from the
+    operator entries, against the oracle and the fused festung_rhs."""
+    o, d = _both(p, neumann=True)
+    X1, X2 = d.XEdge
+    Lv, KD, KN = d.problem.assemble_vectors(d.fCont(0.0, *d.XE), d.cDCont(0.0, X1, X2), d.gNCont(0.0, X1, X2))
+    assert rel_l2(Lv, o.globL.reshape(o.K, o.N)) < TOL
+    assert rel_l2(KD, o.globKD.reshape(o.K, o.N)) < TOL
+    assert rel_l2(KN, o.globKN.reshape(o.K, o.N)) < TOL
+    assert np.abs(o.globKN).max() > 0 and np.abs(o.globKD).max() > 0
+    rng = np.random.default_rng(9 + p)
+    c = rng.standard_normal((o.K, o.N))
+    P = d.problem
+    sysA_c = -P.apply_operator('G1', c) - P.apply_operator('G2', c) + P.apply_operator('R', c)
+    cdot = P.apply_operator('Minv', Lv - KD - KN - sysA_c)
+    ref = OA.computeDot(o, c.reshape(-1)).reshape(o.K, o.N)
+    assert rel_l2(cdot, ref) < TOL
+    P.set_state(c)
+    assert rel_l2(P.rhs(), cdot) < TOL
+
+
+@pytest.mark.parametrize('p', [1, 2, 3, 4])
+def test_compute_vertex_based_limiter_own_inputs(gpu, p):
+    """core/computeVertexBasedLimiter.m on caller data (random centroid and
+    vertex values, Dirichlet data on the marked vertices): alpha and bounds
+    as the oracle computes them, every decision the same."""
+    o, d = _both(p, lim=True)
+    rng = np.random.default_rng(300 + p)
+    K = o.K
+    cen = rng.standard_normal(K)
+    val = cen[:, None] + 0.7 * rng.standard_normal((K, 3))
+    mark = o.g.markV0TbdrD
+    data = np.where(mark, rng.standard_normal((K, 3)), np.nan)
+    aref, (mnr, mxr) = OL.computeVertexBasedLimiter(o.g, cen, val, mark, data)
+    a, (mn, mx) = d.problem.vertex_based_limiter(cen, val, mark, data)
+    assert 0 < np.sum(aref < 1) < K
+    assert np.array_equal(a < 1, aref < 1)
+    assert np.max(np.abs(a - aref)) < 1e-12
+    assert np.array_equal(mn, mnr) and np.array_equal(mx, mxr)
+    a2, _ = d.problem.vertex_based_limiter(cen, val)
+    aref2, _ = OL.computeVertexBasedLimiter(o.g, cen, val, np.zeros_like(mark), np.full((K, 3), np.nan))
+    assert np.max(np.abs(a2 - aref2)) < 1e-12
+
+
+@pytest.mark.parametrize('p', [1, 2, 4])
+def test_apply_slope_limiter_disc_own_inputs(gpu, p):
+    o, d = _both(p, lim=True)
+    rng = np.random.default_rng(400 + p)
+    c = o.cDisc + 0.05 * rng.standard_normal(o.cDisc.shape)
+    cDV0T = OAs.computeFuncContV0T(o.g, lambda x1, x2: o.cDCont(0.0, x1, x2))
+    ref, (mnr, mxr) = OL.applySlopeLimiterDisc(o.g, c, o.g.markV0TbdrD, cDV0T, o.globM, o.globMDiscTaylor,
+                                               o.basesOnQuad, 'linear')
+    got, (mn, mx) = d.problem.apply_slope_limiter_disc(c, o.g.markV0TbdrD, cDV0T)
+    assert rel_l2(got, ref) < TOL
+    assert np.max(np.abs(mn - mnr)) < 1e-14 and np.max(np.abs(mx - mxr)) < 1e-14
+    bad = cDV0T.copy()
+    k, n = np.argwhere(o.g.markV0TbdrD)[0]
+    bad[k, n] += 1.0
+    from festung_amd._lib import FestungError
+    with pytest.raises(FestungError, match='dataV0T differs'):
+        d.problem.apply_slope_limiter_disc(c, o.g.markV0TbdrD, bad)

@@ -330,6 +330,48 @@ def test_free_stream_preservation(p):
     assert np.abs(dot).max() < 1e-12
 
 
+@pytest.mark.parametrize('p', [0, 1, 2, 3, 4])
+def test_neumann_vector_vs_bruteforce(p):
+    """K_N (core/assembleVecEdgePhiIntFuncCont.m:126-135) on the east edges
+    (id 2) against a direct Gauss-Legendre integration of g_N phi_i(F_k^-1 x)
+    along each physical edge (no gammaMap or phi1D tables).  g_N <= 0 is kept
+    whole, g_N > 0 contributes nothing (the upwind part of
+    advection/preprocessSubStep.m:92); g_N linear, so the 2p+1 edge rule is
+    exact."""
+    pd = OA.Struct(testcase='custom', p=p, hmax=1 / 4, numSteps=10)
+    z = lambda t, x1, x2: np.zeros_like(x1)
+    pd.u1Cont = lambda t, x1, x2: 0.5 - x2
+    pd.u2Cont = lambda t, x1, x2: x1 - 0.5
+    pd.c0Cont = lambda x1, x2: np.ones_like(x1)
+    pd.fCont = z; pd.cDCont = z; pd.gNCont = z
+    pd.generateMarkE0TbdrN = lambda g: g.idE0T == 2
+    pd = OA.configureProblem(pd)
+    for sign in (-1.0, 1.0):
+        pd.gNCont = lambda t, x1, x2, s=sign: s * (1.0 + x1 + 0.5 * x2)
+        pd = OA.preprocessProblem(pd)
+        g = pd.g
+        assert g.markE0TbdrN.sum() == 4 and not np.any(g.markE0TbdrD & g.markE0TbdrN)
+        pd.t = np.array([0.0])
+        pd = OA.preprocessSubStep(pd, 1, 1)
+        got = pd.globKN.reshape(pd.K, pd.N)
+        ref = np.zeros_like(got)
+        s_, w_ = np.polynomial.legendre.leggauss(10)
+        s_ = (s_ + 1) / 2; w_ = w_ / 2
+        for k, n in zip(*np.nonzero(g.markE0TbdrN)):
+            a = g.coordV0T[k, (n + 1) % 3]; b = g.coordV0T[k, (n + 2) % 3]
+            X = a[:, None] + np.outer(b - a, s_)
+            xh = np.linalg.solve(g.B[k], X - g.coordV0T[k, 0][:, None])
+            gn = np.minimum(pd.gNCont(0.0, X[0], X[1]), 0.0)
+            L_ = np.hypot(*(b - a))
+            for i in range(pd.N):
+                ref[k, i] = L_ * np.sum(w_ * gn * B.phi(i + 1, xh[0], xh[1]))
+        if sign > 0:
+            assert np.abs(got).max() == 0.0
+        else:
+            assert np.abs(ref).max() > 0.1
+            assert np.allclose(got, ref, rtol=0, atol=1e-14 * np.abs(ref).max())
+
+
 def test_mass_balance():
     """Row i = 1 of the operator telescopes: the total mass changes only by
     the boundary fluxes."""
