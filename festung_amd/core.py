@@ -211,11 +211,14 @@ class Problem:
         n = None if gNe is None else fortran(gNe, (self.K, 3, self.Re))
         check(lib().festung_set_source_points(self._h, ptr(f), ptr(c), ptr(n)))
 
-    def _dev_args(self, tensors, shapes):
-        """Device pointers of float64 CUDA tensors in [cols][K] layout, made
-        ready for the problem's stream (it waits for the producing torch
-        stream; the caching allocator is told the problem's stream uses them).
-        Every tensor is checked before anything is enqueued."""
+    def _dev_call(self, fn, tensors, shapes, pre=(), post=()):
+        """Call a _dev entry point fn(h, *pre, pointers..., *post) on float64
+        CUDA tensors in [cols][K] layout (None passes NULL).  Every tensor is
+        checked before anything is enqueued; the problem's stream waits for
+        torch's current stream before the call, and torch's current stream
+        waits for the problem's after it, so torch neither frees nor reads
+        the tensors before the library is done with them (no allocator stream
+        bookkeeping that could outlive the problem's stream)."""
         import torch
         for t, shape in zip(tensors, shapes):
             if t is None:
@@ -224,36 +227,27 @@ class Problem:
                     and t.numel() == int(np.prod(shape))):
                 raise ValueError('expected a contiguous float64 CUDA tensor with %d values' % int(np.prod(shape)))
         ours = torch.cuda.ExternalStream(self.stream)
-        ours.wait_stream(torch.cuda.current_stream())
-        out = []
-        for t in tensors:
-            if t is None:
-                out.append(None)
-                continue
-            t.record_stream(ours)
-            out.append(C.c_void_p(t.data_ptr()))
-        return out
-
-    def _after_problem_stream(self):
-        """Make torch's current stream wait for the work enqueued so far on the
-        problem's stream (e.g. a copy into a tensor torch reads next)."""
-        import torch
-        torch.cuda.current_stream().wait_stream(torch.cuda.ExternalStream(self.stream))
+        cur = torch.cuda.current_stream()
+        ours.wait_stream(cur)
+        ptrs = [None if t is None else C.c_void_p(t.data_ptr()) for t in tensors]
+        try:
+            check(fn(self._h, *pre, *ptrs, *post))
+        finally:
+            cur.wait_stream(ours)
 
     def set_velocity_points_dev(self, u1q, u2q, u1e, u2e):
         """set_velocity_points from CUDA tensors laid out [RE][K] and [Re][3][K]
         (reference element order)."""
         K = self.K
-        a = self._dev_args([u1q, u2q, u1e, u2e], [(self.RE, K)] * 2 + [(self.Re, 3, K)] * 2)
-        check(lib().festung_set_velocity_points_dev(self._h, *a))
+        self._dev_call(lib().festung_set_velocity_points_dev, [u1q, u2q, u1e, u2e],
+                       [(self.RE, K)] * 2 + [(self.Re, 3, K)] * 2)
 
     def set_source_points_dev(self, fq=None, cDe=None, gNe=None):
-        a = self._dev_args([fq, cDe, gNe], [(self.RE, self.K), (self.Re, 3, self.K), (self.Re, 3, self.K)])
-        check(lib().festung_set_source_points_dev(self._h, *a))
+        self._dev_call(lib().festung_set_source_points_dev, [fq, cDe, gNe],
+                       [(self.RE, self.K), (self.Re, 3, self.K), (self.Re, 3, self.K)])
 
     def set_vertex_dirichlet_dev(self, vertexD):
-        a = self._dev_args([vertexD], [(self.grid.numV,)])
-        check(lib().festung_set_vertex_dirichlet_dev(self._h, *a))
+        self._dev_call(lib().festung_set_vertex_dirichlet_dev, [vertexD], [(self.grid.numV,)])
 
     def set_tau_field(self, tauK):
         t = None if tauK is None else fortran(tauK, (self.K,))
@@ -302,15 +296,12 @@ class Problem:
     def set_state_dev(self, t):
         """State from a contiguous float64 CUDA tensor laid out [N][K]
         (reference element order); ordered after torch's pending work on it."""
-        a = self._dev_args([t], [(self.N, self.K)])
-        check(lib().festung_set_state_dev(self._h, *a))
+        self._dev_call(lib().festung_set_state_dev, [t], [(self.N, self.K)])
 
     def get_state_dev(self, t):
         """Copy the state into a contiguous float64 CUDA tensor laid out
         [N][K]; torch's current stream waits for the copy."""
-        a = self._dev_args([t], [(self.N, self.K)])
-        check(lib().festung_get_state_dev(self._h, *a))
-        self._after_problem_stream()
+        self._dev_call(lib().festung_get_state_dev, [t], [(self.N, self.K)])
 
     def begin_step(self):
         check(lib().festung_begin_step(self._h))
@@ -376,11 +367,13 @@ class Problem:
         Returns dict(J, forward_ms, backward_ms); torch's current stream
         waits for the outputs."""
         K, N, Re = self.K, self.N, self.Re
-        a = self._dev_args([cObs, gradC0, gradU1, gradU2, gradVn], [(N, K)] * 4 + [(Re, 3, K)])
         J = np.zeros(1)
         tm = np.zeros(2)
-        check(lib().festung_adjoint_dev(self._h, int(nsteps), float(tau), a[0], int(ckpt), ptr(J), *a[1:], ptr(tm)))
-        self._after_problem_stream()
+        L_ = lib()
+
+        def call(h, cO, g0, g1, g2, gv):
+            return L_.festung_adjoint_dev(h, int(nsteps), float(tau), cO, int(ckpt), ptr(J), g0, g1, g2, gv, ptr(tm))
+        self._dev_call(call, [cObs, gradC0, gradU1, gradU2, gradVn], [(N, K)] * 4 + [(Re, 3, K)])
         return dict(J=float(J[0]), forward_ms=tm[0], backward_ms=tm[1])
 
     OPERATORS = {'G': L.OP_G, 'G1': L.OP_G1, 'G2': L.OP_G2, 'R': L.OP_R, 'M': L.OP_M, 'Minv': L.OP_MINV}
@@ -396,9 +389,7 @@ class Problem:
 
     def apply_operator_dev(self, op, x, y):
         """apply_operator on float64 CUDA tensors laid out [N][K]."""
-        a = self._dev_args([x, y], [(self.N, self.K)] * 2)
-        check(lib().festung_apply_operator_dev(self._h, self.OPERATORS[op], *a))
-        self._after_problem_stream()
+        self._dev_call(lib().festung_apply_operator_dev, [x, y], [(self.N, self.K)] * 2, pre=(self.OPERATORS[op],))
 
     def assemble_vectors(self, fq=None, cDe=None, gNe=None):
         """(globL, globKD, globKN) as K x N arrays from point values (None

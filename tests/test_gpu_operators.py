@@ -132,7 +132,9 @@ def test_apply_slope_limiter_disc_own_inputs(gpu, p):
     assert rel_l2(got, ref) < TOL
     assert np.max(np.abs(mn - mnr)) < 1e-14 and np.max(np.abs(mx - mxr)) < 1e-14
     bad = cDV0T.copy()
-    k, n = np.argwhere(o.g.markV0TbdrD)[0]
+    marked = np.argwhere(o.g.markV0TbdrD)
+    counts = np.bincount(o.g.V0T[o.g.markV0TbdrD])
+    k, n = next((k, n) for k, n in marked if counts[o.g.V0T[k, n]] > 1)   # a vertex of two elements
     bad[k, n] += 1.0
     from festung_amd._lib import FestungError
     with pytest.raises(FestungError, match='dataV0T differs'):

@@ -88,6 +88,7 @@ def test_state_dev_follows_torch_stream(gpu):
     problem writes is read by torch after the copy (no explicit sync)."""
     import torch
     d = product_problem(TC.rotating_humps(), 2, 1 / 32, 3, True, graph=True)
+    product_step(d, 1)                     # sets the velocity
     K, N = d.K, d.N
     rng = np.random.default_rng(5)
     c = rng.standard_normal((K, N))

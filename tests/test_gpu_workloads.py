@@ -28,6 +28,12 @@ from oracle import advection as OA
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
+# alpha = dMin / (d -/+ tol) (core/computeVertexBasedLimiter.m:100-101) with
+# tol = 1e-8: where the data are locally flat, dMin and d are rounding-level
+# (1e-16 of the data), so a rounding difference moves alpha by up to about
+# 1e-16 / 1e-8 times the data scale; the state it scales (slopes of that
+# size) is checked by the state tolerance.  Decisions (alpha < 1) exactly.
+ALPHA_TOL = 1e-6
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
 
 
@@ -60,7 +66,7 @@ def test_p4_discontinuous_workload_steps(gpu, hmax, nsteps, graph):
         alpha = d.problem.limiter_info()[0]
         flips += int(np.sum((alpha < 1) != (aref < 1)))
         limited += int(np.sum(aref < 1))
-        assert np.max(np.abs(alpha - aref)) < 1e-10, n
+        assert np.max(np.abs(alpha - aref)) < ALPHA_TOL, n
     print('p=4 K=%d: %d limited element-steps, %d alpha flips' % (o.K, limited, flips))
     assert limited > 0 and flips == 0
 
@@ -102,4 +108,4 @@ def test_full_size_against_oracle_fixture(gpu, name):
     print('%s: K=%d, %d elements with alpha < 1, %d flips, block / sample rel diff %.2e / %.2e'
           % (name, d.problem.K, int(ref_bits.sum()), flips, r[0], r[1]))
     assert ref_bits.sum() > 0 and flips == 0
-    assert np.max(np.abs(alpha[f['sample_idx']] - f['alpha_sample'])) < 1e-10
+    assert np.max(np.abs(alpha[f['sample_idx']] - f['alpha_sample'])) < ALPHA_TOL
