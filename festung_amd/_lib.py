@@ -95,6 +95,7 @@ SIGNATURES = {
     'festung_adjoint_dev': (C.c_int, [vp, C.c_int32, C.c_double, vp, C.c_int32, dp, vp, vp, vp, vp, dp]),
     'festung_apply_operator': (C.c_int, [vp, C.c_int, dp, dp]),
     'festung_apply_operator_dev': (C.c_int, [vp, C.c_int, vp, vp]),
+    'festung_apply_operator_dev_noalloc': (C.c_int, [vp, C.c_int, vp, vp]),
     'festung_assemble_vectors': (C.c_int, [vp, dp, dp, dp, dp, dp, dp]),
     'festung_vertex_based_limiter': (C.c_int, [vp, dp, dp, C.POINTER(C.c_uint8), dp, dp, dp, dp]),
     'festung_apply_slope_limiter_disc': (C.c_int, [vp, dp, C.POINTER(C.c_uint8), dp, dp, dp, dp]),

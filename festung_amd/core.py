@@ -391,6 +391,12 @@ class Problem:
         """apply_operator on float64 CUDA tensors laid out [N][K]."""
         self._dev_call(lib().festung_apply_operator_dev, [x, y], [(self.N, self.K)] * 2, pre=(self.OPERATORS[op],))
 
+    def apply_operator_dev_noalloc(self, op, x, y):
+        """apply_operator_dev through the entry point that allocates nothing
+        (library-owned scratch; K*N <= 2^22)."""
+        self._dev_call(lib().festung_apply_operator_dev_noalloc, [x, y], [(self.N, self.K)] * 2,
+                       pre=(self.OPERATORS[op],))
+
     def assemble_vectors(self, fq=None, cDe=None, gNe=None):
         """(globL, globKD, globKN) as K x N arrays from point values (None
         where the input is None)."""

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "checkpoint.hpp"
@@ -1650,6 +1651,53 @@ int festung_apply_operator_dev(festung_problem* h, int op, const double* x, doub
     } reset{h};
     h->ioDev = true;
     apply_operator(h, op, x, y);
+  });
+}
+
+// Scratch of festung_apply_operator_dev_noalloc: the operand and the result
+// in the device element order, kNoAllocValues doubles each (64 MiB in all).
+// Every device the library runs on holds its own copy until the process
+// ends; its address there is looked up once per device (scratch_on).
+constexpr size_t kNoAllocValues = size_t(1) << 22;
+static __device__ __align__(256) double scratch[2 * kNoAllocValues];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+double* g_scratchPtr[kMaxDevices] = {};
+
+// the scratch's device address on `device` (current): hipGetSymbolAddress the
+// first time, the kept pointer afterwards (the host-side name of the array
+// is not a device address)
+double* scratch_on(int device) {
+  if (device < 0 || device >= kMaxDevices) fail(FESTUNG_ERR_UNSUPPORTED, "device %d: too many devices", device);
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchPtr[device]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratchPtr[device] = static_cast<double*>(p);
+  }
+  return g_scratchPtr[device];
+}
+}  // namespace
+
+int festung_apply_operator_dev_noalloc(festung_problem* h, int op, const double* x, double* y) {
+  return guard([&] {
+    check_handle(h);
+    if (op < FESTUNG_OP_G || op > FESTUNG_OP_MINV) fail(FESTUNG_ERR_ARG, "unknown operator %d", op);
+    if (!x || !y) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    const size_t kn = size_t(h->K) * h->N;
+    if (kn > kNoAllocValues)
+      fail(FESTUNG_ERR_ARG, "K*N = %zu exceeds the %zu values of the no-allocation scratch", kn, kNoAllocValues);
+    h->set_device();
+    if (op <= FESTUNG_OP_R) h->require_velocity();
+    double* s = scratch_on(h->device);
+    double* dx = s;
+    double* dy = s + kNoAllocValues;
+    HIPCHK(launch_permute(h->K, h->N, x, h->dEperm.i(), dx, h->stream));
+    HIPCHK(launch_apply(h->p, op, h->mesh(), dx, h->u1.d(), h->u2.d(), h->vnw.d(), h->tab.d(), h->off, dy,
+                        h->stream));
+    HIPCHK(launch_permute(h->K, h->N, dy, h->dEinv.i(), y, h->stream));
   });
 }
 

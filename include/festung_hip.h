@@ -306,6 +306,17 @@ enum {
 int festung_apply_operator(festung_problem* h, int op, const double* x, double* y);
 int festung_apply_operator_dev(festung_problem* h, int op, const double* x, double* y);
 
+/* festung_apply_operator_dev without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving or solver loops, harnesses that forbid
+ * hipMalloc).  The operand and result in the device's element order live in
+ * a library-owned 64 MiB device array (one per GPU, held until the process
+ * ends), so K*N must not exceed 4194304 (2^22; FESTUNG_ERR_ARG otherwise).
+ * Enqueued on the problem's stream; returns without waiting.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (handles on
+ * different streams) race; calls on one handle are ordered by its stream. */
+int festung_apply_operator_dev_noalloc(festung_problem* h, int op, const double* x, double* y);
+
 /* The stage vectors of advection/preprocessSubStep.m:87-98 from point
  * values (as festung_set_source_points takes them), each formed alone and
  * returned K x N: L = globM fDisc from fq (core/projectFuncCont2DataDisc.m,
