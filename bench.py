@@ -14,9 +14,13 @@ wall time (DoF-stage updates per second).
 Also reported: the roofline of the dominant kernel (algorithmic bytes per
 launch / average launch time from HIP events on the problem's stream), a
 per-kernel breakdown, and the oracle (the numpy/scipy restatement of the
-reference's sparse-assembly path) timed on a bounded sample on the host.
+reference's sparse-assembly path) timed on one step of the same mesh on the
+host (one thread and all threads).  The default run adds, as extra keys,
+the same step at K = 1 048 352 (hmax 1/724, about 1M triangles), the
+batched workload (512 problems) and the adjoint workload (K = 32 768, 400
+steps, gradient cost and finite-difference agreement).
 
-    python bench.py [--gpus 1] [--steps 100] [--warmup 10]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--no-extras]
 """
 import argparse
 import json
@@ -60,21 +64,50 @@ def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6):
     return tot
 
 
+def host_info():
+    """Logical CPUs this process may run on, os.cpu_count() and the CPU model."""
+    model = None
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(affinity=len(os.sched_getaffinity(0)), nproc=os.cpu_count(), cpu_model=model)
+
+
 def cpu_baseline(hmax, p, ordRK, lim, tc_factory):
     """The oracle (numpy/scipy restatement of the reference's per-stage
-    sparse assembly, SpMV, block solves and limiter) on one thread, one time
-    step on a mesh of the same family; returns DoF-stage updates/s."""
+    sparse assembly, SpMV, block solves and limiter), one time step on the
+    given mesh with the workload's closures, preprocessing excluded; once on
+    one thread and once with the BLAS thread pools at all the process's
+    CPUs (scipy's sparse kernels are single-threaded either way).  Returns
+    the cpu_baseline record (value = the one-thread rate)."""
     from threadpoolctl import threadpool_limits
     from oracle import advection as OA
     tc = tc_factory()
-    cfg = OA.Struct(testcase='custom', p=p, hmax=hmax, ordRK=ordRK, isSlopeLim=lim, numSteps=100)
-    for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):
-        cfg[k] = tc[k]
-    with threadpool_limits(1):
-        pd = OA.main('advection', cfg, maxSteps=1)
     N = (p + 1) * (p + 2) // 2
     S = {1: 1, 2: 2, 3: 3, 4: 4}[ordRK]
-    return pd.K * N * S / pd.timeLoop, pd.K, pd.timeLoop
+    info = host_info()
+    runs = {}
+    for label, nthr in (('1_thread', 1), ('all_threads', info['affinity'])):
+        cfg = OA.Struct(testcase='custom', p=p, hmax=hmax, ordRK=ordRK, isSlopeLim=lim, numSteps=100)
+        for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):
+            cfg[k] = tc[k]
+        with threadpool_limits(nthr):
+            pd = OA.main('advection', cfg, maxSteps=1)
+        runs[label] = dict(value=pd.K * N * S / pd.timeLoop, loop_s=pd.timeLoop, threads=nthr)
+        K = pd.K
+        del pd
+    return dict(value=runs['1_thread']['value'], unit='DoF-stage updates/s', cores=1, kind='port',
+                all_threads=runs['all_threads'], one_thread_loop_s=runs['1_thread']['loop_s'],
+                host=info, sample_K=K, sample_hmax=hmax,
+                sample='oracle (numpy/scipy reference-structured restatement: per-stage sparse kron/kronVec '
+                       'assembly, SpMV, block solves, limiter), 1 time step (%d stages) on domainSquare hmax=%g '
+                       '(K=%d), the workload\'s p / limiter / closures; preprocessing excluded; 1 thread '
+                       '(value) and %d threads (all_threads)' % (S, hmax, K, info['affinity']))
 
 
 def run_batched(nprob, nsteps, seed0=4000):
@@ -116,7 +149,7 @@ def run_batched(nprob, nsteps, seed0=4000):
     return t_batched, t_single, upd, setup, K_tot
 
 
-def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6):
+def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6, variants=True):
     """Gradient of J = ||c(T) - c_obs||^2_M w.r.t. c0 and nmodes stream-function
     coefficients: cost against the faster plain forward run (eager or captured
     graphs), and central differences (step h) along ndir random directions."""
@@ -162,8 +195,8 @@ def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6):
                     'cost_vs_forward': (r['forward_ms'] + r['backward_ms']) / fwd_ms,
                     'reverse_vs_forward': r['backward_ms'] / fwd_ms, 'wall_s': wall}
     # the other sweep variants, same checkpoint interval
-    for serial, kern in ((True, 'fused'), (False, 'fused'), (True, 'split'), (False, 'split'),
-                         (True, 'deferred'), (False, 'deferred')):
+    for serial, kern in (((True, 'fused'), (False, 'fused'), (True, 'split'), (False, 'split'),
+                          (True, 'deferred'), (False, 'deferred')) if variants else ()):
         mf = AD.StreamModel(hmax, p, modes=modes, c0=TC.gaussian_humps(51), adj_serial=serial, adj_kernel=kern)
         mf.gradient(mf.c0, a, 2, tau, cObs, ckpt=1)  # warm-up
         r = mf.gradient(mf.c0, a, nsteps, tau, cObs, ckpt=ckpt)[3]
@@ -174,6 +207,7 @@ def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6):
     # binomial checkpointing with as many state snapshots as the uniform
     # schedule keeps segment checkpoints (nsteps / ckpt + 1), and with a few
     from festung_amd.core import binomial_schedule
+    binom = {}
     for snaps in (nsteps // max(ckpt, 1) + 1, 4):
         mb = AD.StreamModel(hmax, p, modes=modes, c0=TC.gaussian_humps(51), adj_kernel='fused', adj_binomial=True)
         mb.gradient(mb.c0, a, 2, tau, cObs, ckpt=2)  # warm-up
@@ -182,7 +216,25 @@ def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6):
         out['binomial_%d_snapshots' % snaps] = {
             'forward_steps': nsteps + adv, 'adjoint_forward_ms': r['forward_ms'],
             'adjoint_reverse_ms': r['backward_ms'], 'cost_vs_forward': (r['forward_ms'] + r['backward_ms']) / fwd_ms}
+        binom[snaps] = out['binomial_%d_snapshots' % snaps]
         del mb
+    # snapshots for <= 3x: the binomial schedule runs nsteps + advances(c)
+    # forward steps (each step once more, with its stage states, right before
+    # its transposed stages) and the reverse sweep, one after the other, so its
+    # cost is at least 2 + reverse / forward whatever c
+    rev_per_fwd = out['ckpt%d' % ckpt]['reverse_vs_forward'] if ckpt else out['all_states']['reverse_vs_forward']
+    floor = 2.0 + rev_per_fwd
+    best = None
+    for c in range(2, nsteps + 1):
+        adv = binomial_schedule(nsteps, c)[0]
+        if 1.0 + adv / nsteps + rev_per_fwd <= 3.0:
+            best = c
+            break
+    out['binomial'] = {'measured': binom, 'cost_floor_vs_forward': floor,
+                       'snapshots_for_3x': best,
+                       'note': 'predicted cost 1 + advances(c)/nsteps + reverse/forward; the fixed per-step '
+                               'recomputation makes it >= %.2fx for every snapshot count' % floor
+                       if best is None else 'smallest snapshot count whose predicted cost is <= 3x'}
     rng = np.random.default_rng(54)
     errs = []
     for _ in range(ndir):
@@ -195,6 +247,133 @@ def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6):
         errs.append(abs(fd - an) / abs(an))
     out.update(J=J, K=m.prob.K, fd_max_rel_err=max(errs), fd_directions=ndir, fd_h=h, tau=tau)
     return out
+
+
+def run_step(hmax, p, ordRK, lim, testcase, steps, warmup, kernel_steps, copy_mb):
+    """The drop-in step workload on domainSquare(hmax): set-up through the
+    step functions, the autotuner, `steps` timed steps from captured graphs
+    (inputs resident), then a per-kernel timing pass (eager launches with HIP
+    events at each kernel's own start and end) and, if copy_mb > 0, a device
+    copy of copy_mb MiB per buffer for the attainable bandwidth."""
+    import numpy as np
+    import torch
+    from festung_amd.advection import steps as S
+    t_setup = time.perf_counter()
+    pd = S.Struct(testcase=testcase, p=p, hmax=hmax, ordRK=ordRK, isSlopeLim=lim, numSteps=steps, useGraph=True)
+    pd = S.configureProblem(pd)
+    # tau = CFL 0.1 h / (max|u| (2p+1)), max|u| over the vertices
+    g_dim = int(np.ceil(1.0 / hmax))
+    pd.g = pd.generateGridData(pd.hmax)
+    xy = pd.g.coordV
+    umax = float(np.max(np.hypot(pd.u1Cont(0, xy[:, 0], xy[:, 1]), pd.u2Cont(0, xy[:, 0], xy[:, 1]))))
+    tau = 0.1 * (1.0 / g_dim) / (umax * (2 * p + 1))
+    pd.tEnd = tau * steps
+    pd.generateGridData = lambda h: pd.g
+    pd = S.preprocessProblem(pd)
+    pd = S.initializeProblem(pd)
+    pd = S.solveStep(pd, 1)          # first step sets the device velocity
+    prob = pd.problem
+    K, N, numV = prob.K, prob.N, pd.g.numV
+    maxDeg = int(np.max(np.diff(pd.g.v2tPtr)))
+    nstage = {1: 1, 2: 2, 3: 3, 4: 4}[ordRK]
+    # elements per workgroup: the autotuner times 64 / 128 / 256 and keeps the fastest
+    best_bs, tune_ms = prob.autotune(tau, 10)
+    prob.step(warmup, tau)
+    prob.sync()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+
+    # timed region: `steps` steps from the captured graph, events on the problem's stream
+    stream = torch.cuda.ExternalStream(prob.stream)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    prob.step(steps, tau)
+    e1.record(stream)
+    prob.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1)
+    state = prob.get_state()
+    if not np.all(np.isfinite(state)):
+        print('non-finite state after the timed steps', file=sys.stderr)
+        sys.exit(1)
+
+    # per-kernel timing pass (eager launches bracketed by HIP events)
+    prob.profile(True)
+    prob.step(kernel_steps, tau)
+    kt = prob.profile_read()
+    prob.profile(False)
+    kernels = {}
+    for kind, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        per_step_bytes = kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg)
+        launches_per_step = cnt / kernel_steps
+        avg_ms = ms / cnt
+        bytes_per_launch = per_step_bytes / launches_per_step if launches_per_step else 0.0
+        kernels[kind] = dict(launches_per_step=launches_per_step, avg_us=1e3 * avg_ms,
+                             bytes_per_launch=bytes_per_launch,
+                             GBs=bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0,
+                             share=0.0)
+    tot_ms = sum(v['avg_us'] * v['launches_per_step'] for v in kernels.values())
+    for v in kernels.values():
+        v['share'] = v['avg_us'] * v['launches_per_step'] / tot_ms
+    dom = max(kernels, key=lambda k: kernels[k]['share'])
+    step_bytes = sum(kernel_bytes(k, p, K, numV, lim, ordRK, maxDeg) for k in ('stage', 'vertex', 'limit_dot', 'limit'))
+    secs = wall
+    roof = dict(bound='hbm', kernel=dom, achieved=kernels[dom]['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
+                frac=kernels[dom]['GBs'] / HBM_PEAK_GBS, traffic=None,
+                step_GBs=step_bytes / (secs / steps) / 1e9,
+                step_frac=step_bytes / (secs / steps) / 1e9 / HBM_PEAK_GBS)
+    traffic = traffic_from_profile(dom, p, K)
+    if traffic is not None:
+        roof['traffic'] = traffic
+    del pd, prob, state
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+    if copy_mb > 0:
+        # plain device copy of a buffer larger than the 256 MiB Infinity Cache:
+        # the attainable read+write bandwidth on this box, and the reference
+        # kernel for scaling the FETCH_SIZE / WRITE_SIZE counters
+        nb = copy_mb * (1 << 20)
+        src = torch.ones(nb // 8, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        c0 = torch.cuda.Event(enable_timing=True)
+        c1 = torch.cuda.Event(enable_timing=True)
+        c0.record()
+        for _ in range(10):
+            dst.copy_(src)
+        c1.record()
+        torch.cuda.synchronize()
+        copy_GBs = 2 * nb * 10 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+        del src, dst
+        torch.cuda.empty_cache()
+        roof['copy_GBs'] = copy_GBs
+        roof['frac_of_copy'] = kernels[dom]['GBs'] / copy_GBs
+    return dict(K=K, N=N, stages=nstage, tau=tau, value=K * N * nstage * steps / secs,
+                ms_per_step=1e3 * secs / steps, event_ms_per_step=ev_ms / steps, elem_block=best_bs,
+                autotune_ms_per_step=tune_ms, setup_s=t_setup, roofline=roof, kernels=kernels)
+
+
+def traffic_from_profile(kernel, p, K):
+    """HBM bytes per launch of the dominant kernel measured by the counter
+    passes (FETCH_SIZE, WRITE_SIZE scaled by a plain copy) recorded in
+    profiles/<kernel>_p<p>_K<K>_traffic.md on a line 'bytes_per_launch: B'
+    (None if there is no such summary)."""
+    path = os.path.join(ROOT, 'profiles', 'stage_p%d_K%d_traffic.md' % (p, K)) if kernel == 'stage' else None
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith('bytes_per_launch:'):
+                return float(line.split(':', 1)[1])
+    return None
 
 
 def main():
@@ -216,8 +395,10 @@ def main():
                     help='preset: c1 = hmax 2^-5 p=1 SSP-RK2 200 steps no limiter; c2 = the default; '
                          'c3 = hmax 2^-8 p=4 RK4 200 steps limiter, random_discontinuous')
     ap.add_argument('--kernel-steps', type=int, default=5, help='eager steps timed per kernel')
-    ap.add_argument('--cpu-hmax', type=float, default=2.0 ** -8, help='mesh of the CPU sample')
+    ap.add_argument('--cpu-hmax', type=float, default=2.0 ** -9, help='mesh of the CPU sample (default: the workload mesh)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-extras', action='store_true',
+                    help='skip the K = 1 048 352 step, batched and adjoint measurements of the default run')
     ap.add_argument('--copy-mb', type=int, default=1024, help='size of the device copy probe (MiB per buffer)')
     a = ap.parse_args()
     if a.config == 'c1':
@@ -265,116 +446,51 @@ def main():
                                    'SSP-RK min(p+1,3), %d steps, limiter off' % (a.nprob, Ktot, nsteps)},
             'batched_s': tb, 'one_launch_per_problem_s': ts, 'speedup': ts / tb, 'setup_s': setup}))
         return
-    from festung_amd.advection import steps as S
     from festung_amd.advection import testcases as TC
-
     lim = not a.no_limiter
     p, ordRK = a.p, a.ordRK
-    t_setup = time.perf_counter()
-    pd = S.Struct(testcase=a.testcase, p=p, hmax=a.hmax, ordRK=ordRK, isSlopeLim=lim,
-                  numSteps=a.steps, useGraph=True)
-    pd = S.configureProblem(pd)
-    # tau = CFL 0.1 h / (max|u| (2p+1)), max|u| over the vertices
-    g_dim = int(np.ceil(1.0 / a.hmax))
-    pd.g = pd.generateGridData(pd.hmax)
-    xy = pd.g.coordV
-    umax = float(np.max(np.hypot(pd.u1Cont(0, xy[:, 0], xy[:, 1]), pd.u2Cont(0, xy[:, 0], xy[:, 1]))))
-    tau = 0.1 * (1.0 / g_dim) / (umax * (2 * p + 1))
-    pd.tEnd = tau * a.steps
-    pd.generateGridData = lambda h: pd.g
-    pd = S.preprocessProblem(pd)
-    pd = S.initializeProblem(pd)
-    pd = S.solveStep(pd, 1)          # first step sets the device velocity
-    prob = pd.problem
-    K, N, numV = prob.K, prob.N, pd.g.numV
-    maxDeg = int(np.max(np.diff(pd.g.v2tPtr)))
-    nstage = {1: 1, 2: 2, 3: 3, 4: 4}[ordRK]
-    # elements per workgroup: the autotuner times 64 / 128 / 256 and keeps the fastest
-    best_bs, tune_ms = prob.autotune(tau, 10)
-    prob.step(a.warmup, tau)
-    prob.sync()
-    torch.cuda.synchronize()
-    t_setup = time.perf_counter() - t_setup
+    r = run_step(a.hmax, p, ordRK, lim, a.testcase, a.steps, a.warmup, a.kernel_steps, a.copy_mb)
+    K, N, nstage = r['K'], r['N'], r['stages']
+    log = lambda m: print('[bench] %s' % m, file=sys.stderr, flush=True)
+    log('headline: %.4f ms/step, stage kernel %.1f us (%.3f of HBM peak)'
+        % (r['ms_per_step'], r['kernels'][r['roofline']['kernel']]['avg_us'], r['roofline']['frac']))
 
-    # timed region: K steps from the captured graph, events on the problem's stream
-    stream = torch.cuda.ExternalStream(prob.stream)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    prob.step(a.steps, tau)
-    e1.record(stream)
-    prob.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ev_ms = e0.elapsed_time(e1)
-    state = prob.get_state()
-    if not np.all(np.isfinite(state)):
-        print('non-finite state after the timed steps', file=sys.stderr)
-        sys.exit(1)
-
-    # per-kernel timing pass (eager launches bracketed by HIP events)
-    prob.profile(True)
-    prob.step(a.kernel_steps, tau)
-    kt = prob.profile_read()
-    prob.profile(False)
-    kernels = {}
-    for kind, (ms, cnt) in kt.items():
-        if cnt == 0:
-            continue
-        per_step_bytes = kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg)
-        launches_per_step = cnt / a.kernel_steps
-        avg_ms = ms / cnt
-        bytes_per_launch = per_step_bytes / launches_per_step if launches_per_step else 0.0
-        kernels[kind] = dict(launches_per_step=launches_per_step, avg_us=1e3 * avg_ms,
-                             bytes_per_launch=bytes_per_launch,
-                             GBs=bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0,
-                             share=0.0)
-    tot_ms = sum(v['avg_us'] * v['launches_per_step'] for v in kernels.values())
-    for v in kernels.values():
-        v['share'] = v['avg_us'] * v['launches_per_step'] / tot_ms
-    dom = max(kernels, key=lambda k: kernels[k]['share'])
-    step_bytes = sum(kernel_bytes(k, p, K, numV, lim, ordRK, maxDeg) for k in ('stage', 'vertex', 'limit_dot', 'limit'))
-    secs = wall
-    value = K * N * nstage * a.steps / secs
-    roof = dict(bound='hbm', kernel=dom, achieved=kernels[dom]['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
-                frac=kernels[dom]['GBs'] / HBM_PEAK_GBS, traffic=None,
-                step_GBs=step_bytes / (secs / a.steps) / 1e9,
-                step_frac=step_bytes / (secs / a.steps) / 1e9 / HBM_PEAK_GBS)
-
-    # plain device copy of a buffer larger than the 256 MiB Infinity Cache:
-    # the attainable read+write bandwidth on this box, and the reference
-    # kernel for scaling the FETCH_SIZE / WRITE_SIZE counters
-    nb = a.copy_mb * (1 << 20)
-    src = torch.ones(nb // 8, dtype=torch.float64, device='cuda')
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    c0 = torch.cuda.Event(enable_timing=True)
-    c1 = torch.cuda.Event(enable_timing=True)
-    c0.record()
-    for _ in range(10):
-        dst.copy_(src)
-    c1.record()
-    torch.cuda.synchronize()
-    copy_GBs = 2 * nb * 10 / (c0.elapsed_time(c1) * 1e-3) / 1e9
-    del src, dst
-    roof['copy_GBs'] = copy_GBs
-    roof['frac_of_copy'] = kernels[dom]['GBs'] / copy_GBs
+    extra = {}
+    if not a.no_extras and a.config is None and a.hmax == 2.0 ** -9 and p == 2:
+        # the same p = 2 step at about 1M triangles (hmax 1/724: K = 1 048 352),
+        # outside the 256 MB Infinity Cache
+        r1 = run_step(1.0 / 724, p, ordRK, lim, a.testcase, 40, 5, a.kernel_steps, 0)
+        extra['step_K1048352'] = {k: r1[k] for k in ('K', 'ms_per_step', 'value', 'elem_block')}
+        extra['step_K1048352'].update(
+            step_GBs=r1['roofline']['step_GBs'], step_frac=r1['roofline']['step_frac'],
+            stage_GBs=r1['kernels']['stage']['GBs'], stage_frac=r1['kernels']['stage']['GBs'] / HBM_PEAK_GBS,
+            stage_avg_us=r1['kernels']['stage']['avg_us'],
+            kernels_us={k: v['avg_us'] for k, v in r1['kernels'].items()})
+        log('K=1048352: %.4f ms/step, step %.3f / stage %.3f of HBM peak'
+            % (r1['ms_per_step'], r1['roofline']['step_frac'], extra['step_K1048352']['stage_frac']))
+        del r1
+        # the batched workload (512 problems, one graph launch vs one launch per problem)
+        tb, ts, upd, setup, Ktot = run_batched(a.nprob, 50)
+        extra['batched'] = dict(problems=a.nprob, steps=50, K_total=Ktot, batched_s=tb, one_launch_per_problem_s=ts,
+                                speedup=ts / tb, dof_stage_updates_per_s=upd / tb, setup_s=setup)
+        log('batched: %.2fx (%.3f s vs %.3f s)' % (ts / tb, tb, ts))
+        # the adjoint workload (K = 32 768, 400 SSP-RK3 steps, checkpoints every 20)
+        ra = run_adjoint(2.0 ** -7, 2, 400, 20, variants=False)
+        extra['adjoint'] = dict(K=ra['K'], steps=400, ckpt=20, forward_ms=ra['forward_ms'],
+                                cost_vs_forward=ra['ckpt20']['cost_vs_forward'],
+                                cost_vs_forward_all_states=ra['all_states']['cost_vs_forward'],
+                                fd_max_rel_err=ra['fd_max_rel_err'], fd_directions=ra['fd_directions'],
+                                fd_h=ra['fd_h'], binomial=ra['binomial'])
+        log('adjoint: %.2fx forward, FD max rel err %.2e' % (ra['ckpt20']['cost_vs_forward'], ra['fd_max_rel_err']))
 
     cpu = None
     if not a.no_cpu_baseline:
-        v, Kc, tl = cpu_baseline(a.cpu_hmax, p, ordRK, lim, TC.TESTCASES[a.testcase])
-        cpu = dict(value=v, unit='DoF-stage updates/s', cores=1, kind='port',
-                   sample='oracle (numpy/scipy reference-structured restatement: per-stage sparse '
-                          'kron/kronVec assembly, SpMV, block solves, limiter), 1 thread, 1 time step '
-                          '(%d stages) on domainSquare hmax=%g (K=%d), same p/limiter/closures; loop '
-                          'time %.1f s, preprocessing excluded' % (nstage, a.cpu_hmax, Kc, tl))
+        log('CPU baseline (oracle) on the host ...')
+        cpu = cpu_baseline(a.cpu_hmax, p, ordRK, lim, TC.TESTCASES[a.testcase])
 
     out = {
-        'metric': METRIC, 'value': value, 'unit': 'DoF-stage updates/s', 'n_gpus': 1,
-        'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': 1e3 * secs / a.steps,
+        'metric': METRIC, 'value': r['value'], 'unit': 'DoF-stage updates/s', 'n_gpus': 1,
+        'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': r['ms_per_step'],
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
         'data': 'synthetic',
         'config': {'workload': 'advection drop-in step: domainSquare hmax=2^%g K=%d p=%d N=%d %s '
@@ -384,14 +500,15 @@ def main():
                                   'rotation+16-mode stream function, 8 Gaussian humps'
                                   if a.testcase == 'rotating_humps' else
                                   '32-mode stream function (unit max speed), disc/square indicators'),
-                   'K': K, 'p': p, 'N': N, 'ordRK': ordRK, 'limiter': lim, 'tau': tau,
-                   'stages_per_step': nstage, 'event_ms_per_step': ev_ms / a.steps,
-                   'elem_block': best_bs, 'autotune_ms_per_step': tune_ms,
-                   'setup_s': t_setup},
-        'roofline': roof,
-        'kernels': kernels,
+                   'K': K, 'p': p, 'N': N, 'ordRK': ordRK, 'limiter': lim, 'tau': r['tau'],
+                   'stages_per_step': nstage, 'event_ms_per_step': r['event_ms_per_step'],
+                   'elem_block': r['elem_block'], 'autotune_ms_per_step': r['autotune_ms_per_step'],
+                   'setup_s': r['setup_s']},
+        'roofline': r['roofline'],
+        'kernels': r['kernels'],
         'cpu_baseline': cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

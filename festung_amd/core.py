@@ -192,9 +192,12 @@ class Problem:
         return s.value
 
     def set_velocity_disc(self, u1Disc, u2Disc, vNormalOnQuadEdge):
+        """u1Disc, u2Disc (K x N), vNormalOnQuadEdge (K x 3 x Re); None keeps
+        that field."""
         K, N = self.K, self.N
-        a = fortran(u1Disc, (K, N)); b = fortran(u2Disc, (K, N))
-        v = fortran(vNormalOnQuadEdge, (K, 3, self.Re))
+        a = None if u1Disc is None else fortran(u1Disc, (K, N))
+        b = None if u2Disc is None else fortran(u2Disc, (K, N))
+        v = None if vNormalOnQuadEdge is None else fortran(vNormalOnQuadEdge, (K, 3, self.Re))
         check(lib().festung_set_velocity_disc(self._h, ptr(a), ptr(b), ptr(v)))
 
     def set_velocity_points(self, u1q, u2q, u1e, u2e):

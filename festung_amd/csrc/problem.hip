@@ -146,6 +146,7 @@ struct festung_problem {
   int limType = LT_LINEAR;  // limiter type (core/applySlopeLimiterTaylor.m:69-85)
   int nF = 1;               // Taylor coefficients with vertex-patch bounds (limiter_fields)
   bool hasRhs = false, hasVertexD = false, hasVelocity = false;
+  int velocityParts = 0;  // fields set by festung_set_velocity_disc: 1 u1Disc, 2 u2Disc, 4 vNormal
   int cur = 0;           // buffer holding the current state (0 = X)
   int nextStage = -1;    // next sub-step of a step begun with festung_begin_step
   // captured step sequences: graph[0] one step, graph[1] kGraphSteps steps
@@ -637,13 +638,16 @@ int festung_problem_stream(const festung_problem* h, void** stream) {
 int festung_set_velocity_disc(festung_problem* h, const double* u1Disc, const double* u2Disc, const double* vNormal) {
   return guard([&] {
     check_handle(h);
-    if (!u1Disc || !u2Disc || !vNormal) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    if (!u1Disc && !u2Disc && !vNormal) fail(FESTUNG_ERR_ARG, "null pointer argument");
     h->set_device();
-    h->upload_elem(h->u1, u1Disc, h->N, false);
-    h->upload_elem(h->u2, u2Disc, h->N, false);
-    h->upload_elem(h->vn, vNormal, 3 * h->tabs.Re, false);
-    h->refresh_vnw();
-    h->hasVelocity = true;
+    if (u1Disc) h->upload_elem(h->u1, u1Disc, h->N, false);
+    if (u2Disc) h->upload_elem(h->u2, u2Disc, h->N, false);
+    if (vNormal) {
+      h->upload_elem(h->vn, vNormal, 3 * h->tabs.Re, false);
+      h->refresh_vnw();
+    }
+    h->velocityParts |= (u1Disc ? 1 : 0) | (u2Disc ? 2 : 0) | (vNormal ? 4 : 0);
+    h->hasVelocity = h->velocityParts == 7;
   });
 }
 
@@ -665,6 +669,7 @@ int festung_set_velocity_points(festung_problem* h, const double* u1q, const dou
     HIPCHK(launch_vnormal(K, h->tabs.Re, h->nuE0T.d(), e1.d(), e2.d(), h->vn.d(), h->stream));
     h->refresh_vnw();
     sync(h);
+    h->velocityParts = 7;
     h->hasVelocity = true;
   });
 }
@@ -688,6 +693,7 @@ int festung_set_velocity_points_dev(festung_problem* h, const double* u1q, const
     HIPCHK(launch_permute(K, Re3, u2e, h->dEperm.i(), h->devE2.d(), h->stream));
     HIPCHK(launch_vnormal(K, h->tabs.Re, h->nuE0T.d(), h->devE1.d(), h->devE2.d(), h->vn.d(), h->stream));
     h->refresh_vnw();
+    h->velocityParts = 7;
     h->hasVelocity = true;
   });
 }

@@ -23,7 +23,7 @@
 //   h = festung_mex('create', g, p, ordRK, flags)            advection/preprocessProblem.m
 //   [K, N, RE, RF, Re] = festung_mex('sizes', h)
 //   festung_mex('destroy', h)
-//   festung_mex('velocity', h, u1Disc, u2Disc, vNormalOnQuadEdge)
+//   festung_mex('velocity', h, u1Disc, u2Disc, vNormalOnQuadEdge)   ([] keeps a field)
 //   festung_mex('velocity_points', h, u1q, u2q, u1e, u2e)
 //   festung_mex('source', h, fq, cDe, gNe)                   ([] for zero data)
 //   festung_mex('vertex_dirichlet', h, vD)                   ([] clears)
@@ -365,8 +365,8 @@ void dispatch(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
       for (int i = 0; i < 5 && i < (nlhs > 0 ? nlhs : 1); ++i) plhs[i] = mxCreateDoubleScalar(v[i]);
     } else if (cmd == "velocity") {
       need_args(nrhs, 5, "festung_mex('velocity', h, u1Disc, u2Disc, vNormalOnQuadEdge)");
-      check(festung_set_velocity_disc(h, doubles(prhs[2], KN, "u1Disc"), doubles(prhs[3], KN, "u2Disc"),
-                                      doubles(prhs[4], KE, "vNormalOnQuadEdge")));
+      check(festung_set_velocity_disc(h, doubles(prhs[2], KN, "u1Disc", true), doubles(prhs[3], KN, "u2Disc", true),
+                                      doubles(prhs[4], KE, "vNormalOnQuadEdge", true)));
     } else if (cmd == "velocity_points") {
       need_args(nrhs, 6, "festung_mex('velocity_points', h, u1q, u2q, u1e, u2e)");
       check(festung_set_velocity_points(h, doubles(prhs[2], KQ, "u1q"), doubles(prhs[3], KQ, "u2q"),

@@ -152,7 +152,9 @@ int festung_problem_stream(const festung_problem* h, void** stream);
 
 /* Velocity as DG coefficients u1Disc, u2Disc (K x N) and normal velocity at
  * the edge quadrature points vNormalOnQuadEdge (K x 3 x Re), as computed in
- * advection/preprocessSubStep.m:71-79.  Host arrays. */
+ * advection/preprocessSubStep.m:71-79.  Host arrays.  A NULL field keeps
+ * its value (so the element and edge assembly calls can each hand over
+ * theirs); steps need all three set once. */
 int festung_set_velocity_disc(festung_problem* h, const double* u1Disc, const double* u2Disc,
                               const double* vNormalOnQuadEdge);
 /* Velocity from point values: u1q, u2q at the element points of

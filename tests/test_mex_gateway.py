@@ -242,3 +242,37 @@ def test_device_commands_match_binding(gpu, gw):
     gw.call(0, 'destroy', h)
     gw.call(0, 'grid_destroy', g)
     assert gw.L.standin_lock_count() == 0
+
+
+@pytest.mark.gpu
+def test_operator_objects_run_solve_substep_unchanged(gpu, gw):
+    """What the Matlab operator objects do for advection/solveSubStep.m:62-66
+    (sysA = -globG{1} - globG{2} + globR; cDiscDot = globM \\ (sysV - sysA *
+    cDiscRK{s})): the element and edge assembly shims hand their velocity
+    fields to the device separately ('velocity' with [] for the other
+    fields), every product is a gateway 'apply' on the element-major KN x 1
+    vector, and sysV comes from the reference's own vector assembly (the
+    oracle here).  Against the oracle's sparse path, 1e-12 relative."""
+    from helpers import oracle_problem, oracle_stage_operators, rel_l2
+    from festung_amd.advection import testcases as TC
+    from oracle import advection as OA
+    tc = dict(TC.rotating_humps())
+    tc['cDCont'] = lambda t, x1, x2: 0.2 + 0.1 * x1 + 0 * t
+    for p in (1, 2, 4):
+        o = oracle_problem(tc, p, 1 / 8, min(p + 1, 3), False)
+        o = oracle_stage_operators(o, 0.0)
+        (gh,) = gw.call(1, 'grid_square', 1 / 8)
+        g = gw.keep('oo_g%d' % p, gh)
+        (hh,) = gw.call(1, 'create', g, float(p), float(min(p + 1, 3)), 0.0)
+        h = gw.keep('oo_h%d' % p, hh)
+        gw.call(0, 'velocity', h, o.u1Disc, o.u2Disc, np.empty((0, 0)))     # assembleMatElemDphiPhiFuncDisc
+        gw.call(0, 'velocity', h, np.empty((0, 0)), np.empty((0, 0)), o.vNormalOnQuadEdge)  # ...EdgePhiPhiValUpwind
+        c = np.random.default_rng(60 + p).standard_normal(o.K * o.N)[:, None]
+        sysV = (o.globL - o.globKD - o.globKN)[:, None]
+        G1c, G2c, Rc = (gw.call(1, 'apply', h, op, c)[0] for op in ('G1', 'G2', 'R'))
+        sysAc = -G1c - G2c + Rc
+        (cdot,) = gw.call(1, 'apply', h, 'Minv', sysV - sysAc)
+        ref = OA.computeDot(o, c.ravel())
+        assert rel_l2(cdot.ravel(), ref) < 1e-12, p
+        gw.call(0, 'destroy', h)
+        gw.call(0, 'grid_destroy', g)
