@@ -364,9 +364,9 @@ def run_step(hmax, p, ordRK, lim, testcase, steps, warmup, kernel_steps, copy_mb
 def traffic_from_profile(kernel, p, K):
     """HBM bytes per launch of the dominant kernel measured by the counter
     passes (FETCH_SIZE, WRITE_SIZE scaled by a plain copy) recorded in
-    profiles/<kernel>_p<p>_K<K>_traffic.md on a line 'bytes_per_launch: B'
+    profiles/<kernel>_p<p>_K<K>_traffic.txt on a line 'bytes_per_launch: B'
     (None if there is no such summary)."""
-    path = os.path.join(ROOT, 'profiles', 'stage_p%d_K%d_traffic.md' % (p, K)) if kernel == 'stage' else None
+    path = os.path.join(ROOT, 'profiles', 'stage_p%d_K%d_traffic.txt' % (p, K)) if kernel == 'stage' else None
     if not path or not os.path.exists(path):
         return None
     with open(path) as f:

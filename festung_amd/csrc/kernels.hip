@@ -448,6 +448,28 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   Geo<P> g;
   g.load(a.m.B, kk, K);
 
+  // Neighbour ids are requested up front.  At p <= 2 the neighbours'
+  // coefficients are gathered one edge ahead (edge 0's while the element
+  // term is formed, edge n + 1's while edge n is), so the dependent gathers
+  // are in flight behind arithmetic; at p >= 3 the registers are not there
+  // and each edge gathers its neighbour when it starts.  A missing neighbour
+  // reads the element's own coefficients (in bounds; the select below
+  // discards them).
+  // (p = 3 keeps the gathers, and the id loads, per edge: measured faster)
+  constexpr bool kAhead = P <= 2;
+  constexpr bool kSelect = P != 3;
+  const int fl = kSelect ? a.m.flags[kk] : 0;
+  int nbs[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) nbs[n] = kSelect ? a.m.nbr[n * K + kk] : -1;
+  double cnb[kAhead ? 2 : 1][N];
+  auto gather = [&](int n, double (&dst)[N]) {
+    const int nbk = nbs[n] >= 0 ? nbs[n] : kk;
+#pragma unroll
+    for (int j = 0; j < N; ++j) dst[j] = a.c[j * K + nbk];
+  };
+  if constexpr (kAhead) gather(0, cnb[0]);
+
   if constexpr (!MF && P >= 1) {
     constexpr int UQ = RE <= 6 ? RE : 1;  // p >= 3: not unrolled (measured fastest at p = 3, 4)
 #pragma unroll UQ
@@ -468,33 +490,86 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
     }
   }
 
-  // edge term  -R c  (upwind; interior neighbours gathered, no atomics)
-  const int fl = a.m.flags[kk];
-#pragma unroll 1
-  for (int n = 0; n < 3; ++n) {
-    const int nb = a.m.nbr[n * K + kk];
+  // edge term  -R c  (upwind; interior neighbours gathered, no atomics).
+  // Except at p = 3, both traces are formed at every point and the upwind
+  // one selected: the upwind side differs from lane to lane, and branching on
+  // it made every wave run both sides anyway, with the branch overhead on top
+  // (p = 1, 2, 4: 20.3 -> 16.4, 37.3 -> 33.0, 49.1 -> 43.7 us per launch at
+  // the benchmark sizes); at p = 3 the branches are faster (24.0 against
+  // 26.0 us).  The same trace is selected either way: the same bits.
+  auto edge = [&](int n, const double (&cn)[N]) {
     const int np = flag_np(fl, n);
     const bool outOn = flag_out(fl, n);
-    double cn[N];
-    if (nb >= 0) {
-#pragma unroll
-      for (int j = 0; j < N; ++j) cn[j] = a.c[j * K + nb];
-    }
+    const bool inOn = nbs[n] >= 0;
 #pragma unroll
     for (int r = 0; r < Re; ++r) {
       const double v = a.vnw[(r * 3 + n) * K + kk];  // |E| w_r u.nu: the sign of u.nu
+      const double* th = sTheta + r + n * Re * N + np * Re * N * 3;
       double tr = 0.0;
-      if (v > 0.0 && outOn) {
+      if constexpr (kSelect) {
+        double to = 0.0, tn = 0.0;
 #pragma unroll
-        for (int j = 0; j < N; ++j) tr += c[j] * T[a.off.edgeQ + QIdx<P>::ephi(n, r, j)];
-      } else if (v < 0.0 && nb >= 0) {
-        const double* th = sTheta + r + n * Re * N + np * Re * N * 3;
+        for (int j = 0; j < N; ++j) to += c[j] * T[a.off.edgeQ + QIdx<P>::ephi(n, r, j)];
 #pragma unroll
-        for (int j = 0; j < N; ++j) tr += cn[j] * th[j * Re];
+        for (int j = 0; j < N; ++j) tn += cn[j] * th[j * Re];
+        tr = (v > 0.0 && outOn) ? to : ((v < 0.0 && inOn) ? tn : 0.0);
+      } else {
+        if (v > 0.0 && outOn) {
+#pragma unroll
+          for (int j = 0; j < N; ++j) tr += c[j] * T[a.off.edgeQ + QIdx<P>::ephi(n, r, j)];
+        } else if (v < 0.0 && inOn) {
+#pragma unroll
+          for (int j = 0; j < N; ++j) tr += cn[j] * th[j * Re];
+        }
       }
       const double fx = v * tr;
 #pragma unroll
       for (int i = 0; i < N; ++i) acc[i] -= T[a.off.edgeQ + QIdx<P>::ephi(n, r, i)] * fx;
+    }
+  };
+  if constexpr (kAhead) {
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      if (n < 2) gather(n + 1, cnb[(n + 1) & 1]);
+      edge(n, cnb[n & 1]);
+    }
+  } else if constexpr (kSelect) {
+#pragma unroll 1
+    for (int n = 0; n < 3; ++n) {
+      gather(n, cnb[0]);
+      edge(n, cnb[0]);
+    }
+  } else {
+    // p = 3: the branches, the per-element flag and per-edge id loads here
+    // and only real neighbours gathered, written out (this form schedules
+    // best at p = 3)
+    const int fl3 = a.m.flags[kk];
+#pragma unroll 1
+    for (int n = 0; n < 3; ++n) {
+      const int nb = a.m.nbr[n * K + kk];
+      const int np = flag_np(fl3, n);
+      const bool outOn = flag_out(fl3, n);
+      double cn[N];
+      if (nb >= 0) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) cn[j] = a.c[j * K + nb];
+      }
+#pragma unroll
+      for (int r = 0; r < Re; ++r) {
+        const double v = a.vnw[(r * 3 + n) * K + kk];
+        double tr = 0.0;
+        if (v > 0.0 && outOn) {
+#pragma unroll
+          for (int j = 0; j < N; ++j) tr += c[j] * T[a.off.edgeQ + QIdx<P>::ephi(n, r, j)];
+        } else if (v < 0.0 && nb >= 0) {
+          const double* th = sTheta + r + n * Re * N + np * Re * N * 3;
+#pragma unroll
+          for (int j = 0; j < N; ++j) tr += cn[j] * th[j * Re];
+        }
+        const double fx = v * tr;
+#pragma unroll
+        for (int i = 0; i < N; ++i) acc[i] -= T[a.off.edgeQ + QIdx<P>::ephi(n, r, i)] * fx;
+      }
     }
   }
   if constexpr (MF) {
