@@ -25,9 +25,15 @@ HOST_SRCS = ['tables.cpp', 'grid.cpp', 'checkpoint.cpp', 'capi.cpp']
 HIP_SRCS = ['kernels.hip', 'problem.hip']
 
 CXXFLAGS = ['-O2', '-std=c++17', '-fPIC', '-Wall', '-Wno-unused-function', '-I' + INCLUDE, '-I' + CSRC]
+# -amdgpu-mfma-vgpr-form: the matrix instructions read and write ordinary
+# VGPRs instead of accumulation registers.  The p = 4 stage kernel then has no
+# v_accvgpr_read / write moves between its MFMAs and the pointwise products
+# (56 per point block; a gfx950 SIMD issues no VALU instruction while an FP64
+# MFMA runs, so every such move adds to the launch).
 HIPFLAGS = ['-O3', '-std=c++17', '-fPIC', '--offload-arch=' + ARCH, '-I' + INCLUDE, '-I' + CSRC,
             '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
-            '-munsafe-fp-atomics', '-Wno-unused-command-line-argument']
+            '-munsafe-fp-atomics', '-Wno-unused-command-line-argument',
+            '-mllvm', '-amdgpu-mfma-vgpr-form=1']
 
 
 def _run(cmd):

@@ -364,19 +364,36 @@ def test_time_dependent_data_on_device(gpu, lim):
 
 def test_main_writes_visualisation(gpu, tmp_path):
     """isVisSol: the initial, every outputFrequency-th and final frames, in
-    both formats, holding the device state's Lagrange values."""
+    both formats.  The written values are the oracle's: its state after the
+    same steps, taken to the Lagrange points by its own
+    projectDataDisc2DataLagr (core/projectDataDisc2DataLagr.m, pinned on the
+    CPU against point evaluation), in the writers' node order
+    (core/visualizeDataLagr.m:125-130, 214-219).  Values are written with ten
+    significant digits, hence rtol 1e-9."""
     import festung_amd
-    from festung_amd import output as O
+    from oracle import limiter as OL
     base = str(tmp_path / 'vis' / 'adv')
-    d = festung_amd.main('advection', dict(testcase='solid_body', p=2, hmax=1 / 8, numSteps=4, isVisSol=True,
-                                          outputFrequency=2, outputBasename=base, outputTypes=['vtk', 'tec']))
+    kw = dict(testcase='solid_body', p=2, hmax=1 / 8, numSteps=4)
+    festung_amd.main('advection', dict(kw, isVisSol=True, outputFrequency=2, outputBasename=base,
+                                       outputTypes=['vtk', 'tec']))
     names = sorted(p.name for p in (tmp_path / 'vis').iterdir())
     assert names == ['adv.%d.%s' % (n, e) for n in (0, 2, 4) for e in ('plt', 'vtu')]
+    o = OA.main('advection', dict(kw))
+    want = OL.projectDataDisc2DataLagr(o.cDisc)[:, [0, 1, 2, 5, 3, 4]].ravel()
     import xml.etree.ElementTree as ET
     piece = ET.parse(base + '.4.vtu').getroot().find('UnstructuredGrid/Piece')
     u = np.array([a for a in piece.iter('DataArray') if a.get('Name') == 'u_h'][0].text.split(), dtype=float)
-    want = O.projectDataDisc2DataLagr(d.cDisc)[:, [0, 1, 2, 5, 3, 4]].ravel()
-    assert np.allclose(u, want, rtol=1e-8, atol=1e-12)
+    assert np.allclose(u, want, rtol=1e-9, atol=1e-13)
+    pts = np.array(piece.find('Points/DataArray').text.split(), dtype=float).reshape(-1, 3)
+    V = o.g.coordV[o.g.V0T]                                   # K x 3 x 2 vertices
+    mid = 0.5 * (V[:, [0, 1, 2]] + V[:, [1, 2, 0]])           # edge midpoints, edges 1-2, 2-3, 3-1
+    assert np.allclose(pts[0::6, :2], V[:, 0], atol=1e-12) and np.allclose(pts[1::6, :2], V[:, 1], atol=1e-12)
+    assert np.allclose(pts[3::6, :2], mid[:, 0], atol=1e-12) and np.allclose(pts[5::6, :2], mid[:, 2], atol=1e-12)
+    with open(base + '.4.plt') as f:
+        body = f.read().split('\n\n')
+    nums = np.array(' '.join(body[1:]).split(), dtype=float)
+    nP = 6 * o.K
+    assert np.allclose(nums[2 * nP:3 * nP], want, rtol=1e-9, atol=1e-13)
 
 
 def _delaunay(n, seed):
