@@ -85,6 +85,10 @@ SIGNATURES = {
     'festung_set_elem_block': (C.c_int, [vp, C.c_int32]),
     'festung_get_elem_block': (C.c_int, [vp, C.POINTER(C.c_int32)]),
     'festung_autotune': (C.c_int, [vp, C.c_double, C.c_int32, C.POINTER(C.c_int32), dp]),
+    'festung_tune_save': (C.c_int, [C.c_char_p]),
+    'festung_tune_load': (C.c_int, [C.c_char_p]),
+    'festung_tune_clear': (C.c_int, []),
+    'festung_tune_count': (C.c_int, [C.POINTER(C.c_int32)]),
     'festung_batch_create': (C.c_int, [C.POINTER(vp), C.c_int32, C.POINTER(vp)]),
     'festung_batch_step': (C.c_int, [vp, C.c_int32, dp]),
     'festung_batch_sync': (C.c_int, [vp]),

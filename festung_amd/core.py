@@ -7,6 +7,7 @@ exchanged in column-major (Fortran) order, which is the memory layout the
 C-ABI expects, so no transposes are made.  Ids are 0-based.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -269,8 +270,11 @@ class Problem:
 
     def autotune(self, tau, nsteps=10):
         """Time nsteps steps with 64, 128 and 256 elements per workgroup,
-        keep the fastest; the state is left as it was.  Returns (best,
-        {elements per workgroup: ms per step})."""
+        keep the fastest; the state is left as it was.  The choice is
+        recorded in the process's tuning table (problems created later with
+        the same device, p, ordRK, limiter and ceil(log2 K) start with it;
+        tune_save / tune_load or FESTUNG_TUNE_FILE carry it to other
+        processes).  Returns (best, {elements per workgroup: ms per step})."""
         best = C.c_int32()
         ms = np.zeros(3)
         check(lib().festung_autotune(self._h, float(tau), int(nsteps), C.byref(best), ptr(ms)))
@@ -482,6 +486,26 @@ def rungeKuttaExplicit(ord_, tau, t0):
     t = np.zeros(4); w = np.zeros(4); n = C.c_int32()
     check(lib().festung_runge_kutta_explicit(int(ord_), float(tau), float(t0), ptr(t), ptr(w), C.byref(n)))
     return t[:n.value].copy(), w[:n.value].copy()
+
+
+def tune_save(path):
+    """Write the autotuner's table (festung_tune_save) to a text file."""
+    check(lib().festung_tune_save(os.fsencode(path)))
+
+
+def tune_load(path):
+    """Merge a tuning file written by tune_save (festung_tune_load)."""
+    check(lib().festung_tune_load(os.fsencode(path)))
+
+
+def tune_clear():
+    check(lib().festung_tune_clear())
+
+
+def tune_count():
+    n = C.c_int32()
+    check(lib().festung_tune_count(C.byref(n)))
+    return n.value
 
 
 def binomial_schedule(nsteps, snapshots):

@@ -1,8 +1,14 @@
 // Host-only part of the C-ABI: error reporting, grids and reference tables.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 
 #include "checkpoint.hpp"
 #include "common.hpp"
@@ -13,6 +19,65 @@ namespace festung {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& s) { g_last_error = s; }
 const char* last_error() { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------- tuning --
+namespace {
+using TuneTuple = std::tuple<std::string, int, int, int, int>;
+TuneTuple tuple_of(const TuneKey& k) { return TuneTuple(k.device, k.p, k.ordRK, k.lim, k.lgK); }
+std::mutex g_tuneMu;
+std::map<TuneTuple, int> g_tune;
+
+bool valid_key(int p, int ordRK, int lim, int lgK, int bs) {
+  return p >= 0 && p <= 4 && ordRK >= 1 && ordRK <= 4 && lim >= -1 && lim <= 2 && lgK >= 0 && lgK <= 40 &&
+         (bs == 64 || bs == 128 || bs == 256);
+}
+
+// Parse a tuning file into `out` (all or nothing).
+void parse_tune_file(const char* path, std::map<TuneTuple, int>& out) {
+  std::ifstream f(path);
+  if (!f) fail(FESTUNG_ERR_ARG, "cannot open tuning file %s", path);
+  std::string line;
+  int no = 0;
+  while (std::getline(f, line)) {
+    ++no;
+    const size_t h = line.find('#');
+    if (h != std::string::npos) line.resize(h);
+    std::istringstream is(line);
+    std::string dev;
+    if (!(is >> dev)) continue;  // blank or comment
+    int p, ordRK, lim, lgK, bs;
+    std::string extra;
+    if (!(is >> p >> ordRK >> lim >> lgK >> bs) || (is >> extra) || !valid_key(p, ordRK, lim, lgK, bs))
+      fail(FESTUNG_ERR_ARG, "%s:%d: expected 'device p ordRK limiter log2K elemsPerBlock'", path, no);
+    out[TuneTuple(dev, p, ordRK, lim, lgK)] = bs;
+  }
+}
+}  // namespace
+
+int tune_lookup(const TuneKey& k) {
+  std::lock_guard<std::mutex> lk(g_tuneMu);
+  const auto it = g_tune.find(tuple_of(k));
+  return it == g_tune.end() ? 0 : it->second;
+}
+
+void tune_record(const TuneKey& k, int bs) {
+  std::lock_guard<std::mutex> lk(g_tuneMu);
+  g_tune[tuple_of(k)] = bs;
+}
+
+void tune_load_env() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const char* path = std::getenv("FESTUNG_TUNE_FILE");
+    if (!path || !*path) return;
+    std::ifstream probe(path);
+    if (!probe) return;  // not written yet: nothing tuned
+    std::map<TuneTuple, int> m;
+    parse_tune_file(path, m);
+    std::lock_guard<std::mutex> lk(g_tuneMu);
+    for (const auto& e : m) g_tune[e.first] = e.second;
+  });
+}
 }  // namespace festung
 
 using namespace festung;
@@ -282,5 +347,49 @@ int festung_binomial_schedule(int32_t nsteps, int32_t snapshots, int64_t* advanc
       if (x.kind == CkAction::ADVANCE) adv += x.steps;
     *advances = adv;
     *actions = int64_t(plan.size());
+  });
+}
+
+int festung_tune_save(const char* path) {
+  return guard([&] {
+    if (!path) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    std::ostringstream os;
+    os << "# festung tuning: device p ordRK limiter(-1 none, 0 linear, 1 hierarch_vert, 2 strict) "
+          "log2K elemsPerBlock\n";
+    {
+      std::lock_guard<std::mutex> lk(g_tuneMu);
+      for (const auto& e : g_tune)
+        os << std::get<0>(e.first) << ' ' << std::get<1>(e.first) << ' ' << std::get<2>(e.first) << ' '
+           << std::get<3>(e.first) << ' ' << std::get<4>(e.first) << ' ' << e.second << '\n';
+    }
+    std::ofstream f(path, std::ios::trunc);
+    if (!f) fail(FESTUNG_ERR_ARG, "cannot write tuning file %s", path);
+    f << os.str();
+    if (!f) fail(FESTUNG_ERR_ARG, "cannot write tuning file %s", path);
+  });
+}
+
+int festung_tune_load(const char* path) {
+  return guard([&] {
+    if (!path) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    std::map<TuneTuple, int> m;
+    parse_tune_file(path, m);
+    std::lock_guard<std::mutex> lk(g_tuneMu);
+    for (const auto& e : m) g_tune[e.first] = e.second;
+  });
+}
+
+int festung_tune_clear(void) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(g_tuneMu);
+    g_tune.clear();
+  });
+}
+
+int festung_tune_count(int32_t* n) {
+  return guard([&] {
+    if (!n) fail(FESTUNG_ERR_ARG, "null pointer argument");
+    std::lock_guard<std::mutex> lk(g_tuneMu);
+    *n = int32_t(g_tune.size());
   });
 }

@@ -57,4 +57,15 @@ inline void check_call(int rc) {
 
 inline int nbasis(int p) { return (p + 1) * (p + 2) / 2; }
 
+// Autotuner results (capi.cpp): elements per workgroup per tuning key, for
+// the life of the process and through festung_tune_save / festung_tune_load.
+struct TuneKey {
+  std::string device;  // architecture name and compute-unit count
+  int p, ordRK, lim;   // lim: limiter type, -1 without limiter
+  int lgK;             // ceil(log2 K)
+};
+int tune_lookup(const TuneKey& k);  // 0 when the key has no entry
+void tune_record(const TuneKey& k, int elemsPerBlock);
+void tune_load_env();               // FESTUNG_TUNE_FILE, once per process
+
 }  // namespace festung

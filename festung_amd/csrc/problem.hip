@@ -9,6 +9,7 @@
 #include <limits>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "checkpoint.hpp"
@@ -121,6 +122,31 @@ static std::vector<int32_t> hilbert_order(const double* x, const double* y, int3
 
 using namespace festung;
 
+namespace {
+// Tuning-key device part: architecture name and compute-unit count.
+std::string device_tag(int device) {
+  static std::mutex mu;
+  static std::vector<std::string> tags;
+  std::lock_guard<std::mutex> lk(mu);
+  if (device >= int(tags.size())) tags.resize(device + 1);
+  if (tags[device].empty()) {
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    std::string arch(prop.gcnArchName);
+    for (char& ch : arch)
+      if (ch == ' ') ch = '_';
+    tags[device] = arch + "/" + std::to_string(prop.multiProcessorCount) + "cu";
+  }
+  return tags[device];
+}
+
+int ceil_log2(int K) {
+  int l = 0;
+  while ((int64_t(1) << l) < K) ++l;
+  return l;
+}
+}  // namespace
+
 struct festung_problem {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -159,6 +185,9 @@ struct festung_problem {
   int nblocks = 0;   // workgroups of the element kernels = min/max partials
   int nred = 0;      // workgroups of the reduction kernels (L2 error, misfit)
   int version = 0;   // bumped when a change invalidates captured step sequences
+  TuneKey tune_key() const {
+    return TuneKey{device_tag(device), p, ordRK, (flags & FESTUNG_FLAG_LIMITER) ? limType : -1, ceil_log2(K)};
+  }
   // per-launch timing (HIP events on the problem's stream)
   bool prof = false;
   struct Rec {
@@ -586,13 +615,16 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     HIPCHK(hipMemset(h->X.p, 0, nk));
     // default elements per workgroup: the largest size that still gives at
     // least 1024 workgroups (4 per compute unit), 64 below that
-    h->bs = K >= 1024 * 256 ? 256 : K >= 1024 * 128 ? 128 : 64;
-    h->nblocks = (K + h->bs - 1) / h->bs;
-    h->nred = (K + block_size() - 1) / block_size();
     if ((flags & FESTUNG_FLAG_LIM_HIERARCH) && (flags & FESTUNG_FLAG_LIM_STRICT))
       fail(FESTUNG_ERR_ARG, "choose one limiter type");
     h->limType = (flags & FESTUNG_FLAG_LIM_HIERARCH) ? LT_HIERARCH : (flags & FESTUNG_FLAG_LIM_STRICT) ? LT_STRICT : LT_LINEAR;
     h->nF = limiter_fields(h->limType, p);
+    h->bs = K >= 1024 * 256 ? 256 : K >= 1024 * 128 ? 128 : 64;
+    // an autotuner result for this kind of problem wins over the default
+    tune_load_env();
+    if (const int tuned = tune_lookup(h->tune_key())) h->bs = tuned;
+    h->nblocks = (K + h->bs - 1) / h->bs;
+    h->nred = (K + block_size() - 1) / block_size();
     if (flags & FESTUNG_FLAG_LIMITER) {
       h->T1.alloc(nk);
       h->T2.alloc(nk);
@@ -1148,6 +1180,7 @@ int festung_autotune(festung_problem* h, double tau, int32_t nsteps, int32_t* be
     (void)hipEventDestroy(e1);
     check_call(festung_set_elem_block(h, bestBs));
     sync(h);
+    tune_record(h->tune_key(), bestBs);
     if (best) *best = bestBs;
   });
 }

@@ -240,6 +240,21 @@ int festung_get_elem_block(const festung_problem* h, int32_t* elemsPerBlock);
  * the reference. */
 int festung_autotune(festung_problem* h, double tau, int32_t nsteps, int32_t* best, double* msPerStep);
 
+/* Tuning table: festung_autotune records its choice under the key (device
+ * architecture and compute-unit count, p, ordRK, limiter type or none,
+ * ceil(log2 K)); a problem created later under the same key starts with
+ * that workgroup size instead of the default.  The table lives for the
+ * process; festung_tune_save writes it as text (one entry a line: device p
+ * ordRK limiter log2K elemsPerBlock; '#' starts a comment) and
+ * festung_tune_load merges such a file (all or nothing: a malformed line
+ * leaves the table unchanged, FESTUNG_ERR_ARG names the line).  If the
+ * environment variable FESTUNG_TUNE_FILE names a readable file, the first
+ * festung_create loads it.  No counterpart in the reference. */
+int festung_tune_save(const char* path);
+int festung_tune_load(const char* path);
+int festung_tune_clear(void);
+int festung_tune_count(int32_t* entries);
+
 /* Batched stepping: n problems advanced nsteps steps each (tau[i] for
  * problem i; ignored where a tau field is set) as ONE hipGraph launch whose
  * parallel branches are the problems' step sequences.  The graph is cached

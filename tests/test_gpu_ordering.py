@@ -121,3 +121,30 @@ def test_batch_member_calls_follow_the_batch(gpu):
     b.sync()
     for i in range(len(inputs)):
         assert np.array_equal(got[i], b.state(i)), i
+
+
+def test_tuning_table_applies_to_new_problems(gpu, tmp_path):
+    """The autotuner's choice is recorded under the problem's key; a table
+    entry loaded from a file sets the workgroup size of problems created
+    later under that key, and only under it."""
+    from festung_amd import core
+    core.tune_clear()
+    d = product_problem(TC.rotating_humps(), 1, 1 / 16, 2, False)
+    product_step(d, 1)
+    assert d.problem.elem_block == 64                       # default below 1024 * 128 elements
+    best, ms = d.problem.autotune(d.tau, 2)
+    assert best in (64, 128, 256) and core.tune_count() == 1
+    path = tmp_path / 'tune.txt'
+    core.tune_save(str(path))
+    rows = [l.split() for l in path.read_text().splitlines() if l and not l.startswith('#')]
+    assert len(rows) == 1 and rows[0][1:5] == ['1', '2', '-1', str(int(np.ceil(np.log2(d.K))))]
+    other = {64: 128, 128: 256, 256: 64}[best]
+    rows[0][5] = str(other)
+    path.write_text(' '.join(rows[0]) + '\n')
+    core.tune_clear()
+    core.tune_load(str(path))
+    e = product_problem(TC.rotating_humps(), 1, 1 / 16, 2, False)
+    assert e.problem.elem_block == other
+    f = product_problem(TC.rotating_humps(), 2, 1 / 16, 3, False)   # another key: the default
+    assert f.problem.elem_block == 64
+    core.tune_clear()

@@ -286,3 +286,34 @@ def test_binomial_schedule_is_optimal():
         binomial_schedule(0, 3)
     with pytest.raises(_lib.FestungError):
         binomial_schedule(5, 0)
+
+
+def test_tuning_table_file_roundtrip(tmp_path):
+    """festung_tune_load / festung_tune_save (host only): comments and blank
+    lines are skipped, a malformed file is rejected as a whole with its line
+    number, and a saved table loads back to the same entries."""
+    from festung_amd import core
+    from festung_amd._lib import FestungError
+    core.tune_clear()
+    assert core.tune_count() == 0
+    src = tmp_path / 'in.txt'
+    src.write_text('# device p ordRK limiter log2K elemsPerBlock\n\n'
+                   'gfx950:sramecc+:xnack-/256cu 2 3 0 19 256   # config 2\n'
+                   'gfx950:sramecc+:xnack-/256cu 4 4 -1 17 128\n')
+    core.tune_load(str(src))
+    assert core.tune_count() == 2
+    out = tmp_path / 'out.txt'
+    core.tune_save(str(out))
+    rows = sorted(tuple(l.split()) for l in out.read_text().splitlines() if l and not l.startswith('#'))
+    assert rows == sorted([('gfx950:sramecc+:xnack-/256cu', '2', '3', '0', '19', '256'),
+                           ('gfx950:sramecc+:xnack-/256cu', '4', '4', '-1', '17', '128')])
+    for bad, line in (('x 2 3 0 19 96\n', 1), ('ok 1 2 -1 10 64\nx 2 3\n', 2), ('x 7 3 0 19 64\n', 1)):
+        f = tmp_path / 'bad.txt'
+        f.write_text(bad)
+        with pytest.raises(FestungError, match='bad.txt:%d' % line):
+            core.tune_load(str(f))
+        assert core.tune_count() == 2
+    with pytest.raises(FestungError, match='cannot open'):
+        core.tune_load(str(tmp_path / 'missing.txt'))
+    core.tune_clear()
+    assert core.tune_count() == 0
