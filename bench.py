@@ -113,13 +113,10 @@ def cpu_baseline(hmax, p, ordRK, lim, tc_factory):
 def run_batched(nprob, nsteps, seed0=4000):
     """The batched workload: nprob problems (K log-uniform in [1k, 64k],
     p uniform in {1, 2, 3}, SSP-RK min(p+1, 3), seeded velocities and humps),
-    nsteps steps each, joined into one union problem per degree.  Timed in
-    both graph layouts (the degree groups as parallel branches; one stage
-    launch per sub-step index over the tile table of all groups) and as one
-    graph launch per problem.  Returns a dict of seconds, DoF-stage updates,
-    setup seconds and the total K; the layouts' results are checked to be
-    bitwise equal."""
-    import numpy as np
+    nsteps steps each, joined into one union problem per degree whose step
+    sequences are the parallel branches of one graph; against one graph
+    launch per problem.  Returns a dict of seconds, DoF-stage updates, setup
+    seconds and the total K."""
     import torch
     from festung_amd import batch as BT
     from festung_amd.core import Batch
@@ -127,27 +124,20 @@ def run_batched(nprob, nsteps, seed0=4000):
     specs = BT.batch_specs(nprob, seed0=seed0)
     inputs = [BT.ProblemInputs(s, device='cuda') for s in specs]
     bp = BT.BatchedProblems(inputs)
-    bt = BT.BatchedProblems(inputs, mode='tiles')
     singles = [BT.make_single(inp) for inp in inputs]
     sb = [Batch([s]) for s in singles]
     # warm-up: graph capture + one run of each form
-    for b in (bp, bt):
-        b.step(nsteps)
-        b.sync()
+    bp.step(nsteps)
+    bp.sync()
     for b, inp in zip(sb, inputs):
         b.step(nsteps, [inp.tau])
         b.sync()
     setup = time.perf_counter() - t0
-    t = {}
-    for name, b in (('branches', bp), ('tiles', bt)):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        b.step(nsteps)
-        b.sync()
-        t[name] = time.perf_counter() - t0
-    same = all(np.array_equal(bp.problems[g].get_state(), bt.problems[g].get_state()) for g in range(len(bp.problems)))
-    if not same:
-        raise RuntimeError('batched layouts differ')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bp.step(nsteps)
+    bp.sync()
+    t_batched = time.perf_counter() - t0
     t0 = time.perf_counter()
     for b, inp in zip(sb, inputs):
         b.step(nsteps, [inp.tau])
@@ -157,8 +147,8 @@ def run_batched(nprob, nsteps, seed0=4000):
     K_tot = sum(inp.K for inp in inputs)
     for b in sb:
         b.close()
-    return dict(branches_s=t['branches'], tiles_s=t['tiles'], single_s=t_single, updates=upd, setup_s=setup,
-                K_total=K_tot, groups=len(bp.problems))
+    return dict(batched_s=t_batched, single_s=t_single, updates=upd, setup_s=setup, K_total=K_tot,
+                groups=len(bp.problems))
 
 
 def run_adjoint(hmax, p, nsteps, ckpt, nmodes=32, ndir=16, h=1e-6, variants=True):
@@ -449,8 +439,7 @@ def main():
     if a.workload == 'batched':
         nsteps = a.steps if a.steps != 100 else 50
         rb = run_batched(a.nprob, nsteps)
-        tb, ts, upd, setup, Ktot = (min(rb['branches_s'], rb['tiles_s']), rb['single_s'], rb['updates'],
-                                    rb['setup_s'], rb['K_total'])
+        tb, ts, upd, setup, Ktot = rb['batched_s'], rb['single_s'], rb['updates'], rb['setup_s'], rb['K_total']
         print(json.dumps({
             'metric': 'DoF-stage updates/s, batched independent problems (one graph launch) vs one launch per problem',
             'value': upd / tb, 'unit': 'DoF-stage updates/s', 'n_gpus': 1, 'steps': nsteps, 'warmup': 1,
@@ -458,8 +447,7 @@ def main():
             'dtype': 'f64', 'data': 'synthetic',
             'config': {'workload': '%d problems, domainSquare K log-uniform [1k, 64k] (total K=%d), p in {1,2,3}, '
                                    'SSP-RK min(p+1,3), %d steps, limiter off' % (a.nprob, Ktot, nsteps)},
-            'batched_s': tb, 'one_launch_per_problem_s': ts, 'speedup': ts / tb, 'setup_s': setup,
-            'layouts_s': {'branches': rb['branches_s'], 'tiles': rb['tiles_s']}}))
+            'batched_s': tb, 'one_launch_per_problem_s': ts, 'speedup': ts / tb, 'setup_s': setup}))
         return
     from festung_amd.advection import testcases as TC
     lim = not a.no_limiter
@@ -486,13 +474,11 @@ def main():
         del r1
         # the batched workload (512 problems, one graph launch vs one launch per problem)
         rb = run_batched(a.nprob, 50)
-        tb, ts, upd = min(rb['branches_s'], rb['tiles_s']), rb['single_s'], rb['updates']
+        tb, ts, upd = rb['batched_s'], rb['single_s'], rb['updates']
         extra['batched'] = dict(problems=a.nprob, steps=50, K_total=rb['K_total'], batched_s=tb,
                                 one_launch_per_problem_s=ts, speedup=ts / tb, dof_stage_updates_per_s=upd / tb,
-                                setup_s=rb['setup_s'],
-                                layouts_s={'branches': rb['branches_s'], 'tiles': rb['tiles_s']})
-        log('batched: %.2fx (%.3f s vs %.3f s; branches %.3f s, tiles %.3f s)'
-            % (ts / tb, tb, ts, rb['branches_s'], rb['tiles_s']))
+                                setup_s=rb['setup_s'])
+        log('batched: %.2fx (%.3f s vs %.3f s)' % (ts / tb, tb, ts))
         # the adjoint workload (K = 32 768, 400 SSP-RK3 steps, checkpoints every 20)
         ra = run_adjoint(2.0 ** -7, 2, 400, 20, variants=False)
         extra['adjoint'] = dict(K=ra['K'], steps=400, ckpt=20, forward_ms=ra['forward_ms'],

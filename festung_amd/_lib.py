@@ -92,7 +92,6 @@ SIGNATURES = {
     'festung_batch_create': (C.c_int, [C.POINTER(vp), C.c_int32, C.POINTER(vp)]),
     'festung_batch_step': (C.c_int, [vp, C.c_int32, dp]),
     'festung_batch_sync': (C.c_int, [vp]),
-    'festung_batch_set_mode': (C.c_int, [vp, C.c_int32]),
     'festung_batch_destroy': (None, [vp]),
     'festung_adjoint': (C.c_int, [vp, C.c_int32, C.c_double, dp, C.c_int32, dp, dp, dp, dp, dp, dp]),
     'festung_binomial_schedule': (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

@@ -90,11 +90,9 @@ def make_single(inp, graph=True):
 
 
 class BatchedProblems:
-    """Union problems per degree, advanced as one graph launch: the degree
-    groups as parallel graph branches (mode 'branches') or one stage launch
-    per sub-step index over a tile table of all groups (mode 'tiles')."""
+    """Union problems per degree, advanced as one graph launch."""
 
-    def __init__(self, inputs, mode='branches'):
+    def __init__(self, inputs):
         self.inputs = list(inputs)
         self.groups = {}
         for idx, inp in enumerate(self.inputs):
@@ -114,7 +112,7 @@ class BatchedProblems:
                 self.where[i] = (len(self.problems), off, self.inputs[i].K)
                 off += self.inputs[i].K
             self.problems.append(prob)
-        self.batch = Batch(self.problems, mode)
+        self.batch = Batch(self.problems)
 
     def step(self, nsteps):
         """nsteps time steps of every problem: one graph launch."""

@@ -451,27 +451,16 @@ class Problem:
         check(lib().festung_sync(self._h))
 
 
-BATCH_BRANCHES, BATCH_TILES = 0, 1
-
-
 class Batch:
     """Several problems advanced together as one hipGraph launch
-    (festung_batch_* of the C-ABI).  mode 'branches': each problem's step
-    sequence is a parallel graph branch; 'tiles': one stage launch per
-    sub-step index over all problems (festung_batch_set_mode)."""
+    (festung_batch_* of the C-ABI)."""
 
-    def __init__(self, problems, mode='branches'):
+    def __init__(self, problems):
         self.problems = list(problems)
         arr = (C.c_void_p * len(self.problems))(*[p._h.value for p in self.problems])
         h = C.c_void_p()
         check(lib().festung_batch_create(arr, len(self.problems), C.byref(h)))
         self._h = h
-        self.set_mode(mode)
-
-    def set_mode(self, mode):
-        m = {'branches': BATCH_BRANCHES, 'tiles': BATCH_TILES}[mode]
-        check(lib().festung_batch_set_mode(self._h, m))
-        self.mode = mode
 
     def step(self, nsteps, taus):
         t = np.ascontiguousarray(taus, dtype=np.float64)

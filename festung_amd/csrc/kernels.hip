@@ -404,11 +404,8 @@ __device__ __forceinline__ void elem_term_mfma(const StageArgs& a, const double*
   }
 }
 
-// One workgroup's elements lb * blockDim.x + threadIdx.x of a stage.
-// (a by value, as a kernel parameter is: its fields are loaded once and no
-// store can alias them)
 template <int P, int MODE>
-__device__ __forceinline__ void stage_block(const StageArgs a, const int lb) {
+__global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
   constexpr bool MF = MfmaElem<P>::value;
   __shared__ double sTheta[Re * N * 9];
@@ -424,7 +421,7 @@ __device__ __forceinline__ void stage_block(const StageArgs a, const int lb) {
   const double* __restrict__ EQ = ElemTabInLDS<P>::value ? sQ : a.tab + a.off.elemQ;
 
   const int K = a.m.K;
-  const int k = lb * int(blockDim.x) + threadIdx.x;  // the expression of gtid(): the same code
+  const int k = gtid();
   const bool active = k < K;
   const int kk = active ? k : 0;
   const double* __restrict__ T = a.tab;
@@ -627,33 +624,6 @@ __device__ __forceinline__ void stage_block(const StageArgs a, const int lb) {
       }
       sink_taylor<P>(Tc, active, a.sink);
     }
-  }
-}
-
-template <int P, int MODE>
-__global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
-  stage_block<P, MODE>(a, logical_block());
-}
-
-// Stage (RK update in the DG basis, no limiter) of up to four groups of
-// elements with different degrees in one launch: the grid is the groups'
-// workgroups one after the other (a tile table sorted by degree), each
-// workgroup running its group's stage body.  The groups are independent
-// problems (or unions of problems of one degree); every element computes
-// exactly what stage_kernel computes for it.
-__global__ __launch_bounds__(BLOCK) void stage_multi_kernel(MultiStageArgs m) {
-  int lb = logical_block();
-  for (int g = 0; g < m.n; ++g) {
-    if (lb < m.nb[g]) {
-      switch (m.p[g]) {
-        case 0: stage_block<0, OUT_RK_DG>(m.a[g], lb); return;
-        case 1: stage_block<1, OUT_RK_DG>(m.a[g], lb); return;
-        case 2: stage_block<2, OUT_RK_DG>(m.a[g], lb); return;
-        case 3: stage_block<3, OUT_RK_DG>(m.a[g], lb); return;
-        default: return;  // excluded by the host
-      }
-    }
-    lb -= m.nb[g];
   }
 }
 
@@ -1918,17 +1888,6 @@ hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s) {
   }
 #undef L_STAGE
 #undef L_STAGE1
-  return hipGetLastError();
-}
-
-hipError_t launch_stage_multi(const MultiStageArgs& m, int bs, hipStream_t s) {
-  if (!valid_elem_block(bs) || m.n < 1 || m.n > kMultiGroups) return hipErrorInvalidValue;
-  int total = 0;
-  for (int g = 0; g < m.n; ++g) {
-    if (m.p[g] < 0 || m.p[g] > 3 || m.nb[g] != (m.a[g].m.K + bs - 1) / bs) return hipErrorInvalidValue;
-    total += m.nb[g];
-  }
-  FESTUNG_LAUNCH(stage_multi_kernel, dim3(total), dim3(bs), 0, s, m);
   return hipGetLastError();
 }
 

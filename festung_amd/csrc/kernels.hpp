@@ -264,17 +264,6 @@ struct SensArgs {
 };
 
 hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s);
-// Up to four element groups of degree <= 3, each one stage in the DG basis
-// without limiter (OUT_RK_DG), in one launch of bs-element workgroups:
-// group g owns workgroups nb[g] in grid order after the groups before it.
-constexpr int kMultiGroups = 4;
-struct MultiStageArgs {
-  StageArgs a[kMultiGroups];
-  int p[kMultiGroups];
-  int nb[kMultiGroups];
-  int n;
-};
-hipError_t launch_stage_multi(const MultiStageArgs& m, int bs, hipStream_t s);
 // vnw = |E_kn| w_r vn (the stage kernel's edge-flux factors; the sign is vn's)
 hipError_t launch_edge_flux_weights(int K, int Re, const double* areaE, const double* vn, const double* wEdge,
                                     double* vnw, hipStream_t s);

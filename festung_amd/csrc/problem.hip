@@ -1925,7 +1925,6 @@ struct festung_batch {
   std::vector<hipEvent_t> events;
   hipEvent_t done = nullptr;  // recorded after each launch; every member's stream waits on it
   std::vector<hipEvent_t> pre;  // each member's stream before a launch: the launch waits on them
-  int mode = FESTUNG_BATCH_BRANCHES;
   void drop() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1966,30 +1965,6 @@ int festung_batch_create(festung_problem* const* hs, int32_t n, festung_batch** 
   });
 }
 
-int festung_batch_set_mode(festung_batch* b, int32_t mode) {
-  return guard([&] {
-    if (!b) fail(FESTUNG_ERR_ARG, "null batch");
-    if (mode != FESTUNG_BATCH_BRANCHES && mode != FESTUNG_BATCH_TILES) fail(FESTUNG_ERR_ARG, "unknown batch mode %d", mode);
-    if (mode == FESTUNG_BATCH_TILES) {
-      if (b->hs.size() > size_t(kMultiGroups))
-        fail(FESTUNG_ERR_UNSUPPORTED, "the tile-table launch takes at most %d problems (join problems of one degree "
-             "with festung_grid_concat)", kMultiGroups);
-      for (size_t i = 0; i < b->hs.size(); ++i) {
-        const festung_problem* h = b->hs[i];
-        if (h->p > 3) fail(FESTUNG_ERR_UNSUPPORTED, "problem %d: the tile-table launch takes p <= 3", int(i));
-        if (h->flags & FESTUNG_FLAG_LIMITER)
-          fail(FESTUNG_ERR_UNSUPPORTED, "problem %d: the tile-table launch takes problems without limiter", int(i));
-      }
-    }
-    if (mode != b->mode) {
-      b->hs[0]->set_device();
-      if (b->exec) HIPCHK(hipStreamSynchronize(b->hs[0]->stream));
-      b->drop();
-      b->mode = mode;
-    }
-  });
-}
-
 void festung_batch_destroy(festung_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
@@ -2017,44 +1992,13 @@ int festung_batch_step(festung_batch* b, int32_t nsteps, const double* tau) {
       b->drop();
       HIPCHK(hipStreamBeginCapture(h0->stream, hipStreamCaptureModeThreadLocal));
       try {
-        if (b->mode == FESTUNG_BATCH_TILES) {
-          // one launch per sub-step index for all members that have it: the
-          // tile table of stage_multi_kernel, members in increasing degree
-          std::vector<size_t> order(n);
-          for (size_t i = 0; i < n; ++i) order[i] = i;
-          std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return b->hs[x]->p < b->hs[y]->p; });
-          int bs = 256;
-          for (auto* h : b->hs) bs = std::min(bs, h->bs);
-          size_t maxStages = 0;
-          for (auto* h : b->hs) maxStages = std::max(maxStages, h->plan.size());
-          for (int s = 0; s < nsteps; ++s) {
-            for (size_t j = 0; j < maxStages; ++j) {
-              MultiStageArgs m{};
-              for (size_t i : order) {
-                festung_problem* h = b->hs[i];
-                if (j >= h->plan.size()) continue;
-                m.a[m.n] = h->stage_args(h->plan[j], taus[i]);
-                m.a[m.n].out = h->buf(h->plan[j].out);
-                m.p[m.n] = h->p;
-                m.nb[m.n] = (h->K + bs - 1) / bs;
-                ++m.n;
-              }
-              HIPCHK(launch_stage_multi(m, bs, h0->stream));
-            }
-            for (auto* h : b->hs)
-              if (h->plan.back().out != 0)
-                HIPCHK(hipMemcpyAsync(h->X.p, h->buf(h->plan.back().out), size_t(h->N) * h->K * sizeof(double),
-                                      hipMemcpyDeviceToDevice, h0->stream));
-          }
-        } else {
-          HIPCHK(hipEventRecord(b->events[0], h0->stream));
-          for (size_t i = 1; i < n; ++i) HIPCHK(hipStreamWaitEvent(b->hs[i]->stream, b->events[0], 0));
-          for (size_t i = 0; i < n; ++i)
-            for (int s = 0; s < nsteps; ++s) b->hs[i]->enqueue_step(taus[i]);
-          for (size_t i = 1; i < n; ++i) {
-            HIPCHK(hipEventRecord(b->events[i], b->hs[i]->stream));
-            HIPCHK(hipStreamWaitEvent(h0->stream, b->events[i], 0));
-          }
+        HIPCHK(hipEventRecord(b->events[0], h0->stream));
+        for (size_t i = 1; i < n; ++i) HIPCHK(hipStreamWaitEvent(b->hs[i]->stream, b->events[0], 0));
+        for (size_t i = 0; i < n; ++i)
+          for (int s = 0; s < nsteps; ++s) b->hs[i]->enqueue_step(taus[i]);
+        for (size_t i = 1; i < n; ++i) {
+          HIPCHK(hipEventRecord(b->events[i], b->hs[i]->stream));
+          HIPCHK(hipStreamWaitEvent(h0->stream, b->events[i], 0));
         }
       } catch (...) {
         hipGraph_t g = nullptr;

@@ -267,17 +267,6 @@ int festung_batch_step(festung_batch* b, int32_t nsteps, const double* tau);
 int festung_batch_sync(festung_batch* b);
 void festung_batch_destroy(festung_batch* b);
 
-/* How a batch step is laid out in its graph.  FESTUNG_BATCH_BRANCHES (the
- * default): each problem's own step sequence (its kernels per sub-step) is
- * a parallel branch.  FESTUNG_BATCH_TILES: one stage launch per sub-step
- * index for all problems at once, over a table of workgroup tiles sorted
- * by degree (at most 4 problems, p <= 3, no limiter; problems of one degree
- * are joined first with festung_grid_concat).  Both give the same bits.
- * Changing the mode drops the cached graph.  No counterpart in the
- * reference. */
-enum { FESTUNG_BATCH_BRANCHES = 0, FESTUNG_BATCH_TILES = 1 };
-int festung_batch_set_mode(festung_batch* b, int32_t mode);
-
 /* Discrete adjoint (limiter off, f = 0, c_D = 0, SSP orders 1..3).
  * Runs nsteps steps of size tau from the current state (which becomes the
  * final state), the loss J = (c(T) - cObs)^T globM (c(T) - cObs) (M-weighted

@@ -19,47 +19,14 @@ def inputs():
     return [BT.ProblemInputs(s) for s in specs]
 
 
-@pytest.mark.parametrize('mode', ['branches', 'tiles'])
-def test_batched_equals_individual(gpu, inputs, mode):
-    """Both graph layouts (degree groups as parallel branches; one stage
-    launch per sub-step index over the tile table of all groups) give the
-    bits of the problems run one by one."""
-    b = BT.BatchedProblems(inputs, mode=mode)
+def test_batched_equals_individual(gpu, inputs):
+    b = BT.BatchedProblems(inputs)
     b.step(6)
     b.sync()
     for i, inp in enumerate(inputs):
         single = BT.make_single(inp)
         single.step(6, inp.tau)
         assert np.array_equal(b.state(i), single.get_state()), i
-
-
-def test_batched_tiles_mode_switch_and_limits(gpu, inputs):
-    """Switching the layout between launches keeps the bits; the tile table
-    rejects what it does not take (limiter, p = 4, more than four groups)."""
-    from festung_amd._lib import FestungError
-    from festung_amd.core import Batch, Grid, Problem
-    a = BT.BatchedProblems(inputs, mode='tiles')
-    a.step(2)
-    a.batch.set_mode('branches')
-    a.step(2)
-    a.batch.set_mode('tiles')
-    a.step(2)
-    a.sync()
-    c = BT.BatchedProblems(inputs)
-    c.step(6)
-    c.sync()
-    for i in range(len(inputs)):
-        assert np.array_equal(a.state(i), c.state(i)), i
-    g = Grid.domainSquare(1 / 8)
-    lim = Problem(g, 2, 3, limiter=True)
-    with pytest.raises(FestungError, match='without limiter'):
-        Batch([lim], mode='tiles')
-    p4 = Problem(g, 4, 4)
-    with pytest.raises(FestungError, match='p <= 3'):
-        Batch([p4], mode='tiles')
-    many = [Problem(g, 1, 2) for _ in range(5)]
-    with pytest.raises(FestungError, match='at most 4'):
-        Batch(many, mode='tiles')
 
 
 def test_batched_graph_relaunch(gpu, inputs):

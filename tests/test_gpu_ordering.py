@@ -148,3 +148,23 @@ def test_tuning_table_applies_to_new_problems(gpu, tmp_path):
     f = product_problem(TC.rotating_humps(), 2, 1 / 16, 3, False)   # another key: the default
     assert f.problem.elem_block == 64
     core.tune_clear()
+
+
+def test_batch_follows_members_own_work(gpu):
+    """Steps enqueued on a member's own stream right before batch.step (no
+    host sync between) are done before the batch reads that member."""
+    specs = BT.batch_specs(n=6, seed0=177, kmin=64, kmax=2048)
+    inputs = [BT.ProblemInputs(s) for s in specs]
+    out = []
+    for sync_between in (False, True):
+        b = BT.BatchedProblems(inputs)
+        assert len(b.problems) >= 2
+        m = b.problems[-1]
+        m.step(3, 1e-3)
+        if sync_between:
+            m.sync()
+        b.step(2)
+        b.sync()
+        out.append([b.state(i) for i in range(len(inputs))])
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
