@@ -79,35 +79,42 @@ def host_info():
 
 
 def cpu_baseline(hmax, p, ordRK, lim, tc_factory):
-    """The oracle (numpy/scipy restatement of the reference's per-stage
-    sparse assembly, SpMV, block solves and limiter), one time step on the
-    given mesh with the workload's closures, preprocessing excluded; once on
-    one thread and once with the BLAS thread pools at all the process's
-    CPUs (scipy's sparse kernels are single-threaded either way).  Returns
-    the cpu_baseline record (value = the one-thread rate)."""
-    from threadpoolctl import threadpool_limits
+    """The reference-structured CPU restatement (oracle/refcpu, C++ with
+    OpenMP: every sub-step re-assembles G1, G2 and R as CSR matrices, forms
+    their sparse sum, one SpMV, factors and solves every mass block, and
+    limits through block solves and CSR vertex patches), one time step on
+    the given mesh with the workload's closures, set-up excluded; once on
+    one thread and once on all the process's CPUs.  Returns the
+    cpu_baseline record (value = the all-threads rate)."""
     from oracle import advection as OA
+    from oracle import refcpu
     tc = tc_factory()
     N = (p + 1) * (p + 2) // 2
     S = {1: 1, 2: 2, 3: 3, 4: 4}[ordRK]
     info = host_info()
+    cfg = OA.Struct(testcase='custom', p=p, hmax=hmax, ordRK=ordRK, isSlopeLim=lim, numSteps=100)
+    for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):
+        cfg[k] = tc[k]
+    for f in (OA.configureProblem, OA.preprocessProblem, OA.initializeProblem):
+        cfg = f(cfg)
+    r = refcpu.RefCpu(cfg)
+    K = cfg.K
+    del cfg
     runs = {}
     for label, nthr in (('1_thread', 1), ('all_threads', info['affinity'])):
-        cfg = OA.Struct(testcase='custom', p=p, hmax=hmax, ordRK=ordRK, isSlopeLim=lim, numSteps=100)
-        for k in ('c0Cont', 'u1Cont', 'u2Cont', 'fCont', 'cDCont', 'gNCont'):
-            cfg[k] = tc[k]
-        with threadpool_limits(nthr):
-            pd = OA.main('advection', cfg, maxSteps=1)
-        runs[label] = dict(value=pd.K * N * S / pd.timeLoop, loop_s=pd.timeLoop, threads=nthr)
-        K = pd.K
-        del pd
-    return dict(value=runs['1_thread']['value'], unit='DoF-stage updates/s', cores=1, kind='port',
-                all_threads=runs['all_threads'], one_thread_loop_s=runs['1_thread']['loop_s'],
+        c0 = r.state()
+        sec = r.steps(1, threads=nthr)
+        r.set_state(c0)
+        runs[label] = dict(value=K * N * S / sec, loop_s=sec, threads=nthr)
+    del r
+    return dict(value=runs['all_threads']['value'], unit='DoF-stage updates/s', cores=info['affinity'],
+                kind='port', one_thread=runs['1_thread'], all_threads_loop_s=runs['all_threads']['loop_s'],
                 host=info, sample_K=K, sample_hmax=hmax,
-                sample='oracle (numpy/scipy reference-structured restatement: per-stage sparse kron/kronVec '
-                       'assembly, SpMV, block solves, limiter), 1 time step (%d stages) on domainSquare hmax=%g '
-                       '(K=%d), the workload\'s p / limiter / closures; preprocessing excluded; 1 thread '
-                       '(value) and %d threads (all_threads)' % (S, hmax, K, info['affinity']))
+                sample='oracle/refcpu (C++/OpenMP reference-structured restatement: per-stage CSR assembly of '
+                       'G1, G2, R, their sparse sum, SpMV, mass-block factorisations, limiter by block solves '
+                       'and CSR vertex patches), 1 time step (%d stages) on domainSquare hmax=%g (K=%d), the '
+                       'workload\'s p / limiter / closures; set-up excluded; %d threads (value) and 1 thread '
+                       '(one_thread)' % (S, hmax, K, info['affinity']))
 
 
 def run_batched(nprob, nsteps, seed0=4000):
@@ -410,7 +417,7 @@ def main():
     elif a.config == 'c3':
         a.hmax, a.p, a.ordRK, a.no_limiter, a.testcase = 2.0 ** -8, 4, 4, False, 'random_discontinuous'
         a.steps = 200 if a.steps == 100 else a.steps
-        a.cpu_hmax = 2.0 ** -6
+        a.cpu_hmax = 2.0 ** -8
     if a.gpus != 1:
         print('bench.py measures one GPU only: the path runs on a single MI355X; '
               'multi-GPU sharding is not run.', file=sys.stderr)

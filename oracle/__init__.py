@@ -15,5 +15,9 @@ Index conventions differ from Matlab only in being 0-based: vertex, element
 and edge ids start at 0 and "no neighbour" is -1.  Local edge / vertex numbers
 n = 0, 1, 2 correspond to Matlab's 1, 2, 3.
 
+``oracle/refcpu`` (C++, OpenMP, ``oracle.refcpu`` wraps it) restates the same
+per-stage algorithm in compiled code for the timed CPU baseline; it is
+checked against the numpy restatement in ``tests/test_oracle.py``.
+
 Parity status of each module is stated in its own docstring.
 """
