@@ -1256,6 +1256,30 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
 #pragma unroll
     for (int i = 0; i < 3 * Re; ++i) gvn[i] = 0.0;
   }
+  // The transposed stage alone (ADJ_STAGE) requests the neighbour ids at
+  // entry and gathers each neighbour's mu and edge-flux factors one edge
+  // ahead (edge 0's while the element term is formed), so that the dependent
+  // gathers are in flight behind arithmetic.  A missing neighbour reads the
+  // element's own data (in bounds; its contribution is skipped below).
+  // (p >= 3 keeps the loop below: the registers are not there)
+  constexpr bool FWA = W == ADJ_STAGE && P <= 2;
+  constexpr bool kAheadA = FWA;
+  [[maybe_unused]] int nbs[3];
+  [[maybe_unused]] const int flA = FWA ? a.m.flags[k] : 0;
+  double mnb[kAheadA ? 2 : 1][N], vnb[kAheadA ? 2 : 1][Re];
+  auto gatherA = [&](int n, double (&dm)[N], double (&dv)[Re]) {
+    const int nbk = nbs[n] >= 0 ? nbs[n] : k;
+    const int npk = flag_np(flA, n);
+#pragma unroll
+    for (int i = 0; i < N; ++i) dm[i] = a.mu[i * K + nbk];
+#pragma unroll
+    for (int r = 0; r < Re; ++r) dv[r] = a.vnw[(r * 3 + npk) * K + nbk];
+  };
+  if constexpr (FWA) {
+#pragma unroll
+    for (int n = 0; n < 3; ++n) nbs[n] = a.m.nbr[n * K + k];
+    if constexpr (kAheadA) gatherA(0, mnb[0], vnb[0]);
+  }
   if constexpr (P >= 1 && (ADJ || SENS_U)) {
     constexpr int UQ = RE <= 6 ? RE : 1;
 #pragma unroll UQ
@@ -1288,10 +1312,50 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
       }
     }
   }
+  if constexpr (FWA) {
+    // Same contributions in the same order as the loop below; an inactive
+    // point adds T * 0 (acc unchanged) instead of branching, since the
+    // upwind side differs from lane to lane.
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const int b = kAheadA ? (n & 1) : 0;
+      if constexpr (kAheadA) {
+        if (n < 2) gatherA(n + 1, mnb[b ^ 1], vnb[b ^ 1]);
+      } else {
+        gatherA(n, mnb[0], vnb[0]);
+      }
+      const int nb = nbs[n];
+      const int np = flag_np(flA, n);
+      const bool outOn = flag_out(flA, n);
+#pragma unroll
+      for (int r = 0; r < Re; ++r) {
+        const double v = a.vnw[(r * 3 + n) * K + k];
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) t += mu[i] * T[a.off.edgeQ + QIdx<P>::ephi(n, r, i)];
+        const double fx = (v > 0.0 && outOn) ? v * t : 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) acc[j] -= T[a.off.edgeQ + QIdx<P>::ephi(n, r, j)] * fx;
+      }
+      if (nb >= 0) {
+#pragma unroll
+        for (int r = 0; r < Re; ++r) {
+          const double v = vnb[b][r];
+          double t = 0.0;
+#pragma unroll
+          for (int i = 0; i < N; ++i) t += mnb[b][i] * sPhi1D[r + i * Re + np * Re * N];
+          const double fx = v < 0.0 ? v * t : 0.0;
+          const double* th = sTheta + r + np * Re * N + n * Re * N * 3;
+#pragma unroll
+          for (int j = 0; j < N; ++j) acc[j] -= th[j * Re] * fx;
+        }
+      }
+    }
+  }
   const int fl = a.m.flags[k];
   constexpr int kEdgeUnroll = UNR ? 3 : 1;
 #pragma unroll kEdgeUnroll
-  for (int n = 0; n < (ADJ || SENS_V ? 3 : 0); ++n) {
+  for (int n = 0; n < (!FWA && (ADJ || SENS_V) ? 3 : 0); ++n) {
     // the transposed stage alone reads the edge-flux factors |E| w_r vn
     // (the same products as area * v * w_r below)
     constexpr bool FW = W == ADJ_STAGE;
@@ -1474,14 +1538,22 @@ __device__ __forceinline__ void sens_elem_stage(const double* __restrict__ T, co
   }
 }
 
-// Partial sums of the velocity sensitivities over one chunk of a segment's
-// adjoint stages (blockIdx.y = chunk, stages in sweep order): role 0 the
-// element-term ones (G_u1, G_u2), role 1 the edge-term ones (G_vn), 64
-// elements per role; role 0 loads the next stage's inputs while it forms the
-// current one.  part[(chunk * F + f) * K + k], F = 2N + 3(p+1).
+// Velocity sensitivities of one pass over consecutive adjoint stages (sweep
+// order), in chunks of one time step: a chunk's stage contributions are
+// summed from zero in sweep order and the chunk sum is added to the running
+// G (loaded at entry, written once at the end), so the bits do not depend on
+// how the stages are split into passes (the checkpoint interval).  Role 0
+// (the first wave of a workgroup) forms the element-term ones (G_u1, G_u2),
+// role 1 the edge-term ones (G_vn), for the same 64 elements; role 0 loads
+// the next stage's inputs while it forms the current one.  Each thread runs
+// through the whole pass: on small meshes the pass holds one wave per SIMD
+// and runs beside the reverse sweep's kernels instead of crowding them out
+// (a grid of one thread per element and chunk, with the partial sums folded
+// afterwards, took the whole GPU for 0.24 ms per 20 steps at K = 32 768 and
+// stalled the sweep meanwhile: gradient 3.04 -> 2.90x the forward pass).
 template <int P>
-__global__ __launch_bounds__(BLOCK) void sens_chunk_kernel(SensArgs a) {
-  constexpr int N = Dims<P>::N, Re = Dims<P>::Re, F = 2 * N + 3 * Re;
+__global__ __launch_bounds__(BLOCK) void sens_seq_kernel(SensArgs a) {
+  constexpr int N = Dims<P>::N, Re = Dims<P>::Re;
   __shared__ double sTheta[Re * N * 9];
   fill_lds(sTheta, a.tab + a.off.theta, Re * N * 9);
   __syncthreads();
@@ -1490,24 +1562,24 @@ __global__ __launch_bounds__(BLOCK) void sens_chunk_kernel(SensArgs a) {
   const int k = logical_block() * E + (threadIdx.x - role * E);
   const int K = a.m.K;
   if (k >= K) return;
-  const int chunk = blockIdx.y;
-  const int len = (a.count + gridDim.y - 1) / gridDim.y;
-  const int e0 = a.first + chunk * len, e1 = min(a.first + a.count, e0 + len);
-  double* __restrict__ part = a.part + size_t(chunk) * F * K;
+  const int len = a.count / a.nchunk;  // stages per chunk (one time step)
+  const int e0 = a.first, e1 = a.first + a.count;
   const double* __restrict__ T = a.tab;
   Geo<P> g;
   g.load(a.m.B, k, K);
   const double tk = a.tauK ? a.tauK[k] : a.tau;
   if (role == 0) {
-    double G1[N], G2[N], mu[N], c[N];
+    double G1[N], G2[N], P1[N], P2[N], mu[N], c[N];
 #pragma unroll
-    for (int l = 0; l < N; ++l) G1[l] = G2[l] = 0.0;
-    if (e0 < e1) {
+    for (int l = 0; l < N; ++l) {
+      G1[l] = a.Gu1[l * K + k];
+      G2[l] = a.Gu2[l * K + k];
+      P1[l] = P2[l] = 0.0;
+    }
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        mu[i] = a.mus[e0][i * K + k];
-        c[i] = a.cs[e0][i * K + k];
-      }
+    for (int i = 0; i < N; ++i) {
+      mu[i] = a.mus[e0][i * K + k];
+      c[i] = a.cs[e0][i * K + k];
     }
 #pragma unroll 1
     for (int e = e0; e < e1; ++e) {
@@ -1529,8 +1601,16 @@ __global__ __launch_bounds__(BLOCK) void sens_chunk_kernel(SensArgs a) {
       sens_elem_stage<P>(T + tz, a.off, g, mu, c, al, g1, g2);
 #pragma unroll
       for (int l = 0; l < N; ++l) {
-        G1[l] += g1[l];
-        G2[l] += g2[l];
+        P1[l] += g1[l];
+        P2[l] += g2[l];
+      }
+      if ((e - e0 + 1) % len == 0) {  // end of a chunk: add its sum
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          G1[l] += P1[l];
+          G2[l] += P2[l];
+          P1[l] = P2[l] = 0.0;
+        }
       }
       if (more) {
 #pragma unroll
@@ -1542,11 +1622,10 @@ __global__ __launch_bounds__(BLOCK) void sens_chunk_kernel(SensArgs a) {
     }
 #pragma unroll
     for (int l = 0; l < N; ++l) {
-      part[l * K + k] = G1[l];
-      part[(N + l) * K + k] = G2[l];
+      a.Gu1[l * K + k] = G1[l];
+      a.Gu2[l * K + k] = G2[l];
     }
   } else {
-    // the edge data do not change over the segment
     const int fl = a.m.flags[k];
     int nb[3];
     double area[3], v[3][Re];
@@ -1557,15 +1636,18 @@ __global__ __launch_bounds__(BLOCK) void sens_chunk_kernel(SensArgs a) {
 #pragma unroll
       for (int r = 0; r < Re; ++r) v[n][r] = a.vn[(r * 3 + n) * K + k];
     }
-    double G[3 * Re];
+    double G[3 * Re], Pc[3 * Re];
 #pragma unroll
-    for (int i = 0; i < 3 * Re; ++i) G[i] = 0.0;
+    for (int i = 0; i < 3 * Re; ++i) {
+      G[i] = a.Gvn[i * K + k];
+      Pc[i] = 0.0;
+    }
 #pragma unroll 1
     for (int e = e0; e < e1; ++e) {
       const double* __restrict__ mp = a.mus[e];
       const double* __restrict__ cp = a.cs[e];
       const double al = (1.0 - a.omegas[e]) * tk;
-      int tz = 0;  // as above
+      int tz = 0;
       asm volatile("" : "+s"(tz));
       const double* __restrict__ Te = T + tz;
       const double* thb = sTheta + tz;
@@ -1609,25 +1691,18 @@ __global__ __launch_bounds__(BLOCK) void sens_chunk_kernel(SensArgs a) {
         }
       }
 #pragma unroll
-      for (int i = 0; i < 3 * Re; ++i) G[i] += gv[i];
+      for (int i = 0; i < 3 * Re; ++i) Pc[i] += gv[i];
+      if ((e - e0 + 1) % len == 0) {
+#pragma unroll
+        for (int i = 0; i < 3 * Re; ++i) {
+          G[i] += Pc[i];
+          Pc[i] = 0.0;
+        }
+      }
     }
 #pragma unroll
-    for (int i = 0; i < 3 * Re; ++i) part[(2 * N + i) * K + k] = G[i];
+    for (int i = 0; i < 3 * Re; ++i) a.Gvn[i * K + k] = G[i];
   }
-}
-
-// G += the chunks' partial sums, in chunk order.
-template <int P>
-__global__ __launch_bounds__(BLOCK) void sens_fold_kernel(SensArgs a, int nchunk) {
-  constexpr int N = Dims<P>::N, Re = Dims<P>::Re, F = 2 * N + 3 * Re;
-  const int K = a.m.K;
-  const int k = gtid();
-  if (k >= K) return;
-  const int f = blockIdx.y;
-  double* G = f < N ? a.Gu1 + size_t(f) * K : f < 2 * N ? a.Gu2 + size_t(f - N) * K : a.Gvn + size_t(f - 2 * N) * K;
-  double s = G[k];
-  for (int c = 0; c < nchunk; ++c) s += a.part[(size_t(c) * F + f) * K + k];
-  G[k] = s;
 }
 
 template <int P>
@@ -2025,12 +2100,9 @@ hipError_t launch_adj_stage(int p, const AdjArgs& a, hipStream_t s) {
 
 hipError_t launch_sens_segment(int p, const SensArgs& a, hipStream_t s) {
   if (a.count < 1 || a.first < 0 || a.nchunk < 1 || a.nchunk > a.count) return hipErrorInvalidValue;
+  if (a.count % a.nchunk) return hipErrorInvalidValue;
   constexpr int E = 64;  // two wave-sized roles per workgroup
-  const int N = (p + 1) * (p + 2) / 2, F = 2 * N + 3 * (p + 1);
-  dim3 gr(grid_for(a.m.K, E).x, a.nchunk);
-  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((sens_chunk_kernel<PP>), gr, dim3(2 * E), 0, s, a));
-  dim3 gf(grid_for(a.m.K).x, F);
-  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((sens_fold_kernel<PP>), gf, dim3(BLOCK), 0, s, a, a.nchunk));
+  FESTUNG_DISPATCH_P(p, FESTUNG_LAUNCH((sens_seq_kernel<PP>), grid_for(a.m.K, E), dim3(2 * E), 0, s, a));
   return hipGetLastError();
 }
 

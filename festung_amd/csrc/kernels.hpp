@@ -242,10 +242,10 @@ struct AdjArgs {
 // sweep from the adjoint stage inputs kept for it: for every element and the
 // segment's adjoint stages e = first .. first + count - 1,
 //   G += (1 - omega_e) tau_k d/d(u1Disc, u2Disc, vn) [mu_e^T A c_e]
-// summed in nchunk equal consecutive chunks of stages (partial sums in part,
-// nchunk x (2N + 3(p+1)) x K), then added to G in chunk order: the same
-// per-stage contributions as the stage kernels' ADJ_FUSED / ADJ_SPLIT
-// variants, added in another (fixed) order.
+// summed in nchunk equal consecutive chunks of stages (one time step each),
+// the chunk sums added to G in chunk order: the same per-stage contributions
+// as the stage kernels' ADJ_FUSED / ADJ_SPLIT variants, added in another
+// (fixed) order.
 struct SensArgs {
   Mesh m;
   const double* vn;
@@ -257,7 +257,6 @@ struct SensArgs {
   const double* const* cs;   // per adjoint stage: the forward stage state
   const double* omegas;      // per adjoint stage: omega of its RK stage
   int first, count, nchunk;
-  double* part;
   double* Gu1;
   double* Gu2;
   double* Gvn;

@@ -1503,14 +1503,12 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
     };
 
     // deferred sensitivities: per adjoint stage (sweep order) its input mu,
-    // forward stage state and omega, on the device for the segment passes
-    // The pass sums the stages of each time step as one chunk, all chunks in
-    // parallel, and adds the chunk sums in sweep order: the same order for
-    // any checkpoint interval.  At most nChunk steps per pass (partial sums
-    // of at most 512 MB).
-    const size_t chunkBytes = size_t(2 * N + 3 * Re) * K * sizeof(double);
-    const int nChunk = int(std::max<size_t>(1, std::min<size_t>(32, (size_t(512) << 20) / chunkBytes)));
-    DBuf dMus, dCs, dOm, dPart;
+    // forward stage state and omega, on the device for the segment passes.
+    // A pass sums the stages of each time step as one chunk and adds the
+    // chunk sums in sweep order: the same order for any checkpoint interval.
+    // At most nChunk steps per pass (the sweep's end waits for the last one).
+    const int nChunk = 32;
+    DBuf dMus, dCs, dOm;
     if (defer) {
       std::vector<const double*> hm, hc;
       std::vector<double> ho;
@@ -1527,7 +1525,6 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
       h->upload(dMus, hm.data(), hm.size());
       h->upload(dCs, hc.data(), hc.size());
       h->upload(dOm, ho.data(), ho.size());
-      dPart.alloc(size_t(nChunk) * chunkBytes);
     }
     // forward, storing checkpoints (and the stage states of the last segments)
     HIPCHK(hipEventRecord(ev[0], A));
@@ -1580,7 +1577,6 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
         sa.mus = static_cast<const double* const*>(dMus.p);
         sa.cs = static_cast<const double* const*>(dCs.p);
         sa.omegas = dOm.d();
-        sa.part = dPart.d();
         sa.Gu1 = Gu1.d();
         sa.Gu2 = Gu2.d();
         sa.Gvn = Gvn.d();
