@@ -412,12 +412,28 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   __shared__ double sQ[MF ? 1 : ElemTabInLDS<P>::size];
   __shared__ double sM[MF ? MfmaElem<P>::size : 1];
   extern __shared__ double sXchg[];  // MF: MfmaElem<P>::xchg per wave (dynamic)
-  fill_lds(sTheta, a.tab + a.off.theta, Re * N * 9);
-  if constexpr (MF)
-    fill_lds(sM, a.tab + a.off.mfmaE, MfmaElem<P>::size);
-  else if constexpr (ElemTabInLDS<P>::value)
-    fill_lds(sQ, a.tab + a.off.elemQ, ElemTabInLDS<P>::size);
-  __syncthreads();
+  // At p <= 3 only the edge term reads LDS (the neighbour-trace table): its
+  // loads are issued first, the element's own loads behind them, and the
+  // table is written to LDS after those are issued, with a barrier that
+  // waits for LDS only -- so the element's loads stay in flight across it
+  // instead of starting after the workgroup's table fill.
+  constexpr bool kLateFill = !MF && !ElemTabInLDS<P>::value;
+  constexpr int NTH = Re * N * 9, TPT = (NTH + 63) / 64;
+  [[maybe_unused]] double thv[kLateFill ? TPT : 1];
+  if constexpr (kLateFill) {
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+      const int i = int(threadIdx.x) + u * int(blockDim.x);
+      thv[u] = i < NTH ? a.tab[a.off.theta + i] : 0.0;
+    }
+  } else {
+    fill_lds(sTheta, a.tab + a.off.theta, NTH);
+    if constexpr (MF)
+      fill_lds(sM, a.tab + a.off.mfmaE, MfmaElem<P>::size);
+    else if constexpr (ElemTabInLDS<P>::value)
+      fill_lds(sQ, a.tab + a.off.elemQ, ElemTabInLDS<P>::size);
+    __syncthreads();
+  }
   const double* __restrict__ EQ = ElemTabInLDS<P>::value ? sQ : a.tab + a.off.elemQ;
 
   const int K = a.m.K;
@@ -469,6 +485,16 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
     for (int j = 0; j < N; ++j) dst[j] = a.c[j * K + nbk];
   };
   if constexpr (kAhead) gather(0, cnb[0]);
+  if constexpr (kLateFill) {
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+      const int i = int(threadIdx.x) + u * int(blockDim.x);
+      if (i < NTH) sTheta[i] = thv[u];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  }
 
   if constexpr (!MF && P >= 1) {
     constexpr int UQ = RE <= 6 ? RE : 1;  // p >= 3: not unrolled (measured fastest at p = 3, 4)
