@@ -42,6 +42,10 @@ def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6):
     once per element / vertex)."""
     N = (p + 1) * (p + 2) // 2
     Re = p + 1
+    # velocity data the stage kernel reads per element: at p = 1 .. 3 its two
+    # weighted factors at the RE points of quadRule2D(2p), at p = 4 (matrix
+    # cores) the DG coefficients u1, u2
+    NU = 2 * {0: 0, 1: 3, 2: 6, 3: 12}[p] if p <= 3 else 2 * N
     # per stage: (reads c0, reads the RK4 accumulator, writes it)
     rk = {1: [(0, 0, 0)], 2: [(0, 0, 0), (1, 0, 0)], 3: [(0, 0, 0), (1, 0, 0), (1, 0, 0)],
           4: [(1, 0, 1), (1, 1, 1), (1, 1, 1), (0, 1, 0)]}[ordRK]
@@ -50,10 +54,10 @@ def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6):
         comb = N * (c0 + ai + ao)    # RK-combination operands besides c and the output
         if kind == 'stage':
             # edge flux factors |E| w_r u.nu: 3 Re; B: 4; nbr, flags: 16 B
-            if lim and p >= 2:       # cdot -> Taylor: c, u1, u2, flux factors, B, nbr, flags -> cdotT
-                tot += K * (8 * (N + 2 * N + 3 * Re + 4 + N) + 4 * 4)
+            if lim and p >= 2:       # cdot -> Taylor: c, velocity, flux factors, B, nbr, flags -> cdotT
+                tot += K * (8 * (N + NU + 3 * Re + 4 + N) + 4 * 4)
             else:                    # RK -> Taylor / DG
-                tot += K * (8 * (N + comb + 2 * N + 3 * Re + 4 + N) + 16)
+                tot += K * (8 * (N + comb + NU + 3 * Re + 4 + N) + 16)
         elif kind == 'vertex' and lim:
             nv = 2 if p >= 2 else 1
             tot += nv * (4 * maxDeg * numV + 8 * K + 16 * numV)   # padded lists, centroids, bounds

@@ -447,7 +447,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   // even waves and after it in odd ones, so that the two waves sharing a SIMD
   // tend to overlap one's MFMA work with the other's gathers; else in this
   // lane's loop below.  Either way acc = edge term + element term in the end.
-  double c[N], u1[N], u2[N], acc[N];
+  double c[N], acc[N];
   [[maybe_unused]] double* X = sXchg + (threadIdx.x >> 6) * MfmaElem<P>::xchg;
   [[maybe_unused]] const bool mfmaFirst = ((k >> 6) & 1) == 0;  // wave-uniform
   if constexpr (MF)
@@ -455,10 +455,6 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     c[i] = a.c[i * K + kk];
-    if constexpr (!MF) {
-      u1[i] = a.u1[i * K + kk];
-      u2[i] = a.u2[i * K + kk];
-    }
     acc[i] = 0.0;
   }
   Geo<P> g;
@@ -497,20 +493,15 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   }
 
   if constexpr (!MF && P >= 1) {
+    // the velocity enters through its weighted factors at the points (uq)
     constexpr int UQ = RE <= 6 ? RE : 1;  // p >= 3: not unrolled (measured fastest at p = 3, 4)
 #pragma unroll UQ
     for (int q = 0; q < RE; ++q) {
-      double cq = 0.0, u1q = 0.0, u2q = 0.0;
+      double cq = 0.0;
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        const double ph = EQ[QIdx<P>::phi(q, j)];
-        cq += c[j] * ph;
-        u1q += u1[j] * ph;
-        u2q += u2[j] * ph;
-      }
-      const double wc = EQ[QIdx<P>::w(q)] * cq;
-      const double f1 = wc * (g.B22 * u1q - g.B12 * u2q);
-      const double f2 = wc * (g.B11 * u2q - g.B21 * u1q);
+      for (int j = 0; j < N; ++j) cq += c[j] * EQ[QIdx<P>::phi(q, j)];
+      const double f1 = cq * a.uq[q * K + kk];
+      const double f2 = cq * a.uq[(RE + q) * K + kk];
 #pragma unroll
       for (int i = 1; i < N; ++i) acc[i] += EQ[QIdx<P>::d1(q, i)] * f1 + EQ[QIdx<P>::d2(q, i)] * f2;
     }
@@ -978,6 +969,43 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
       }
       sink_taylor<P>(Tc, active, a.sink);
     }
+  }
+}
+
+// ------------------------------------------------------ velocity weights --
+// The stage kernel's element term needs the velocity only through
+//   w_q (B22 u1(q) - B12 u2(q)),  w_q (B11 u2(q) - B21 u1(q))
+// at the points of quadRule2D(2p) (core/assembleMatElemDphiPhiFuncDisc.m:196-197
+// in quadrature form); formed once per velocity instead of every stage.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void velocity_weights_kernel(int K, const double* __restrict__ Bp,
+                                                                 const double* __restrict__ u1,
+                                                                 const double* __restrict__ u2,
+                                                                 const double* __restrict__ tab, TabOff off,
+                                                                 double* __restrict__ uq) {
+  constexpr int N = Dims<P>::N, RE = Dims<P>::RE;
+  const int k = gtid();
+  if (k >= K) return;
+  double a[N], b[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    a[i] = u1[i * K + k];
+    b[i] = u2[i * K + k];
+  }
+  Geo<P> g;
+  g.load(Bp, k, K);
+  const double* __restrict__ EQ = tab + off.elemQ;
+  for (int q = 0; q < RE; ++q) {
+    double u1q = 0.0, u2q = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const double ph = EQ[QIdx<P>::phi(q, j)];
+      u1q += a[j] * ph;
+      u2q += b[j] * ph;
+    }
+    const double w = EQ[QIdx<P>::w(q)];
+    uq[size_t(q) * K + k] = w * (g.B22 * u1q - g.B12 * u2q);
+    uq[size_t(RE + q) * K + k] = w * (g.B11 * u2q - g.B21 * u1q);
   }
 }
 
@@ -2048,6 +2076,17 @@ hipError_t launch_limit_disc_linear(int p, const DiscLinArgs& a, hipStream_t s) 
   if (!valid_elem_block(BS)) return hipErrorInvalidValue;
   dim3 gr = grid_for(a.m.K, BS);
   FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((limit_disc_linear_kernel<PP>), gr, dim3(BS), 0, s, a));
+  return hipGetLastError();
+}
+
+hipError_t launch_velocity_weights(int p, int K, const double* B, const double* u1, const double* u2, const double* tab,
+                                   const TabOff& off, double* uq, hipStream_t s) {
+  switch (p) {
+    case 1: FESTUNG_LAUNCH(velocity_weights_kernel<1>, grid_for(K), dim3(BLOCK), 0, s, K, B, u1, u2, tab, off, uq); break;
+    case 2: FESTUNG_LAUNCH(velocity_weights_kernel<2>, grid_for(K), dim3(BLOCK), 0, s, K, B, u1, u2, tab, off, uq); break;
+    case 3: FESTUNG_LAUNCH(velocity_weights_kernel<3>, grid_for(K), dim3(BLOCK), 0, s, K, B, u1, u2, tab, off, uq); break;
+    default: return hipSuccess;
+  }
   return hipGetLastError();
 }
 

@@ -135,6 +135,10 @@ struct StageArgs {
   double* accOut;        // RK4 accumulator (out) or null
   const double* u1;
   const double* u2;
+  // p = 1 .. 3: 2 x RE x K, the element term's velocity factors at the points
+  // q of quadRule2D(2p): w_q (B22 u1 - B12 u2)(q), w_q (B11 u2 - B21 u1)(q)
+  // (launch_velocity_weights; formed whenever u1Disc or u2Disc is set)
+  const double* uq;
   const double* vnw;     // Re x 3 x K: |E_kn| w_r (u . nu)(x_knr), edge_flux_weights
   const double* rhs;     // N x K or null (globL - globKD - globKN)
   const double* tab;
@@ -290,6 +294,9 @@ hipError_t launch_to_taylor(int p, const ToTaylorArgs& a, hipStream_t s);
 hipError_t launch_centroid(int K, int bs, const double* c, double phi1c, double* cen, const MinMaxSink& sink,
                            hipStream_t s);
 hipError_t launch_limit_disc_linear(int p, const DiscLinArgs& a, hipStream_t s);
+// StageArgs::uq from u1Disc, u2Disc and B (p = 1 .. 3; nothing to do otherwise)
+hipError_t launch_velocity_weights(int p, int K, const double* B, const double* u1, const double* u2, const double* tab,
+                                   const TabOff& off, double* uq, hipStream_t s);
 hipError_t launch_project(int p, bool ordF, int K, const double* fq, const double* tab, const TabOff& off,
                           double* out, hipStream_t s);
 hipError_t launch_vnormal(int K, int Re, const double* nuE0T, const double* u1e, const double* u2e, double* vn,

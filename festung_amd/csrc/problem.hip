@@ -170,6 +170,7 @@ struct festung_problem {
   bool hasTauK = false;
   DBuf X, Y, Z, acc, T1, T2, partials, gmm, vmin, vmax, vertexD, u1, u2, vn, rhs, alpha;
   DBuf vnw;  // the stage kernel's edge-flux factors |E| w_r vn, formed whenever vn is set
+  DBuf uq;   // the stage kernel's velocity factors at the element points (p = 1 .. 3), formed with u1, u2
   int limType = LT_LINEAR;  // limiter type (core/applySlopeLimiterTaylor.m:69-85)
   int nF = 1;               // Taylor coefficients with vertex-patch bounds (limiter_fields)
   bool hasRhs = false, hasVertexD = false, hasVelocity = false;
@@ -344,6 +345,9 @@ struct festung_problem {
     drop_graph();
   }
 
+  void refresh_uq() {
+    HIPCHK(launch_velocity_weights(p, K, B.d(), u1.d(), u2.d(), tab.d(), off, uq.d(), stream));
+  }
   void refresh_vnw() {
     HIPCHK(launch_edge_flux_weights(K, tabs.Re, area.d(), vn.d(), tab.d() + off.wEdge, vnw.d(), stream));
   }
@@ -366,6 +370,7 @@ struct festung_problem {
     a.accOut = sp.accOut ? acc.d() : nullptr;
     a.u1 = u1.d();
     a.u2 = u2.d();
+    a.uq = uq.d();
     a.vnw = vnw.d();
     a.rhs = hasRhs ? rhs.d() : nullptr;
     a.tab = tab.d();
@@ -619,6 +624,7 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     h->u2.alloc(nk);
     h->vn.alloc(size_t(3) * t.Re * K * sizeof(double));
     h->vnw.alloc(size_t(3) * t.Re * K * sizeof(double));
+    if (p >= 1 && p <= 3) h->uq.alloc(size_t(2) * t.RE * K * sizeof(double));
     HIPCHK(hipMemset(h->X.p, 0, nk));
     // default elements per workgroup: the largest size that still gives at
     // least 1024 workgroups (4 per compute unit), 64 below that
@@ -681,6 +687,7 @@ int festung_set_velocity_disc(festung_problem* h, const double* u1Disc, const do
     h->set_device();
     if (u1Disc) h->upload_elem(h->u1, u1Disc, h->N, false);
     if (u2Disc) h->upload_elem(h->u2, u2Disc, h->N, false);
+    if (u1Disc || u2Disc) h->refresh_uq();
     if (vNormal) {
       h->upload_elem(h->vn, vNormal, 3 * h->tabs.Re, false);
       h->refresh_vnw();
@@ -703,6 +710,7 @@ int festung_set_velocity_points(festung_problem* h, const double* u1q, const dou
     sync(h);
     h->upload_elem(q, u2q, h->tabs.RE, false);
     HIPCHK(launch_project(h->p, false, K, q.d(), h->tab.d(), h->off, h->u2.d(), h->stream));
+    h->refresh_uq();
     h->upload_elem(e1, u1e, 3 * h->tabs.Re);
     h->upload_elem(e2, u2e, 3 * h->tabs.Re);
     HIPCHK(launch_vnormal(K, h->tabs.Re, h->nuE0T.d(), e1.d(), e2.d(), h->vn.d(), h->stream));
@@ -728,6 +736,7 @@ int festung_set_velocity_points_dev(festung_problem* h, const double* u1q, const
     HIPCHK(launch_project(h->p, false, K, h->devQ1.d(), h->tab.d(), h->off, h->u1.d(), h->stream));
     HIPCHK(launch_permute(K, RE, u2q, h->dEperm.i(), h->devQ1.d(), h->stream));
     HIPCHK(launch_project(h->p, false, K, h->devQ1.d(), h->tab.d(), h->off, h->u2.d(), h->stream));
+    h->refresh_uq();
     HIPCHK(launch_permute(K, Re3, u1e, h->dEperm.i(), h->devE1.d(), h->stream));
     HIPCHK(launch_permute(K, Re3, u2e, h->dEperm.i(), h->devE2.d(), h->stream));
     HIPCHK(launch_vnormal(K, h->tabs.Re, h->nuE0T.d(), h->devE1.d(), h->devE2.d(), h->vn.d(), h->stream));
@@ -1035,6 +1044,7 @@ int festung_rhs(festung_problem* h, double* cDot) {
     a.c0 = h->X.d();
     a.u1 = h->u1.d();
     a.u2 = h->u2.d();
+    a.uq = h->uq.d();
     a.vnw = h->vnw.d();
     a.rhs = h->hasRhs ? h->rhs.d() : nullptr;
     a.tab = h->tab.d();
@@ -1281,6 +1291,7 @@ static void adjoint_binomial(festung_problem* h, int nsteps, double tau, const d
       a.out = st + 1 < S ? outs[st].d() : out;
       a.u1 = h->u1.d();
       a.u2 = h->u2.d();
+      a.uq = h->uq.d();
       a.vnw = h->vnw.d();
       a.tab = h->tab.d();
       a.off = h->off;
@@ -1489,6 +1500,7 @@ int festung_adjoint(festung_problem* h, int32_t nsteps, double tau, const double
           }
           a.u1 = h->u1.d();
           a.u2 = h->u2.d();
+          a.uq = h->uq.d();
           a.vnw = h->vnw.d();
           a.tab = h->tab.d();
           a.off = h->off;
