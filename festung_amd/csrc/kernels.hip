@@ -310,6 +310,34 @@ __device__ __forceinline__ void rk_combine(const RKCoef& rk, double tau, const d
   }
 }
 
+// rk_combine with c0 and the accumulator already loaded (c0v, accv: read
+// only where rk_combine reads c0 / accIn); the same operations
+template <int N>
+__device__ __forceinline__ void rk_combine_pre(const RKCoef& rk, double tau, const double (&c)[N],
+                                               const double (&cdot)[N], const double (&c0v)[N],
+                                               const double (&accv)[N], double* __restrict__ accOut, int K, int kk,
+                                               bool active, double (&nx)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) nx[i] = rk.a1 * c[i] + rk.b * tau * cdot[i];
+  if (rk.a0 != 0.0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) nx[i] += rk.a0 * c0v[i];
+  }
+  if (rk.g != 0.0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) nx[i] += rk.g * accv[i];
+  }
+  if (accOut && active) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double v = rk.e * tau * cdot[i];
+      if (rk.d0 != 0.0) v += rk.d0 * c0v[i];
+      if (rk.d1 != 0.0) v += rk.d1 * accv[i];
+      accOut[i * K + kk] = v;
+    }
+  }
+}
+
 // ----------------------------------------------------------------- stage --
 
 // Large element tables (p = 4) are staged in LDS: the scalar cache cannot
@@ -754,6 +782,43 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
   for (int i = 0; i < N; ++i) x[i] = a.taylor[i * K + kk];
   Geo<P> g;
   g.load(a.m.B, kk, K);
+  // Every load that does not depend on the limiting is issued here, before
+  // any of it is computed: the vertex ids, then (the 'linear' type) the
+  // vertex-patch bounds and Dirichlet values they index, and for the dc/dt
+  // limiter the RK operands.  Loaded inside the vertex loop and after the
+  // limiting, the compiler waited for each vertex's id and bounds in turn
+  // (the loop body branches) and for the RK operands at the end: one memory
+  // round trip after another.  Same operations, same bits.
+  int vtx[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) vtx[n] = a.m.V0T[n * K + kk];
+  [[maybe_unused]] double vlo[3], vhi[3], vdr[3];
+  if constexpr (LT == LT_LINEAR) {
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      vlo[n] = a.vmin[vtx[n]];
+      vhi[n] = a.vmax[vtx[n]];
+    }
+    if (a.vertexD) {
+#pragma unroll
+      for (int n = 0; n < 3; ++n) vdr[n] = a.vertexD[vtx[n]];
+    }
+  }
+  constexpr bool RKOP = MODE != LIM_SOLUTION && P <= 2;  // (p >= 3: the registers are not there)
+  [[maybe_unused]] double cs[RKOP ? N : 1], c0v[RKOP ? N : 1], accv[RKOP ? N : 1];
+  if constexpr (RKOP) {
+    const RKCoef& rk = a.rk;
+#pragma unroll
+    for (int i = 0; i < N; ++i) cs[i] = a.c[i * K + kk];
+    if (rk.a0 != 0.0 || (a.accOut && rk.d0 != 0.0)) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) c0v[i] = a.c0[i * K + kk];
+    }
+    if (rk.g != 0.0 || (a.accOut && rk.d1 != 0.0)) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) accv[i] = a.acc[i * K + kk];
+    }
+  }
 
   // vertex values of the linear part, core/applySlopeLimiterTaylorLinear.m:117
   // (phi^Taylor_2,3 at vertex n = dX * (x_n - x_c))
@@ -774,11 +839,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
       const double val = x[0] + x[1] * (px[n] * g.dX) + x[2] * (py[n] * g.dX);
-      const int v = a.m.V0T[n * K + kk];
-      double lo = (a.vmin[v] - sNeg) + sNeg;
-      double hi = (a.vmax[v] + sPos) - sPos;
+      double lo = (vlo[n] - sNeg) + sNeg;
+      double hi = (vhi[n] + sPos) - sPos;
       if (a.vertexD) {
-        const double dv = a.vertexD[v];
+        const double dv = vdr[n];
         lo = fmin(lo, dv);
         hi = fmax(hi, dv);
       }
@@ -800,11 +864,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
     // coefficient; for ord > 1 the element's own vertex value stands in for
     // the Dirichlet data at Dirichlet vertices.
     const int fl = a.m.flags[kk];
-    int vtx[3];
     double xi[3], eta[3];
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
-      vtx[n] = a.m.V0T[n * K + kk];
       xi[n] = px[n] * g.dX;
       eta[n] = py[n] * g.dX;
     }
@@ -951,11 +1013,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
 #pragma unroll
       for (int i = 0; i < N; ++i) a.out[i * K + k] = y[i];
   } else {
-    double cs[N], nx[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) cs[i] = a.c[i * K + kk];
+    double nx[N];
     const double tk = a.tauK ? a.tauK[kk] : a.tau;
-    rk_combine<N>(a.rk, tk, cs, y, a.c0, a.acc, a.accOut, K, kk, active, nx);
+    if constexpr (RKOP) {
+      rk_combine_pre<N>(a.rk, tk, cs, y, c0v, accv, a.accOut, K, kk, active, nx);
+    } else {
+      double cl[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) cl[i] = a.c[i * K + kk];
+      rk_combine<N>(a.rk, tk, cl, y, a.c0, a.acc, a.accOut, K, kk, active, nx);
+    }
     if constexpr (MODE == LIM_DOT_RK_DG) {
       if (active)
 #pragma unroll
