@@ -46,6 +46,15 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 }
 __device__ __forceinline__ int logical_block() { return xcd_block(blockIdx.x, gridDim.x); }
 __device__ __forceinline__ int gtid() { return logical_block() * int(blockDim.x) + threadIdx.x; }
+// The same for a kernel launched with bs threads per workgroup over K
+// elements (ceil(K / bs) workgroups), from the kernel's own arguments:
+// blockDim.x and gridDim.x come from the dispatch packet through a 16-bit
+// vector load, and the kernel's first memory wait then also waits for every
+// load issued before it (the stage kernel's table loads, for one).
+__device__ __forceinline__ int elem_blocks(int K, int bs) { return (K + bs - 1) / bs; }
+__device__ __forceinline__ int elem_index(int K, int bs) {
+  return xcd_block(blockIdx.x, elem_blocks(K, bs)) * bs + threadIdx.x;
+}
 
 // ---------------------------------------------------------------- Taylor --
 // E_D(A): the degree-D symmetric power of a 2x2 matrix A acting on
@@ -249,8 +258,10 @@ __host__ __device__ constexpr double tfab(int i) { return fact4(tdeg(i) - tb(i))
 // each active thread -> partials (see MinMaxSink).  Plain stores: the
 // consumer is a later kernel on the same stream.  Every thread of the
 // workgroup must call this (it synchronises the workgroup).
+// (bs: threads per workgroup, nb: workgroups, as launched)
 template <int P>
-__device__ __forceinline__ void sink_taylor(const double (&Tc)[Dims<P>::N], bool active, const MinMaxSink& s) {
+__device__ __forceinline__ void sink_taylor(const double (&Tc)[Dims<P>::N], bool active, const MinMaxSink& s, int bs,
+                                            int nb) {
   constexpr int FM = P * (P + 1) / 2 > 1 ? P * (P + 1) / 2 : 1;  // most fields a limiter type needs
   __shared__ double smn[FM][BLOCK / 64], smx[FM][BLOCK / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -273,11 +284,11 @@ __device__ __forceinline__ void sink_taylor(const double (&Tc)[Dims<P>::N], bool
   const int f = threadIdx.x;
   if (f < s.F) {
     double a = smn[f][0], b = smx[f][0];
-    for (int i = 1; i < int(blockDim.x) / 64; ++i) {
+    for (int i = 1; i < bs / 64; ++i) {
       a = fmin(a, smn[f][i]);
       b = fmax(b, smx[f][i]);
     }
-    const size_t o = 2 * (size_t(f) * gridDim.x + blockIdx.x);
+    const size_t o = 2 * (size_t(f) * nb + blockIdx.x);
     s.partials[o] = a;
     s.partials[o + 1] = b;
   }
@@ -353,9 +364,8 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // Copy n doubles of a table into LDS with the whole workgroup; up to 8 loads
 // per thread are in flight before the first store (a plain strided loop
 // waits for each load in turn).
-__device__ __forceinline__ void fill_lds(double* __restrict__ dst, const double* __restrict__ src, int n) {
+__device__ __forceinline__ void fill_lds(double* __restrict__ dst, const double* __restrict__ src, int n, int step) {
   constexpr int U = 8;
-  const int step = blockDim.x;
 #pragma unroll 1
   for (int i0 = threadIdx.x; i0 < n; i0 += U * step) {
     double v[U];
@@ -451,21 +461,21 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   if constexpr (kLateFill) {
 #pragma unroll
     for (int u = 0; u < TPT; ++u) {
-      const int i = int(threadIdx.x) + u * int(blockDim.x);
+      const int i = int(threadIdx.x) + u * a.m.bs;
       thv[u] = i < NTH ? a.tab[a.off.theta + i] : 0.0;
     }
   } else {
-    fill_lds(sTheta, a.tab + a.off.theta, NTH);
+    fill_lds(sTheta, a.tab + a.off.theta, NTH, a.m.bs);
     if constexpr (MF)
-      fill_lds(sM, a.tab + a.off.mfmaE, MfmaElem<P>::size);
+      fill_lds(sM, a.tab + a.off.mfmaE, MfmaElem<P>::size, a.m.bs);
     else if constexpr (ElemTabInLDS<P>::value)
-      fill_lds(sQ, a.tab + a.off.elemQ, ElemTabInLDS<P>::size);
+      fill_lds(sQ, a.tab + a.off.elemQ, ElemTabInLDS<P>::size, a.m.bs);
     __syncthreads();
   }
   const double* __restrict__ EQ = ElemTabInLDS<P>::value ? sQ : a.tab + a.off.elemQ;
 
   const int K = a.m.K;
-  const int k = gtid();
+  const int k = elem_index(K, a.m.bs);
   const bool active = k < K;
   const int kk = active ? k : 0;
   const double* __restrict__ T = a.tab;
@@ -512,7 +522,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   if constexpr (kLateFill) {
 #pragma unroll
     for (int u = 0; u < TPT; ++u) {
-      const int i = int(threadIdx.x) + u * int(blockDim.x);
+      const int i = int(threadIdx.x) + u * a.m.bs;
       if (i < NTH) sTheta[i] = thv[u];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -650,7 +660,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 #pragma unroll
       for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
     }
-    sink_taylor<P>(Tc, active, a.sink);
+    sink_taylor<P>(Tc, active, a.sink, a.m.bs, elem_blocks(K, a.m.bs));
   } else {
     // RK combination (advection/solveSubStep.m:83 for the SSP weights)
     double nx[N];
@@ -667,7 +677,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 #pragma unroll
         for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
       }
-      sink_taylor<P>(Tc, active, a.sink);
+      sink_taylor<P>(Tc, active, a.sink, a.m.bs, elem_blocks(K, a.m.bs));
     }
   }
 }
@@ -722,7 +732,7 @@ __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
     return;
   }
   // workgroups 1 .. numV / BLOCK: one vertex per thread
-  const int v = xcd_block(blockIdx.x - 1, gridDim.x - 1) * int(blockDim.x) + threadIdx.x;
+  const int v = xcd_block(blockIdx.x - 1, elem_blocks(a.numV, BLOCK)) * BLOCK + threadIdx.x;
   if (v >= a.numV) return;
   for (int j0 = 0; j0 < a.maxDeg; j0 += 8) {
     int id[8];
@@ -772,7 +782,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
     LimitArgs a) {
   constexpr int N = Dims<P>::N;
   const int K = a.m.K;
-  const int k = gtid();
+  const int k = elem_index(K, a.m.bs);
   const bool active = k < K;
   const int kk = active ? k : 0;
   const double* __restrict__ T = a.tab;
@@ -1034,7 +1044,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
 #pragma unroll
         for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
       }
-      sink_taylor<P>(Tc, active, a.sink);
+      sink_taylor<P>(Tc, active, a.sink, a.m.bs, elem_blocks(K, a.m.bs));
     }
   }
 }
@@ -1082,7 +1092,7 @@ template <int P>
 __global__ __launch_bounds__(BLOCK) void to_taylor_kernel(ToTaylorArgs a) {
   constexpr int N = Dims<P>::N;
   const int K = a.K;
-  const int k = gtid();
+  const int k = elem_index(K, a.bs);
   const bool active = k < K;
   const int kk = active ? k : 0;
   double x[N], Tc[N];
@@ -1095,7 +1105,7 @@ __global__ __launch_bounds__(BLOCK) void to_taylor_kernel(ToTaylorArgs a) {
 #pragma unroll
     for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
   }
-  sink_taylor<P>(Tc, active, a.sink);
+  sink_taylor<P>(Tc, active, a.sink, a.bs, elem_blocks(K, a.bs));
 }
 
 // ---------------------------------------------------- DG-linear limit --
@@ -1112,7 +1122,7 @@ __global__ __launch_bounds__(BLOCK) void centroid_kernel(int K, const double* __
   const bool active = k < K;
   const double v[1] = {active ? c[k] * phi1c : 0.0};
   if (active) cen[k] = v[0];
-  sink_taylor<0>(v, active, sink);
+  sink_taylor<0>(v, active, sink, int(blockDim.x), int(gridDim.x));
 }
 
 template <int P>
@@ -1343,8 +1353,29 @@ __global__ __launch_bounds__(BLOCK) void l2_kernel(int K, const double* __restri
 // sensitivities; ADJ_SENS: the sensitivities only; ADJ_SENS_U / ADJ_SENS_V:
 // only those w.r.t. u1Disc, u2Disc (element term) / u.nu (edge term).
 // UNR: unroll the edge loop (more loads in flight per thread, more registers).
-template <int P, int W, bool UNR = false>
-__device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* sTheta, const double* sPhi1D) {
+// LATE (the transposed stage alone, p <= 2): the caller has loaded this
+// thread's share of the two LDS tables into tv (adj_table_regs) and not yet
+// written them; adj_body writes them after issuing the element's own loads
+// and waits at a barrier that waits for LDS only, so those loads stay in
+// flight across it.  Every thread of the workgroup must then call adj_body;
+// inactive ones (active false) pass an element in range and store nothing.
+template <int P>
+struct AdjTab {
+  static constexpr int N1 = Dims<P>::Re * Dims<P>::N * 9, N2 = Dims<P>::Re * Dims<P>::N * 3, TPT = (N1 + N2 + 63) / 64;
+};
+template <int P>
+__device__ __forceinline__ void adj_table_regs(const AdjArgs& a, double (&tv)[AdjTab<P>::TPT]) {
+  using AT = AdjTab<P>;
+#pragma unroll
+  for (int u = 0; u < AT::TPT; ++u) {
+    const int i = int(threadIdx.x) + u * a.m.bs;
+    tv[u] = i < AT::N1 ? a.tab[a.off.theta + i] : i < AT::N1 + AT::N2 ? a.tab[a.off.phi1D + i - AT::N1] : 0.0;
+  }
+}
+
+template <int P, int W, bool UNR = false, bool LATE = false>
+__device__ __forceinline__ void adj_body(const AdjArgs& a, int k, double* sTheta, double* sPhi1D, bool active = true,
+                                         const double* tv = nullptr) {
   constexpr bool ADJ = W == ADJ_STAGE || W == ADJ_FUSED;
   constexpr bool SENS_U = W == ADJ_FUSED || W == ADJ_SENS || W == ADJ_SENS_U;
   constexpr bool SENS_V = W == ADJ_FUSED || W == ADJ_SENS || W == ADJ_SENS_V;
@@ -1400,6 +1431,18 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
 #pragma unroll
     for (int n = 0; n < 3; ++n) nbs[n] = a.m.nbr[n * K + k];
     if constexpr (kAheadA) gatherA(0, mnb[0], vnb[0]);
+  }
+  if constexpr (LATE) {
+    using AT = AdjTab<P>;
+#pragma unroll
+    for (int u = 0; u < AT::TPT; ++u) {
+      const int i = int(threadIdx.x) + u * a.m.bs;
+      if (i < AT::N1) sTheta[i] = tv[u];
+      else if (i < AT::N1 + AT::N2) sPhi1D[i - AT::N1] = tv[u];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
   }
   if constexpr (P >= 1 && (ADJ || SENS_U)) {
     constexpr int UQ = RE <= 6 ? RE : 1;
@@ -1556,6 +1599,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
     for (int j = 0; j < N; ++j) s += T[a.off.hatMinvR + i * N + j] * acc[j];
     z[i] = s * idet;
   }
+  if (!active) return;
   if (a.mode == 4) {
 #pragma unroll
     for (int i = 0; i < N; ++i) a.out[i * K + k] = z[i];
@@ -1586,8 +1630,8 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, const double* 
 template <int P>
 __device__ __forceinline__ void adj_tables(const AdjArgs& a, double* sTheta, double* sPhi1D) {
   constexpr int N = Dims<P>::N, Re = Dims<P>::Re;
-  fill_lds(sTheta, a.tab + a.off.theta, Re * N * 9);
-  fill_lds(sPhi1D, a.tab + a.off.phi1D, Re * N * 3);
+  fill_lds(sTheta, a.tab + a.off.theta, Re * N * 9, int(blockDim.x));
+  fill_lds(sPhi1D, a.tab + a.off.phi1D, Re * N * 3, int(blockDim.x));
   __syncthreads();
 }
 
@@ -1597,10 +1641,19 @@ __global__ __launch_bounds__(BLOCK) void adj_stage_kernel(AdjArgs a) {
   constexpr int N = Dims<P>::N, Re = Dims<P>::Re;
   __shared__ double sTheta[Re * N * 9];
   __shared__ double sPhi1D[Re * N * 3];
-  adj_tables<P>(a, sTheta, sPhi1D);
-  const int k = gtid();
-  if (k >= a.m.K) return;
-  adj_body<P, W>(a, k, sTheta, sPhi1D);
+  if constexpr (W == ADJ_STAGE && P <= 2) {
+    // the tables are written to LDS inside adj_body, after the element's loads
+    double tv[AdjTab<P>::TPT];
+    adj_table_regs<P>(a, tv);
+    const int k = elem_index(a.m.K, a.m.bs);
+    const bool active = k < a.m.K;
+    adj_body<P, W, false, true>(a, active ? k : 0, sTheta, sPhi1D, active, tv);
+  } else {
+    adj_tables<P>(a, sTheta, sPhi1D);
+    const int k = gtid();
+    if (k >= a.m.K) return;
+    adj_body<P, W>(a, k, sTheta, sPhi1D);
+  }
 }
 
 // Three threads per element: the first third of the workgroup applies the
@@ -1676,7 +1729,7 @@ template <int P>
 __global__ __launch_bounds__(BLOCK) void sens_seq_kernel(SensArgs a) {
   constexpr int N = Dims<P>::N, Re = Dims<P>::Re;
   __shared__ double sTheta[Re * N * 9];
-  fill_lds(sTheta, a.tab + a.off.theta, Re * N * 9);
+  fill_lds(sTheta, a.tab + a.off.theta, Re * N * 9, int(blockDim.x));
   __syncthreads();
   constexpr int E = 64;
   const int role = threadIdx.x / E;
