@@ -694,13 +694,22 @@ __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
       const double* part = a.partials + 2 * size_t(f) * a.nPartials;
       double mn = INFINITY, mx = -INFINITY;
       for (int i0 = 0; i0 < a.nPartials; i0 += 8 * BLOCK) {
+        // (out-of-range slots read partial 0 and are masked after the
+        // loads, so that the eight loads issue together instead of each
+        // behind a branch and its own wait)
         double lo[8], hi[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const int i = i0 + u * BLOCK + threadIdx.x;
-          const bool ok = i < a.nPartials;
-          lo[u] = ok ? part[2 * i] : INFINITY;
-          hi[u] = ok ? part[2 * i + 1] : -INFINITY;
+          const int ic = i < a.nPartials ? i : 0;
+          lo[u] = part[2 * ic];
+          hi[u] = part[2 * ic + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const bool ok = i0 + u * BLOCK + int(threadIdx.x) < a.nPartials;
+          lo[u] = ok ? lo[u] : INFINITY;
+          hi[u] = ok ? hi[u] : -INFINITY;
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -742,12 +751,16 @@ __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
       const double* fld = a.fields + size_t(f) * a.K;
       double m = j0 == 0 ? INFINITY : a.vmin[size_t(f) * a.numV + v];
       double M = j0 == 0 ? -INFINITY : a.vmax[size_t(f) * a.numV + v];
+      // all gathers first (a padding slot reads element 0, masked below):
+      // behind a per-slot branch each one waited for the one before
+      double cv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) cv[u] = fld[id[u] >= 0 ? id[u] : 0];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         if (id[u] >= 0) {
-          const double c = fld[id[u]];
-          m = fmin(m, c);
-          M = fmax(M, c);
+          m = fmin(m, cv[u]);
+          M = fmax(M, cv[u]);
         }
       }
       a.vmin[size_t(f) * a.numV + v] = m;
