@@ -497,6 +497,25 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   }
   Geo<P> g;
   g.load(a.m.B, kk, K);
+  // RK operands (c_0, the accumulator) and the element's time step with the
+  // element's other loads: read where they are combined, each load came
+  // after the previous coefficient's store (the output may alias them) and
+  // waited alone.  (p = 4: the registers are not there.)
+  constexpr bool kRKPre = (MODE == OUT_RK_DG || MODE == OUT_RK_TAYLOR) && P <= 3;
+  [[maybe_unused]] double c0v[kRKPre ? N : 1], accv[kRKPre ? N : 1];
+  [[maybe_unused]] double tkp = a.tau;
+  if constexpr (kRKPre) {
+    const RKCoef& rk = a.rk;
+    if (rk.a0 != 0.0 || (a.accOut && rk.d0 != 0.0)) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) c0v[i] = a.c0[i * K + kk];
+    }
+    if (rk.g != 0.0 || (a.accOut && rk.d1 != 0.0)) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) accv[i] = a.acc[i * K + kk];
+    }
+    if (a.tauK) tkp = a.tauK[kk];
+  }
 
   // Neighbour ids are requested up front.  At p <= 2 the neighbours'
   // coefficients are gathered one edge ahead (edge 0's while the element
@@ -664,8 +683,12 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   } else {
     // RK combination (advection/solveSubStep.m:83 for the SSP weights)
     double nx[N];
-    const double tk = a.tauK ? a.tauK[kk] : a.tau;
-    rk_combine<N>(a.rk, tk, c, cdot, a.c0, a.acc, a.accOut, K, kk, active, nx);
+    if constexpr (kRKPre) {
+      rk_combine_pre<N>(a.rk, tkp, c, cdot, c0v, accv, a.accOut, K, kk, active, nx);
+    } else {
+      const double tk = a.tauK ? a.tauK[kk] : a.tau;
+      rk_combine<N>(a.rk, tk, c, cdot, a.c0, a.acc, a.accOut, K, kk, active, nx);
+    }
     if constexpr (MODE == OUT_RK_DG) {
       if (active)
 #pragma unroll
@@ -1408,6 +1431,15 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, double* sTheta
   g.load(a.m.B, k, K);
   const double tk = a.tauK ? a.tauK[k] : a.tau;
   const double al = (1.0 - a.omega) * tk;   // weight of this stage's sensitivity
+  // the running c_0 sum of modes 1, 2 with the element's other loads (read
+  // where it is used, each load waited alone behind the previous store)
+  [[maybe_unused]] double L0v[ADJ ? N : 1];
+  if constexpr (ADJ) {
+    if (a.mode == 1 || a.mode == 2) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) L0v[i] = a.L0[i * K + k];
+    }
+  }
   double c[SENS ? N : 1], gu1[SENS_U ? N : 1], gu2[SENS_U ? N : 1], gvn[SENS_V ? 3 * Re : 1];
   if constexpr (SENS) {
 #pragma unroll
@@ -1629,11 +1661,11 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, double* sTheta
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       a.out[i * K + k] = w1 * (mu[i] + tk * z[i]);
-      a.L0[i * K + k] += a.omega * mu[i];
+      a.L0[i * K + k] = L0v[i] + a.omega * mu[i];
     }
   } else if (a.mode == 2) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) a.out[i * K + k] = a.L0[i * K + k] + a.omega * mu[i] + w1 * (mu[i] + tk * z[i]);
+    for (int i = 0; i < N; ++i) a.out[i * K + k] = L0v[i] + a.omega * mu[i] + w1 * (mu[i] + tk * z[i]);
   } else {
 #pragma unroll
     for (int i = 0; i < N; ++i) a.out[i * K + k] = a.omega * mu[i] + w1 * (mu[i] + tk * z[i]);
