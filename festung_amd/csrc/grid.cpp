@@ -325,6 +325,21 @@ Grid concat_grids(const std::vector<const Grid*>& gs) {
   g.numT = int32_t(sT);
   g.numV = int32_t(sV);
   g.numE = int32_t(sE);
+  // part boundaries (parts of parts are kept)
+  for (size_t i = 0; i < n; ++i) {
+    const Grid& q = *gs[i];
+    if (q.partT.size() >= 2) {
+      for (size_t j = 0; j + 1 < q.partT.size(); ++j) {
+        g.partT.push_back(q.partT[j] + offT[i]);
+        g.partV.push_back(q.partV[j] + offV[i]);
+      }
+    } else {
+      g.partT.push_back(offT[i]);
+      g.partV.push_back(offV[i]);
+    }
+  }
+  g.partT.push_back(g.numT);
+  g.partV.push_back(g.numV);
   auto shift = [](int32_t v, int32_t o) { return v < 0 ? v : v + o; };
   g.coordV = cat_cols<double>(gs, V, 2, [&](size_t i, size_t j) { return gs[i]->coordV[j]; });
   g.V0T = cat_cols<int32_t>(gs, T, 3, [&](size_t i, size_t j) { return shift(gs[i]->V0T[j], offV[i]); });

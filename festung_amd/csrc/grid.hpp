@@ -28,6 +28,9 @@ struct Grid {
   // boundary marks (advection/configureProblem.m:148-150,
   // advection/preprocessProblem.m:72-76)
   std::vector<uint8_t> markE0Tint, markE0TbdrD, markE0TbdrN, markV0TbdrD;  // numT x 3
+  // concat_grids: first element / vertex of each part, then the totals
+  // (empty for a grid of one part)
+  std::vector<int32_t> partT, partV;
 };
 
 // core/generateGridData.m:130-198; V0T 0-based, numT x 3 column-major

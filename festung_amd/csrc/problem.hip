@@ -562,8 +562,22 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
     h->upload(h->tab, tb.data(), tb.size());
 
     // device order of elements and vertices (Hilbert curve)
-    h->eperm = hilbert_order(g.baryT.data(), g.baryT.data() + K, K);
-    h->vperm = hilbert_order(g.coordV.data(), g.coordV.data() + g.numV, g.numV);
+    // (a union of grids: each part along its own curve, parts in order, so
+    // that a part's elements, and its neighbour gathers, stay together --
+    // the parts all cover the same square, and one curve over the union
+    // interleaved them)
+    auto order = [](const std::vector<int32_t>& part, const double* xy, int32_t n) {
+      if (part.size() < 3) return hilbert_order(xy, xy + n, n);
+      std::vector<int32_t> perm;
+      perm.reserve(n);
+      for (size_t i = 0; i + 1 < part.size(); ++i) {
+        const int32_t a = part[i], b = part[i + 1];
+        for (int32_t x : hilbert_order(xy + a, xy + n + a, b - a)) perm.push_back(x + a);
+      }
+      return perm;
+    };
+    h->eperm = order(g.partT, g.baryT.data(), K);
+    h->vperm = order(g.partV, g.coordV.data(), g.numV);
     h->einv.assign(K, 0);
     h->vinv.assign(g.numV, 0);
     for (int32_t d = 0; d < K; ++d) h->einv[h->eperm[d]] = d;
