@@ -296,29 +296,29 @@ __device__ __forceinline__ void sink_taylor(const double (&Tc)[Dims<P>::N], bool
 
 // ------------------------------------------------------------------- RK --
 // out = a0 c0 + a1 c + b tau cdot + g acc;  accOut = d0 c0 + d1 acc + e tau cdot
+// (rk_combine_pre below does the arithmetic; rk_combine first loads c0 and
+// the accumulator where they are used -- all of them before the first store:
+// accOut may be the accumulator itself, and loads placed after a store to it
+// would each wait alone)
+template <int N>
+__device__ __forceinline__ void rk_combine_pre(const RKCoef& rk, double tau, const double (&c)[N],
+                                               const double (&cdot)[N], const double (&c0v)[N],
+                                               const double (&accv)[N], double* __restrict__ accOut, int K, int kk,
+                                               bool active, double (&nx)[N]);
 template <int N>
 __device__ __forceinline__ void rk_combine(const RKCoef& rk, double tau, const double (&c)[N], const double (&cdot)[N],
                                            const double* __restrict__ c0, const double* __restrict__ accIn,
                                            double* __restrict__ accOut, int K, int kk, bool active, double (&nx)[N]) {
+  double c0v[N], accv[N];
+  if (rk.a0 != 0.0 || (accOut && rk.d0 != 0.0)) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) nx[i] = rk.a1 * c[i] + rk.b * tau * cdot[i];
-  if (rk.a0 != 0.0) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) nx[i] += rk.a0 * c0[i * K + kk];
+    for (int i = 0; i < N; ++i) c0v[i] = c0[i * K + kk];
   }
-  if (rk.g != 0.0) {
+  if (rk.g != 0.0 || (accOut && rk.d1 != 0.0)) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) nx[i] += rk.g * accIn[i * K + kk];
+    for (int i = 0; i < N; ++i) accv[i] = accIn[i * K + kk];
   }
-  if (accOut && active) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double v = rk.e * tau * cdot[i];
-      if (rk.d0 != 0.0) v += rk.d0 * c0[i * K + kk];
-      if (rk.d1 != 0.0) v += rk.d1 * accIn[i * K + kk];
-      accOut[i * K + kk] = v;
-    }
-  }
+  rk_combine_pre<N>(rk, tau, c, cdot, c0v, accv, accOut, K, kk, active, nx);
 }
 
 // rk_combine with c0 and the accumulator already loaded (c0v, accv: read
