@@ -143,6 +143,12 @@ std::vector<double> ref_table(int p, const std::string& name, int ord) {
   const int N = t.N;
   if (name == "hatM") return t.hatM;
   if (name == "hatG") return t.hatG;
+  if (name == "quadG") {
+    std::vector<double> v(t.ruleG.q1);
+    v.insert(v.end(), t.ruleG.q2.begin(), t.ruleG.q2.end());
+    v.insert(v.end(), t.ruleG.w.begin(), t.ruleG.w.end());
+    return v;
+  }
   if (name == "hatRdiag") return t.hatRdiag;
   if (name == "hatRoff") return t.hatRoff;
   if (name == "taylorF") return t.F;

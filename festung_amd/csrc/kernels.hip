@@ -378,7 +378,10 @@ __device__ __forceinline__ void fill_lds(double* __restrict__ dst, const double*
 }
 
 // Element term  (G1 + G2) c  of the 64 elements e0 .. e0+63 of one wave on
-// the matrix cores, as two GEMMs per 16-element group and 16-point block:
+// the matrix cores, as two GEMMs per 16-element group and 16-point block of
+// the element term's rule (28 points at p = 4: the products are polynomials
+// of degree 11, which that rule integrates exactly, as the reference's
+// 64-point quadRule2D(8) does):
 //   V[q][e] = sum_j phi_j(q) X_j[e]            for X = c, u1, u2
 //   acc[i][e] += sum_q w_q d1phi_i(q) F1[q][e] + w_q d2phi_i(q) F2[q][e]
 // with F1 = c (B22 u1 - B12 u2), F2 = c (B11 u2 - B21 u1) pointwise.  In the
@@ -386,6 +389,8 @@ __device__ __forceinline__ void fill_lds(double* __restrict__ dst, const double*
 // D[l / 16 + 4 t][l % 16] in accumulator register t, so register t of V is
 // the B operand of the second GEMM for the points q = 16 qb + 4 t + l / 16
 // (the table D1 / D2 is laid out to match): no data moves between lanes.
+// The last block's rows past the rule are zero in A; its registers t with
+// 16 qb + 4 t >= RG are skipped.
 // The result goes to the wave's LDS buffer X as X[i][element - e0] (16 rows),
 // where the one-element-per-lane code picks it up.  Every lane of the wave
 // must call this (the MFMA is a wave operation).
@@ -418,7 +423,7 @@ __device__ __forceinline__ void elem_term_mfma(const StageArgs& a, const double*
     if (g < 3) load(g + 1, b ^ 1);
     const double B11 = Bg[b][0], B21 = Bg[b][1], B12 = Bg[b][2], B22 = Bg[b][3];
     dbl4 r1 = {0.0, 0.0, 0.0, 0.0}, r2 = r1;
-#pragma unroll 1
+#pragma unroll
     for (int qb = 0; qb < M::QB; ++qb) {
       dbl4 vc = {0.0, 0.0, 0.0, 0.0}, v1 = vc, v2 = vc;
 #pragma unroll
@@ -430,6 +435,7 @@ __device__ __forceinline__ void elem_term_mfma(const StageArgs& a, const double*
       }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
+        if (t >= M::steps(qb)) break;
         const double f1 = vc[t] * (B22 * v1[t] - B12 * v2[t]);
         const double f2 = vc[t] * (B11 * v2[t] - B21 * v1[t]);
         r1 = __builtin_amdgcn_mfma_f64_16x16x4f64(sM[M::offD1 + (qb * 4 + t) * 64 + l], f1, r1, 0, 0, 0);
@@ -1490,24 +1496,24 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, double* sTheta
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
   }
   if constexpr (P >= 1 && (ADJ || SENS_U)) {
-    constexpr int UQ = RE <= 6 ? RE : 1;
+    constexpr int RG = Dims<P>::RG, UQ = RG <= 6 ? RG : 1;
 #pragma unroll UQ
-    for (int q = 0; q < RE; ++q) {
+    for (int q = 0; q < RG; ++q) {
       double m1 = 0.0, m2 = 0.0, u1q = 0.0, u2q = 0.0, cq = 0.0;
 #pragma unroll
       for (int j = 0; j < N; ++j) {
-        const double ph = T[a.off.elemQ + QIdx<P>::phi(q, j)];
-        m1 += mu[j] * T[a.off.elemQ + QIdx<P>::d1(q, j)];
-        m2 += mu[j] * T[a.off.elemQ + QIdx<P>::d2(q, j)];
+        const double ph = T[a.off.elemG + QIdx<P>::phi(q, j)];
+        m1 += mu[j] * T[a.off.elemG + QIdx<P>::d1(q, j)];
+        m2 += mu[j] * T[a.off.elemG + QIdx<P>::d2(q, j)];
         u1q += u1[j] * ph;
         u2q += u2[j] * ph;
         if constexpr (SENS_U) cq += c[j] * ph;
       }
-      const double wq = T[a.off.elemQ + QIdx<P>::w(q)];
+      const double wq = T[a.off.elemG + QIdx<P>::w(q)];
       if constexpr (ADJ) {
         const double f = wq * ((g.B22 * u1q - g.B12 * u2q) * m1 + (g.B11 * u2q - g.B21 * u1q) * m2);
 #pragma unroll
-        for (int j = 0; j < N; ++j) acc[j] += T[a.off.elemQ + QIdx<P>::phi(q, j)] * f;
+        for (int j = 0; j < N; ++j) acc[j] += T[a.off.elemG + QIdx<P>::phi(q, j)] * f;
       }
       if constexpr (SENS_U) {
         const double wc = al * wq * cq;
@@ -1515,8 +1521,8 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, double* sTheta
         const double f2 = wc * (g.B11 * m2 - g.B12 * m1);
 #pragma unroll
         for (int l = 0; l < N; ++l) {
-          gu1[l] += T[a.off.elemQ + QIdx<P>::phi(q, l)] * f1;
-          gu2[l] += T[a.off.elemQ + QIdx<P>::phi(q, l)] * f2;
+          gu1[l] += T[a.off.elemG + QIdx<P>::phi(q, l)] * f1;
+          gu2[l] += T[a.off.elemG + QIdx<P>::phi(q, l)] * f2;
         }
       }
     }
@@ -1731,28 +1737,28 @@ template <int P>
 __device__ __forceinline__ void sens_elem_stage(const double* __restrict__ T, const TabOff& off, const Geo<P>& g,
                                                 const double (&mu)[Dims<P>::N], const double (&c)[Dims<P>::N],
                                                 double al, double (&gu1)[Dims<P>::N], double (&gu2)[Dims<P>::N]) {
-  constexpr int N = Dims<P>::N, RE = Dims<P>::RE;
+  constexpr int N = Dims<P>::N, RG = Dims<P>::RG;
 #pragma unroll
   for (int l = 0; l < N; ++l) gu1[l] = gu2[l] = 0.0;
-  constexpr int UQ = RE <= 6 ? RE : 1;
+  constexpr int UQ = RG <= 6 ? RG : 1;
 #pragma unroll UQ
-  for (int q = 0; q < RE; ++q) {
+  for (int q = 0; q < RG; ++q) {
     double m1 = 0.0, m2 = 0.0, cq = 0.0;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-      const double ph = T[off.elemQ + QIdx<P>::phi(q, j)];
-      m1 += mu[j] * T[off.elemQ + QIdx<P>::d1(q, j)];
-      m2 += mu[j] * T[off.elemQ + QIdx<P>::d2(q, j)];
+      const double ph = T[off.elemG + QIdx<P>::phi(q, j)];
+      m1 += mu[j] * T[off.elemG + QIdx<P>::d1(q, j)];
+      m2 += mu[j] * T[off.elemG + QIdx<P>::d2(q, j)];
       cq += c[j] * ph;
     }
-    const double wq = T[off.elemQ + QIdx<P>::w(q)];
+    const double wq = T[off.elemG + QIdx<P>::w(q)];
     const double wc = al * wq * cq;
     const double f1 = wc * (g.B22 * m1 - g.B21 * m2);
     const double f2 = wc * (g.B11 * m2 - g.B12 * m1);
 #pragma unroll
     for (int l = 0; l < N; ++l) {
-      gu1[l] += T[off.elemQ + QIdx<P>::phi(q, l)] * f1;
-      gu2[l] += T[off.elemQ + QIdx<P>::phi(q, l)] * f2;
+      gu1[l] += T[off.elemG + QIdx<P>::phi(q, l)] * f1;
+      gu2[l] += T[off.elemG + QIdx<P>::phi(q, l)] * f2;
     }
   }
 }
@@ -1998,7 +2004,7 @@ __global__ __launch_bounds__(BLOCK) void apply_kernel(Mesh m, int op, const doub
                                                       const double* __restrict__ u1, const double* __restrict__ u2,
                                                       const double* __restrict__ vnw, const double* __restrict__ tab,
                                                       TabOff off, double* __restrict__ y) {
-  constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
+  constexpr int N = Dims<P>::N, RG = Dims<P>::RG, Re = Dims<P>::Re;
   const int K = m.K;
   const int k = gtid();
   if (k >= K) return;
@@ -2010,10 +2016,10 @@ __global__ __launch_bounds__(BLOCK) void apply_kernel(Mesh m, int op, const doub
   }
   Geo<P> g;
   g.load(m.B, k, K);
-  const double* __restrict__ EQ = tab + off.elemQ;
+  const double* __restrict__ EQ = tab + off.elemG;  // the element term's rule
   if (op == OP_G || op == OP_G1 || op == OP_G2) {
     const bool w1 = op != OP_G2, w2 = op != OP_G1;
-    for (int q = 0; q < RE; ++q) {
+    for (int q = 0; q < RG; ++q) {
       double cq = 0.0, u1q = 0.0, u2q = 0.0;
 #pragma unroll
       for (int j = 0; j < N; ++j) {

@@ -13,6 +13,9 @@ struct Dims {
   static constexpr int RE = P == 0 ? 1 : P == 1 ? 3 : P == 2 ? 6 : P == 3 ? 12 : 64;
   static constexpr int RF = P == 0 ? 1 : P == 1 ? 4 : P == 2 ? 7 : P == 3 ? 49 : 81;
   static constexpr int Re = P + 1;  // points of quadRule1D(2P + 1)
+  // points of the element term's rule (RefTables::ruleG): quadRule2D(max(2P, 1))
+  // for P <= 3; at P = 4 a 28-point rule exact to degree 11
+  static constexpr int RG = P == 4 ? 28 : RE;
 };
 
 // Nonzero pattern of the change to the Taylor basis (bit j of row r set when
@@ -42,17 +45,23 @@ __host__ __device__ constexpr bool taylor_nz(bool inv, int p, int r, int j) {
 }
 
 // The stage kernel's element term on the matrix cores (FP64 16x16x4 MFMA)
-// when the element rule splits into 16-point blocks and the basis fits 16
-// rows: p = 4 (64 points, N = 15).  Lower degrees stay on the vector ALU,
-// where the padding to 16 would cost more than the unit saves.
+// when the basis fits 16 rows: p = 4 (N = 15; 28 points of the element
+// term's rule, in two blocks of 16 rows, the last four rows zero).  Lower
+// degrees stay on the vector ALU, where the padding to 16 would cost more
+// than the unit saves.
 inline constexpr bool mfma_elem(int p) { return p == 4; }
 template <int P>
 struct MfmaElem {
   static constexpr bool value = mfma_elem(P);
-  static constexpr int QB = Dims<P>::RE / 16;    // point blocks
+  static constexpr int RG = Dims<P>::RG;
+  static_assert(!value || RG % 4 == 0, "point count must fill whole 4-point steps");
+  static constexpr int QB = (RG + 15) / 16;       // 16-row point blocks
   static constexpr int S = (Dims<P>::N + 3) / 4;  // 4-wide blocks of basis functions
-  // table (doubles, lane index fastest): A[qb][s][lane], D1[qb][r][lane], D2[qb][r][lane]
-  static constexpr int offD1 = QB * S * 64, offD2 = offD1 + QB * 4 * 64, size = offD2 + QB * 4 * 64;
+  static constexpr int NT = RG / 4;               // 4-point steps of the second GEMM
+  // 4-point steps (accumulator registers) in use in point block qb
+  static constexpr int steps(int qb) { return RG - 16 * qb >= 16 ? 4 : (RG - 16 * qb) / 4; }
+  // table (doubles, lane index fastest): A[qb][s][lane], D1[4qb + t][lane], D2[4qb + t][lane]
+  static constexpr int offD1 = QB * S * 64, offD2 = offD1 + NT * 64, size = offD2 + NT * 64;
   // hand-over buffer of one wave: 16 rows x 64 elements
   static constexpr int xchg = 16 * 64;
 };
@@ -68,9 +77,11 @@ struct TabOff {
   int elemQ;                // per point q of quadRule2D(max(2p,1)): w_q, phi_j(q), d1phi_j(q), d2phi_j(q)
   int edgeQ;                // per edge n, point r: w_r, phi_j(gamma_n(s_r))
   int hatMinvR, FR, FinvR;  // N x N row-major (FR, FinvR: the Taylor change of basis and its inverse)
-  // operands of the MFMA element term (MfmaElem), per lane l (r = l % 16,
-  // k = l / 16): A[qb][s] = phi_(4s+k)(q = 16qb + r); D1/D2[qb][t] =
-  // w_q d1phi_r(q) / w_q d2phi_r(q) at q = 16qb + 4t + k (zero past N); -1 if unused
+  int elemG;                // elemQ's layout for the element term's rule (RefTables::ruleG)
+  // operands of the MFMA element term (MfmaElem) on that rule, per lane l
+  // (r = l % 16, k = l / 16): A[qb][s] = phi_(4s+k)(q = 16qb + r) (zero past
+  // RG); D1/D2[4qb + t] = w_q d1phi_r(q) / w_q d2phi_r(q) at q = 16qb + 4t + k
+  // (zero past N); -1 if unused
   int mfmaE;
   int total;
 };

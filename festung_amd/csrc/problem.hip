@@ -530,6 +530,18 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
             fail(FESTUNG_ERR_INTERNAL, "Taylor change of basis (%d, %d) outside its nonzero pattern", i, j);
         }
       o.elemQ = put(eq);
+      if (t.RG == t.RE) {
+        o.elemG = o.elemQ;
+      } else {
+        std::vector<double> eg;
+        for (int q = 0; q < t.RG; ++q) {
+          eg.push_back(t.ruleG.w[q]);
+          for (int j = 0; j < N; ++j) eg.push_back(t.phiG[q + j * t.RG]);
+          for (int m = 0; m < 2; ++m)
+            for (int j = 0; j < N; ++j) eg.push_back(t.gradG[q + j * t.RG + m * t.RG * N]);
+        }
+        o.elemG = put(eg);
+      }
       o.edgeQ = put(ed);
       o.hatMinvR = put(mi);
       o.FR = put(fr);
@@ -537,24 +549,26 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
       o.mfmaE = -1;
       if (mfma_elem(p)) {
         // lane-distributed operands of the MFMA element term (see TabOff::mfmaE)
-        const int QBn = RE / 16, S = (N + 3) / 4;
-        std::vector<double> mt(size_t(QBn) * (S + 8) * 64, 0.0);
+        const int RG = t.RG, QBn = (RG + 15) / 16, S = (N + 3) / 4, NT = RG / 4;
+        if (RG % 4 != 0 || RG != Dims<4>::RG) fail(FESTUNG_ERR_INTERNAL, "element-term rule of %d points", RG);
+        std::vector<double> mt(size_t(QBn * S + 2 * NT) * 64, 0.0);
         for (int qb = 0; qb < QBn; ++qb)
           for (int l = 0; l < 64; ++l) {
             const int r = l % 16, k = l / 16;
             for (int s = 0; s < S; ++s) {
               const int q = 16 * qb + r, j = 4 * s + k;
-              if (j < N) mt[(size_t(qb) * S + s) * 64 + l] = t.phiE[q + size_t(j) * RE];
+              if (j < N && q < RG) mt[(size_t(qb) * S + s) * 64 + l] = t.phiG[q + size_t(j) * RG];
             }
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 4 && 16 * qb + 4 * u < RG; ++u) {
               const int q = 16 * qb + 4 * u + k, i = r;
               if (i >= N) continue;
               const size_t d1 = size_t(QBn) * S * 64 + (size_t(qb) * 4 + u) * 64 + l;
-              const size_t d2 = d1 + size_t(QBn) * 4 * 64;
-              mt[d1] = t.ruleE.w[q] * t.gradE[q + size_t(i) * RE];
-              mt[d2] = t.ruleE.w[q] * t.gradE[q + size_t(i) * RE + size_t(RE) * N];
+              const size_t d2 = d1 + size_t(NT) * 64;
+              mt[d1] = t.ruleG.w[q] * t.gradG[q + size_t(i) * RG];
+              mt[d2] = t.ruleG.w[q] * t.gradG[q + size_t(i) * RG + size_t(RG) * N];
             }
           }
+        if (mt.size() != size_t(MfmaElem<4>::size)) fail(FESTUNG_ERR_INTERNAL, "MFMA element table size");
         o.mfmaE = put(mt);
       }
     }

@@ -477,6 +477,27 @@ RefTables build_ref_tables(int p) {
       for (int m = 0; m < 2; ++m)
         t.gradE[q + i * RE + m * RE * N] = grad_phi(i + 1, m + 1, t.ruleE.q1[q], t.ruleE.q2[q]);
     }
+  if (p == 4) {
+    static const double kRule11[28][3] = {
+#include "tri_rule11.inc"
+    };
+    for (const auto& r : kRule11) {
+      t.ruleG.q1.push_back(r[0]);
+      t.ruleG.q2.push_back(r[1]);
+      t.ruleG.w.push_back(r[2]);
+    }
+  } else {
+    t.ruleG = t.ruleE;
+  }
+  const int RG = t.RG = int(t.ruleG.w.size());
+  t.phiG.resize(size_t(RG) * N);
+  t.gradG.resize(size_t(RG) * N * 2);
+  for (int i = 0; i < N; ++i)
+    for (int q = 0; q < RG; ++q) {
+      t.phiG[q + i * RG] = phi(i + 1, t.ruleG.q1[q], t.ruleG.q2[q]);
+      for (int m = 0; m < 2; ++m)
+        t.gradG[q + i * RG + m * RG * N] = grad_phi(i + 1, m + 1, t.ruleG.q1[q], t.ruleG.q2[q]);
+    }
   t.phiF.resize(size_t(RF) * N);
   for (int i = 0; i < N; ++i)
     for (int q = 0; q < RF; ++q) t.phiF[q + i * RF] = phi(i + 1, t.ruleF.q1[q], t.ruleF.q2[q]);

@@ -36,6 +36,14 @@ struct RefTables {
   Rule1D ruleEdge;                 // quadRule1D(2p+1)
   int RE = 0, RF = 0, Re = 0;      // numbers of points
   std::vector<double> phiE, gradE; // phi2D{ordE} (RE x N), gradPhi2D{ordE} (RE x N x 2)
+  // Rule of the element term's integrals grad(phi_i) c_h u_h (degree 3p - 1):
+  // quadRule2D(ordE) for p <= 3, where that rule is not exact for them and
+  // the reference's rounding of the integrals is kept; at p = 4, where
+  // quadRule2D(8) integrates them exactly, the 28-point rule of
+  // tri_rule11.inc (exact to degree 11), which gives the same integrals.
+  Rule2D ruleG;
+  int RG = 0;
+  std::vector<double> phiG, gradG; // at ruleG's points (RG x N, RG x N x 2)
   std::vector<double> phiF;        // phi2D{ordF} (RF x N)
   std::vector<double> phi1D;       // phi1D{2p+1} (Re x N x 3)
   std::vector<double> thetaPhi1D;  // thetaPhi1D{2p+1} (Re x N x 3 x 3)

@@ -109,6 +109,10 @@ void festung_grid_destroy(festung_grid* g);
  *  "phi2D", "gradPhi2D", "phi1D", "thetaPhi1D" for order ord
  *                                     core/computeBasesOnQuad.m:72-134
  *  "taylorF", "taylorFinv" (N x N)    Taylor change-of-basis tables
+ *  "quadG"    (R x 3: Q1, Q2, W)      the element term's rule: quadRule2D(max(2p,1))
+ *                                     for p <= 3; at p = 4 a 28-point rule exact to
+ *                                     degree 11, which integrates the integrands of
+ *                                     hatG exactly, as quadRule2D(8) does
  * `ord` selects the quadrature order for the quad/phi tables (ignored
  * otherwise).  festung_ref_size gives the element count. */
 /* The orthonormal basis at reference points: out (npts x N, column-major)

@@ -118,6 +118,41 @@ def test_quadrature_and_tables_match_oracle():
             assert np.array_equal(ref_table(p, 'thetaPhi1D', ord_), bq['thetaPhi1D'][ord_].flatten(order='F'))
 
 
+def test_element_term_rule():
+    """The element term's rule: the reference's quadRule2D(max(2p, 1)) for
+    p <= 3; at p = 4 a fully symmetric 28-point rule with interior points and
+    positive weights that integrates every monomial of degree <= 11 exactly
+    (closed form a! b! / (a + b + 2)!) and fails at degree 12, and with which
+    the hatG block (integrands of degree 11) equals the oracle's
+    quadRule2D(8) block to rounding."""
+    from math import factorial
+    for p in range(4):
+        got = ref_table(p, 'quadG')
+        ref = np.concatenate(quadRule2D(max(2 * p, 1)))
+        assert np.array_equal(got, ref), p
+    x, y, w = ref_table(4, 'quadG').reshape(3, -1)
+    assert x.size == 28
+    assert (w > 0).all() and (x > 0).all() and (y > 0).all() and (x + y < 1).all()
+    bary = np.stack([1 - x - y, x, y], axis=1)
+    for perm in ([1, 0, 2], [0, 2, 1], [2, 1, 0]):
+        pb = bary[:, perm]
+        for r in range(28):
+            d = np.abs(bary - pb[r]).max(axis=1) + np.abs(w - w[r])
+            assert d.min() < 1e-15
+    for deg in range(13):
+        err = max(abs((w * x ** a * y ** (deg - a)).sum() - factorial(a) * factorial(deg - a) / factorial(deg + 2))
+                  * factorial(deg + 2) / (factorial(a) * factorial(deg - a)) for a in range(deg + 1))
+        assert (err < 2e-15) if deg <= 11 else (err > 1e-6), (deg, err)
+    N = 15
+    phi = np.array([OB.phi(i, x, y) for i in range(1, N + 1)])
+    grad = np.array([[OB.gradPhi(i, m, x, y) for m in (1, 2)] for i in range(1, N + 1)])
+    G28 = np.einsum('q,imq,jq,lq->ijlm', w, grad, phi, phi)
+    Gref = OB.integrateRefElemDphiPhiPhi(N, OB.computeBasesOnQuad(N))
+    # both sums round at about 1e-14 of the largest entry (terms of size ~50
+    # cancel); a wrong point or weight shows at 1e-4 and above
+    assert np.abs(G28 - Gref).max() <= 3e-14 * np.abs(Gref).max()
+
+
 @pytest.mark.parametrize('p', [1, 2, 3, 4])
 def test_taylor_tables(p):
     """F maps DG coefficients to [mean; reference derivatives at the centroid]:
