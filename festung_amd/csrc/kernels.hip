@@ -349,6 +349,32 @@ __device__ __forceinline__ void rk_combine_pre(const RKCoef& rk, double tau, con
   }
 }
 
+// Block mass solve  out = (det hatM)^-1 rhs  (det = 2|T|).  The basis is
+// orthonormal and quadRule2D(max(2p, 1)) integrates phi_i phi_j exactly, so
+// hatM is the identity up to the rounding of the rule: at p = 4, whose rule
+// (core/triquad.m) is computed to full precision, within 6.4e-15 (summation
+// rounding; the host checks it), and the product with hatM^-1 is skipped
+// (225 products per element whose table streamed through the scalar cache);
+// at p <= 3 the reference's 15-digit rule constants leave hatM off the
+// identity by up to 1.7e-14, and the product stays.
+template <int P>
+__device__ __forceinline__ void mass_solve(const double* __restrict__ minvR, const double (&rhs)[Dims<P>::N],
+                                           double idet, double (&out)[Dims<P>::N]) {
+  constexpr int N = Dims<P>::N;
+  if constexpr (P == 4) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = rhs[i] * idet;
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s += minvR[i * N + j] * rhs[j];
+      out[i] = s * idet;
+    }
+  }
+}
+
 // ----------------------------------------------------------------- stage --
 
 // Large element tables (p = 4) are staged in LDS: the scalar cache cannot
@@ -486,16 +512,15 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   const int kk = active ? k : 0;
   const double* __restrict__ T = a.tab;
 
-  // element term  (G1 + G2) c  in quadrature form on quadRule2D(max(2p,1)):
-  // on the matrix cores, into the wave's LDS buffer, before the edge term in
-  // even waves and after it in odd ones, so that the two waves sharing a SIMD
-  // tend to overlap one's MFMA work with the other's gathers; else in this
-  // lane's loop below.  Either way acc = edge term + element term in the end.
+  // element term  (G1 + G2) c  in quadrature form on the element term's
+  // rule: on the matrix cores, into the wave's LDS buffer, before the edge
+  // term; else in this lane's loop below.  Either way acc = edge term +
+  // element term in the end.  (With 38 MFMAs per 16 elements, staggering the
+  // two waves of a SIMD -- one forming the element term first, the other
+  // last -- no longer pays: 32.7 against 34.4 us at K = 131 072.)
   double c[N], acc[N];
   [[maybe_unused]] double* X = sXchg + (threadIdx.x >> 6) * MfmaElem<P>::xchg;
-  [[maybe_unused]] const bool mfmaFirst = ((k >> 6) & 1) == 0;  // wave-uniform
-  if constexpr (MF)
-    if (mfmaFirst) elem_term_mfma<P>(a, sM, X, k - (threadIdx.x & 63));
+  if constexpr (MF) elem_term_mfma<P>(a, sM, X, k - (threadIdx.x & 63));
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     c[i] = a.c[i * K + kk];
@@ -653,7 +678,6 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
     }
   }
   if constexpr (MF) {
-    if (!mfmaFirst) elem_term_mfma<P>(a, sM, X, k - (threadIdx.x & 63));
     __builtin_amdgcn_wave_barrier();  // X is written and read by this wave only
 #pragma unroll
     for (int i = 1; i < N; ++i) acc[i] += X[i * 64 + (threadIdx.x & 63)];
@@ -664,14 +688,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   }
   // block mass solve: cdot = (detJ hatM)^-1 rhs
   double cdot[N];
-  const double idet = 1.0 / g.det;
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    double s = 0.0;
-#pragma unroll
-    for (int j = 0; j < N; ++j) s += T[a.off.hatMinvR + i * N + j] * acc[j];
-    cdot[i] = s * idet;
-  }
+  mass_solve<P>(T + a.off.hatMinvR, acc, 1.0 / g.det, cdot);
 
   if constexpr (MODE == OUT_DOT_DG) {
     if (active)
@@ -1642,14 +1659,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, double* sTheta
   }
   if constexpr (!ADJ) return;
   double z[N];
-  const double idet = 1.0 / g.det;
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    double s = 0.0;
-#pragma unroll
-    for (int j = 0; j < N; ++j) s += T[a.off.hatMinvR + i * N + j] * acc[j];
-    z[i] = s * idet;
-  }
+  mass_solve<P>(T + a.off.hatMinvR, acc, 1.0 / g.det, z);
   if (!active) return;
   if (a.mode == 4) {
 #pragma unroll

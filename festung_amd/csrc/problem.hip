@@ -546,6 +546,13 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
       o.hatMinvR = put(mi);
       o.FR = put(fr);
       o.FinvR = put(fir);
+      if (p == 4) {
+        // the kernels' mass solve takes hatM as the identity at p = 4
+        for (int i = 0; i < N; ++i)
+          for (int j = 0; j < N; ++j)
+            if (std::fabs(t.hatM[i + size_t(j) * N] - (i == j ? 1.0 : 0.0)) > 1e-13)
+              fail(FESTUNG_ERR_INTERNAL, "hatM(%d, %d) off the identity", i, j);
+      }
       o.mfmaE = -1;
       if (mfma_elem(p)) {
         // lane-distributed operands of the MFMA element term (see TabOff::mfmaE)
