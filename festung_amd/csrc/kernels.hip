@@ -299,7 +299,8 @@ __device__ __forceinline__ void sink_taylor(const double (&Tc)[Dims<P>::N], bool
 // (rk_combine_pre below does the arithmetic; rk_combine first loads c0 and
 // the accumulator where they are used -- all of them before the first store:
 // accOut may be the accumulator itself, and loads placed after a store to it
-// would each wait alone)
+// would each wait alone).  c is read only where a1 != 0 (classic RK4 has
+// a1 = 0 at every stage, and its callers then leave c unloaded).
 template <int N>
 __device__ __forceinline__ void rk_combine_pre(const RKCoef& rk, double tau, const double (&c)[N],
                                                const double (&cdot)[N], const double (&c0v)[N],
@@ -328,8 +329,13 @@ __device__ __forceinline__ void rk_combine_pre(const RKCoef& rk, double tau, con
                                                const double (&cdot)[N], const double (&c0v)[N],
                                                const double (&accv)[N], double* __restrict__ accOut, int K, int kk,
                                                bool active, double (&nx)[N]) {
+  if (rk.a1 != 0.0) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) nx[i] = rk.a1 * c[i] + rk.b * tau * cdot[i];
+    for (int i = 0; i < N; ++i) nx[i] = rk.a1 * c[i] + rk.b * tau * cdot[i];
+  } else {  // classic RK4: c is not read
+#pragma unroll
+    for (int i = 0; i < N; ++i) nx[i] = rk.b * tau * cdot[i];
+  }
   if (rk.a0 != 0.0) {
 #pragma unroll
     for (int i = 0; i < N; ++i) nx[i] += rk.a0 * c0v[i];
@@ -877,8 +883,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
   [[maybe_unused]] double cs[RKOP ? N : 1], c0v[RKOP ? N : 1], accv[RKOP ? N : 1];
   if constexpr (RKOP) {
     const RKCoef& rk = a.rk;
+    if (rk.a1 != 0.0) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) cs[i] = a.c[i * K + kk];
+      for (int i = 0; i < N; ++i) cs[i] = a.c[i * K + kk];
+    }
     if (rk.a0 != 0.0 || (a.accOut && rk.d0 != 0.0)) {
 #pragma unroll
       for (int i = 0; i < N; ++i) c0v[i] = a.c0[i * K + kk];
@@ -1088,8 +1096,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
       rk_combine_pre<N>(a.rk, tk, cs, y, c0v, accv, a.accOut, K, kk, active, nx);
     } else {
       double cl[N];
+      if (a.rk.a1 != 0.0) {
 #pragma unroll
-      for (int i = 0; i < N; ++i) cl[i] = a.c[i * K + kk];
+        for (int i = 0; i < N; ++i) cl[i] = a.c[i * K + kk];
+      }
       rk_combine<N>(a.rk, tk, cl, y, a.c0, a.acc, a.accOut, K, kk, active, nx);
     }
     if constexpr (MODE == LIM_DOT_RK_DG) {
