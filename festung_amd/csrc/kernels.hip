@@ -166,6 +166,49 @@ __device__ __forceinline__ void taylor_to_dg(const double (&T)[Dims<P>::N], cons
   }
 }
 
+// The Taylor coefficients of degree <= 1 (mean and gradient) of dg_to_taylor,
+// with the same operations in the same order.
+template <int P>
+__device__ __forceinline__ void dg_to_taylor_lin(const double (&x)[Dims<P>::N], const Geo<P>& g,
+                                                 const double* __restrict__ F, double (&T)[3]) {
+  constexpr int N = Dims<P>::N;
+  double y[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if (taylor_nz(false, P, r, j)) s += F[r * N + j] * x[j];
+    y[r] = s;
+  }
+  const double s = 1.0 / (g.det * g.dX);
+  const double C11 = g.B22 * s, C12 = -g.B12 * s, C21 = -g.B21 * s, C22 = g.B11 * s;
+  T[0] = y[0];
+  apply_sym<1>(C11, C21, C12, C22, y + 1, T + 1);
+}
+
+// taylor_to_dg of a Taylor vector whose coefficients of degree >= 2 are zero
+// (the 'linear' limiter's output where it limits): the products with the
+// zero coefficients are left out, which leaves every sum unchanged (adding
+// c * 0 to a sum does not change it).
+template <int P>
+__device__ __forceinline__ void taylor_lin_to_dg(const double (&T)[3], const Geo<P>& g,
+                                                 const double* __restrict__ Finv, double (&x)[Dims<P>::N]) {
+  constexpr int N = Dims<P>::N;
+  double z[3];
+  const double A11 = g.dX * g.B11, A21 = g.dX * g.B21, A12 = g.dX * g.B12, A22 = g.dX * g.B22;
+  z[0] = T[0];
+  apply_sym<1>(A11, A21, A12, A22, T + 1, z + 1);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (taylor_nz(true, P, i, j)) s += Finv[i * N + j] * z[j];
+    x[i] = s;
+  }
+}
+
 // Element moments mu_(a,b) = mean over the element of xi^a eta^b, xi = dX
 // (x - x_c), degree by degree.  For the linear form z = s xi + t eta the mean
 // of z^D over a triangle is 2 D! / (D + 2)! h_D(z_0, z_1, z_2), with z_m its
@@ -701,6 +744,17 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
 #pragma unroll
       for (int i = 0; i < N; ++i) a.out[i * K + k] = cdot[i];
     return;
+  } else if constexpr (MODE == OUT_DOT_DG_CEN) {
+    // dc/dt in DG for limit_dg_kernel, with the min / max partials of its
+    // Taylor mean (dg_to_taylor's T_0 = F_00 x_0)
+    if (active)
+#pragma unroll
+      for (int i = 0; i < N; ++i) a.out[i * K + k] = cdot[i];
+    double Tc[N];
+    Tc[0] = T[a.off.FR] * cdot[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i) Tc[i] = 0.0;
+    sink_taylor<P>(Tc, active, a.sink, a.m.bs, elem_blocks(K, a.m.bs));
   } else if constexpr (MODE == OUT_DOT_TAYLOR) {
     double Tc[N];
     dg_to_taylor<P>(cdot, g, T + a.off.FR, Tc);
@@ -801,13 +855,14 @@ __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
     for (int u = 0; u < 8; ++u) id[u] = j0 + u < a.maxDeg ? a.v2tPad[(j0 + u) * a.numV + v] : -1;
     for (int f = 0; f < a.F; ++f) {
       const double* fld = a.fields + size_t(f) * a.K;
+      const double sc = f == 0 ? a.scale0 : 1.0;
       double m = j0 == 0 ? INFINITY : a.vmin[size_t(f) * a.numV + v];
       double M = j0 == 0 ? -INFINITY : a.vmax[size_t(f) * a.numV + v];
       // all gathers first (a padding slot reads element 0, masked below):
       // behind a per-slot branch each one waited for the one before
       double cv[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) cv[u] = fld[id[u] >= 0 ? id[u] : 0];
+      for (int u = 0; u < 8; ++u) cv[u] = sc * fld[id[u] >= 0 ? id[u] : 0];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         if (id[u] >= 0) {
@@ -822,6 +877,48 @@ __global__ __launch_bounds__(BLOCK) void vertex_minmax_kernel(VertexArgs a) {
 }
 
 // ----------------------------------------------------------------- limit --
+
+// Selective mass lumping of the dc/dt limiter (advection/solveSubStep.m:75-78):
+// lim += diag(MT)^-1 MT (orig - lim) on the Taylor coefficients, MT the
+// element's Taylor mass matrix.
+template <int P>
+__device__ __forceinline__ void lump_correct(const Geo<P>& g, const double (&x)[Dims<P>::N], double (&lim)[Dims<P>::N]) {
+  constexpr int N = Dims<P>::N;
+  // corr_i = f_i / (mu_2i - mu_i^2) * (sum_j mu_(i+j) d_j - mu_i sum_j mu_j d_j),
+  // d_j = (orig_j - lim_j) / f_j, f = a! b!: row i of diag(MT)^-1 MT applied
+  // to (orig - lim), with MT the element's Taylor mass matrix
+  double dj[N];
+  dj[0] = 0.0;
+#pragma unroll
+  for (int j = 1; j < N; ++j) dj[j] = (x[j] - lim[j]) * (1.0 / tfab(j));
+  // one pass over the moment degrees D = 0 .. 2p: the moments of degree
+  // <= p are kept (mlo); degree D adds mu_(i+j) d_j to the rows i with
+  // deg i + deg j = D and sets den_i = mu_(2i) - mu_i^2 at D = 2 deg i
+  double mlo[N], s[N], den[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) s[i] = den[i] = 0.0;
+  moment_degrees<2 * P>(g, [&](auto dc, const double* m) {
+    constexpr int D = decltype(dc)::value;
+    if constexpr (D <= P) {
+#pragma unroll
+      for (int b = 0; b <= D; ++b) mlo[D * (D + 1) / 2 + b] = m[b];
+    }
+    if constexpr (D >= 2) {
+#pragma unroll
+      for (int i = 1; i < N; ++i) {
+#pragma unroll
+        for (int j = 1; j < N; ++j)
+          if (tdeg(i) + tdeg(j) == D) s[i] += m[tb(i) + tb(j)] * dj[j];
+        if (2 * tdeg(i) == D) den[i] = m[2 * tb(i)] - mlo[i] * mlo[i];
+      }
+    }
+  });
+  double S0 = 0.0;
+#pragma unroll
+  for (int j = 1; j < N; ++j) S0 += mlo[j] * dj[j];
+#pragma unroll
+  for (int i = 1; i < N; ++i) lim[i] += (s[i] - mlo[i] * S0) * tfab(i) / den[i];
+}
 
 // Occupancy floor of the limiter at p = 3, 4 (waves per SIMD): the register
 // allocator needs the hint to stay within 168 / 256 registers there.
@@ -1044,42 +1141,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
 
   if constexpr (MODE != LIM_SOLUTION && P >= 2) {
     // selective mass lumping: lim + diag(MT)^-1 MT (orig - lim)
-    if (changed) {
-      // corr_i = f_i / (mu_2i - mu_i^2) * (sum_j mu_(i+j) d_j - mu_i sum_j mu_j d_j),
-      // d_j = (orig_j - lim_j) / f_j, f = a! b!: row i of diag(MT)^-1 MT applied
-      // to (orig - lim), with MT the element's Taylor mass matrix
-      double dj[N];
-      dj[0] = 0.0;
-#pragma unroll
-      for (int j = 1; j < N; ++j) dj[j] = (x[j] - lim[j]) * (1.0 / tfab(j));
-      // one pass over the moment degrees D = 0 .. 2p: the moments of degree
-      // <= p are kept (mlo); degree D adds mu_(i+j) d_j to the rows i with
-      // deg i + deg j = D and sets den_i = mu_(2i) - mu_i^2 at D = 2 deg i
-      double mlo[N], s[N], den[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) s[i] = den[i] = 0.0;
-      moment_degrees<2 * P>(g, [&](auto dc, const double* m) {
-        constexpr int D = decltype(dc)::value;
-        if constexpr (D <= P) {
-#pragma unroll
-          for (int b = 0; b <= D; ++b) mlo[D * (D + 1) / 2 + b] = m[b];
-        }
-        if constexpr (D >= 2) {
-#pragma unroll
-          for (int i = 1; i < N; ++i) {
-#pragma unroll
-            for (int j = 1; j < N; ++j)
-              if (tdeg(i) + tdeg(j) == D) s[i] += m[tb(i) + tb(j)] * dj[j];
-            if (2 * tdeg(i) == D) den[i] = m[2 * tb(i)] - mlo[i] * mlo[i];
-          }
-        }
-      });
-      double S0 = 0.0;
-#pragma unroll
-      for (int j = 1; j < N; ++j) S0 += mlo[j] * dj[j];
-#pragma unroll
-      for (int i = 1; i < N; ++i) lim[i] += (s[i] - mlo[i] * S0) * tfab(i) / den[i];
-    }
+    if (changed) lump_correct<P>(g, x, lim);
   }
   double y[N];
   taylor_to_dg<P>(lim, g, T + a.off.FinvR, y);
@@ -1115,6 +1177,139 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
       }
       sink_taylor<P>(Tc, active, a.sink, a.m.bs, elem_blocks(K, a.m.bs));
     }
+  }
+}
+
+// The 'linear' limiter on DG coefficients (LIM_DOT_RK_DG: dc/dt limiting,
+// RK update, new state in DG with its centroid min / max partials;
+// LIM_SOLUTION_DG: the state limited in place).  Its decision needs only the
+// Taylor coefficients of degree <= 1, which dg_to_taylor_lin forms with the
+// operations of dg_to_taylor; a limited element then takes limit_kernel's
+// path with the same operations, and an element the limiter keeps
+// stays as it was -- where the reference maps it to the Taylor basis and back
+// (core/applySlopeLimiterDisc.m:56-60), which returns it up to rounding.  So
+// the full changes of basis run only in waves that hold a limited element,
+// and the solution limiter stores only limited elements.
+template <int P, int MODE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min_waves<P>()))) void limit_dg_kernel(
+    LimitArgs a) {
+  static_assert(MODE == LIM_DOT_RK_DG || MODE == LIM_SOLUTION_DG, "DG-input modes");
+  constexpr int N = Dims<P>::N;
+  constexpr bool DOT = MODE == LIM_DOT_RK_DG;
+  const int K = a.m.K;
+  const int k = elem_index(K, a.m.bs);
+  const bool active = k < K;
+  const int kk = active ? k : 0;
+  const double* __restrict__ T = a.tab;
+
+  double xd[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) xd[i] = a.taylor[i * K + kk];
+  Geo<P> g;
+  g.load(a.m.B, kk, K);
+  // every load that does not depend on the limiting up front (as in limit_kernel)
+  int vtx[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) vtx[n] = a.m.V0T[n * K + kk];
+  double vlo[3], vhi[3], vdr[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    vlo[n] = a.vmin[vtx[n]];
+    vhi[n] = a.vmax[vtx[n]];
+  }
+  if (a.vertexD) {
+#pragma unroll
+    for (int n = 0; n < 3; ++n) vdr[n] = a.vertexD[vtx[n]];
+  }
+  constexpr bool RKOP = DOT && P <= 2;
+  [[maybe_unused]] double cs[RKOP ? N : 1], c0v[RKOP ? N : 1], accv[RKOP ? N : 1];
+  if constexpr (RKOP) {
+    const RKCoef& rk = a.rk;
+    if (rk.a1 != 0.0) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) cs[i] = a.c[i * K + kk];
+    }
+    if (rk.a0 != 0.0 || (a.accOut && rk.d0 != 0.0)) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) c0v[i] = a.c0[i * K + kk];
+    }
+    if (rk.g != 0.0 || (a.accOut && rk.d1 != 0.0)) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) accv[i] = a.acc[i * K + kk];
+    }
+  }
+
+  // core/applySlopeLimiterTaylorLinear.m:108-122 on the mean and gradient
+  double t1[3];
+  dg_to_taylor_lin<P>(xd, g, T + a.off.FR, t1);
+  const double ex = (g.B11 + g.B12) / 3.0, ey = (g.B21 + g.B22) / 3.0;
+  const double px[3] = {-ex, g.B11 - ex, g.B12 - ex};
+  const double py[3] = {-ey, g.B21 - ey, g.B22 - ey};
+  const double sNeg = fabs(a.gmm[1]) + 1.0;
+  const double sPos = fabs(a.gmm[0]) + 1.0;
+  double alpha = 1.0;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const double val = t1[0] + t1[1] * (px[n] * g.dX) + t1[2] * (py[n] * g.dX);
+    double lo = (vlo[n] - sNeg) + sNeg;
+    double hi = (vhi[n] + sPos) - sPos;
+    if (a.vertexD) {
+      const double dv = vdr[n];
+      lo = fmin(lo, dv);
+      hi = fmax(hi, dv);
+    }
+    alpha = fmin(alpha, vertex_alpha(val, t1[0], lo, hi));
+  }
+  const bool changed = alpha != 1.0;
+  if (a.alphaOut && active) a.alphaOut[k] = alpha;
+
+  if constexpr (!DOT) {
+    if (changed && active) {
+      const double lim[3] = {t1[0], alpha * t1[1], alpha * t1[2]};
+      double y[N];
+      taylor_lin_to_dg<P>(lim, g, T + a.off.FinvR, y);
+#pragma unroll
+      for (int i = 0; i < N; ++i) a.out[i * K + k] = y[i];
+    }
+  } else {
+    double y[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) y[i] = xd[i];
+    if (changed) {
+      // limit_kernel's path: all Taylor coefficients, the limited ones, the
+      // mass lumping, back to DG
+      double x[N], lim[N];
+      dg_to_taylor<P>(xd, g, T + a.off.FR, x);
+      lim[0] = x[0];
+      lim[1] = alpha * x[1];
+      lim[2] = alpha * x[2];
+#pragma unroll
+      for (int i = 3; i < N; ++i) lim[i] = 0.0 * x[i];
+      if constexpr (P >= 2) lump_correct<P>(g, x, lim);
+      taylor_to_dg<P>(lim, g, T + a.off.FinvR, y);
+    }
+    double nx[N];
+    const double tk = a.tauK ? a.tauK[kk] : a.tau;
+    if constexpr (RKOP) {
+      rk_combine_pre<N>(a.rk, tk, cs, y, c0v, accv, a.accOut, K, kk, active, nx);
+    } else {
+      double cl[N];
+      if (a.rk.a1 != 0.0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) cl[i] = a.c[i * K + kk];
+      }
+      rk_combine<N>(a.rk, tk, cl, y, a.c0, a.acc, a.accOut, K, kk, active, nx);
+    }
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) a.out[i * K + k] = nx[i];
+    }
+    // centroid (Taylor mean) min / max partials: dg_to_taylor's T_0 = F_00 x_0
+    double Tc[N];
+    Tc[0] = T[a.off.FR] * nx[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i) Tc[i] = 0.0;
+    sink_taylor<P>(Tc, active, a.sink, a.m.bs, elem_blocks(K, a.m.bs));
   }
 }
 
@@ -2204,6 +2399,7 @@ hipError_t launch_stage(int p, int mode, const StageArgs& a, hipStream_t s) {
     case OUT_RK_TAYLOR: L_STAGE1(OUT_RK_TAYLOR); break;
     case OUT_DOT_TAYLOR: L_STAGE1(OUT_DOT_TAYLOR); break;
     case OUT_DOT_DG: L_STAGE(OUT_DOT_DG); break;
+    case OUT_DOT_DG_CEN: L_STAGE1(OUT_DOT_DG_CEN); break;
     default: return hipErrorInvalidValue;
   }
 #undef L_STAGE
@@ -2236,12 +2432,22 @@ hipError_t launch_limit(int p, int mode, int type, const LimitArgs& a, hipStream
     case LT_STRICT: L_LIM(MODE_, LT_STRICT); break;     \
     default: return hipErrorInvalidValue;              \
   }
+#define L_LIM_DG(MODE_) \
+  FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((limit_dg_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
   switch (mode) {
     case LIM_SOLUTION: L_LIM_T(LIM_SOLUTION); break;
-    case LIM_DOT_RK_DG: L_LIM_T(LIM_DOT_RK_DG); break;
     case LIM_DOT_RK_TAYLOR: L_LIM_T(LIM_DOT_RK_TAYLOR); break;
+    case LIM_DOT_RK_DG:
+      if (type != LT_LINEAR || a.sink.F != 1) return hipErrorInvalidValue;
+      L_LIM_DG(LIM_DOT_RK_DG);
+      break;
+    case LIM_SOLUTION_DG:
+      if (type != LT_LINEAR || a.out != a.taylor) return hipErrorInvalidValue;
+      L_LIM_DG(LIM_SOLUTION_DG);
+      break;
     default: return hipErrorInvalidValue;
   }
+#undef L_LIM_DG
 #undef L_LIM
 #undef L_LIM_T
   return hipGetLastError();

@@ -127,12 +127,18 @@ struct Mesh {
 
 // Operators of festung_apply_operator (values of FESTUNG_OP_* in the C-ABI)
 enum OpKind { OP_G = 0, OP_G1 = 1, OP_G2 = 2, OP_R = 3, OP_M = 4, OP_MINV = 5 };
-enum StageOut { OUT_RK_DG = 0, OUT_RK_TAYLOR = 1, OUT_DOT_TAYLOR = 2, OUT_DOT_DG = 3 };
+// OUT_DOT_DG_CEN: dc/dt in DG plus the min / max partials of its Taylor mean
+// (the input of the 'linear' limiter on DG coefficients, limit_dg_kernel)
+enum StageOut { OUT_RK_DG = 0, OUT_RK_TAYLOR = 1, OUT_DOT_TAYLOR = 2, OUT_DOT_DG = 3, OUT_DOT_DG_CEN = 4 };
 enum AdjWhat { ADJ_STAGE = 0, ADJ_FUSED = 1, ADJ_SENS = 2, ADJ_SPLIT = 3, ADJ_SENS_U = 4, ADJ_SENS_V = 5 };
 // festung_adjoint's sweep with the sensitivities deferred to one pass per
 // checkpoint segment (launch_sens_segment); its stage kernels run ADJ_STAGE
 constexpr int ADJ_DEFER = 6;
-enum LimitMode { LIM_SOLUTION = 0, LIM_DOT_RK_DG = 1, LIM_DOT_RK_TAYLOR = 2 };
+// LIM_SOLUTION / LIM_DOT_RK_TAYLOR: Taylor input, any limiter type.
+// LIM_DOT_RK_DG / LIM_SOLUTION_DG: the 'linear' type on DG input
+// (limit_dg_kernel): dc/dt limiting + RK with the new state in DG and its
+// centroid partials, and the state limited in place (out == taylor).
+enum LimitMode { LIM_SOLUTION = 0, LIM_DOT_RK_DG = 1, LIM_DOT_RK_TAYLOR = 2, LIM_SOLUTION_DG = 3 };
 // Limiter types of core/applySlopeLimiterTaylor.m:69-85
 enum LimiterType { LT_LINEAR = 0, LT_HIERARCH = 1, LT_STRICT = 2 };
 // Taylor coefficients whose vertex-patch bounds a limiter type needs
@@ -168,6 +174,7 @@ struct VertexArgs {
   const double* fields;  // Taylor coefficients (N x K); the first F are reduced
   int K;
   int F;
+  double scale0;         // factor on field 0: 1, or F_00 when `fields` are DG coefficients (the Taylor mean)
   const double* partials;  // producer's per-block min / max, F x nPartials x 2
   int nPartials;
   double* gmm;             // out: global min, max per coefficient (2F, written by block 0)
@@ -177,7 +184,7 @@ struct VertexArgs {
 
 struct LimitArgs {
   Mesh m;
-  const double* taylor;  // Taylor coefficients to limit (N x K)
+  const double* taylor;  // coefficients to limit (N x K): Taylor, or DG in the DG-input modes
   const double* vmin;    // raw patch min / max per coefficient and vertex (F x numV)
   const double* vmax;
   const double* gmm;     // global min / max per coefficient (2F)
@@ -192,7 +199,7 @@ struct LimitArgs {
   double tau;
   const double* tauK;    // per-element time step or null
   RKCoef rk;
-  double* out;           // DG (LIM_SOLUTION, LIM_DOT_RK_DG) or Taylor (LIM_DOT_RK_TAYLOR)
+  double* out;           // DG (LIM_SOLUTION, the DG-input modes) or Taylor (LIM_DOT_RK_TAYLOR)
   MinMaxSink sink;
   double* alphaOut;      // K or null
 };

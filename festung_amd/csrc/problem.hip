@@ -352,8 +352,9 @@ struct festung_problem {
     HIPCHK(launch_edge_flux_weights(K, tabs.Re, area.d(), vn.d(), tab.d() + off.wEdge, vnw.d(), stream));
   }
 
-  void enqueue_vertex(const double* fields, int F = -1) {
-    VertexArgs va{numV, maxDeg, v2tPad.i(), fields, K, F < 0 ? nF : F, partials.d(), nblocks, gmm.d(), vmin.d(), vmax.d()};
+  void enqueue_vertex(const double* fields, int F = -1, double scale0 = 1.0) {
+    VertexArgs va{numV,   maxDeg, v2tPad.i(), fields,   K,        F < 0 ? nF : F,
+                  scale0, partials.d(), nblocks, gmm.d(), vmin.d(), vmax.d()};
     timed(FESTUNG_KERNEL_VERTEX, [&] { HIPCHK(launch_vertex(va, stream)); });
   }
 
@@ -402,6 +403,32 @@ struct festung_problem {
     l.off = off;
     l.sink = sink();
     l.alphaOut = alpha.d();
+    if (p >= 2 && limType == LT_LINEAR) {
+      // the same three steps on DG coefficients (limit_dg_kernel): dc/dt in
+      // T1, the new state written to `out` and limited there in place; the
+      // vertex passes reduce the Taylor means F_00 c_1
+      const double f00 = tabs.F[0];
+      a.out = T1.d();
+      timed(FESTUNG_KERNEL_STAGE, [&] { HIPCHK(launch_stage(p, OUT_DOT_DG_CEN, a, stream)); });
+      enqueue_vertex(T1.d(), -1, f00);
+      l.taylor = T1.d();
+      l.c = cin;
+      l.c0 = X.d();
+      l.acc = acc.d();
+      l.accOut = sp.accOut ? acc.d() : nullptr;
+      l.tau = tau;
+      l.tauK = hasTauK ? tauK.d() : nullptr;
+      l.rk = sp.rk;
+      l.out = cout;
+      l.alphaOut = nullptr;
+      timed(FESTUNG_KERNEL_LIMIT_DOT, [&] { HIPCHK(launch_limit(p, LIM_DOT_RK_DG, limType, l, stream)); });
+      enqueue_vertex(cout, -1, f00);
+      l.taylor = cout;
+      l.vertexD = hasVertexD ? vertexD.d() : nullptr;
+      l.alphaOut = keepAlpha ? alpha.d() : nullptr;
+      timed(FESTUNG_KERNEL_LIMIT, [&] { HIPCHK(launch_limit(p, LIM_SOLUTION_DG, limType, l, stream)); });
+      return;
+    }
     if (p >= 2) {
       // dc/dt limiting (advection/solveSubStep.m:69-79), then RK, then
       // solution limiting (:86-97)
@@ -1846,7 +1873,7 @@ int festung_vertex_based_limiter(festung_problem* h, const double* valCentroid, 
     mn.alloc(3 * size_t(K) * sizeof(double));
     mx.alloc(3 * size_t(K) * sizeof(double));
     HIPCHK(launch_centroid(K, h->bs, cenIn.d(), 1.0, cen.d(), MinMaxSink{part.d(), 1}, h->stream));
-    VertexArgs va{numV, h->maxDeg, h->v2tPad.i(), cen.d(), K, 1, part.d(), h->nblocks, gm.d(), vmn.d(), vmx.d()};
+    VertexArgs va{numV, h->maxDeg, h->v2tPad.i(), cen.d(), K, 1, 1.0, part.d(), h->nblocks, gm.d(), vmn.d(), vmx.d()};
     HIPCHK(launch_vertex(va, h->stream));
     HIPCHK(launch_vertex_limiter(h->mesh(), cen.d(), val.d(), hasD ? dD.d() : nullptr, vmn.d(), vmx.d(), gm.d(),
                                  al.d(), mn.d(), mx.d(), h->stream));
