@@ -746,12 +746,13 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
     return;
   } else if constexpr (MODE == OUT_DOT_DG_CEN) {
     // dc/dt in DG for limit_dg_kernel, with the min / max partials of its
-    // Taylor mean (dg_to_taylor's T_0 = F_00 x_0)
+    // Taylor mean (dg_to_taylor's T_0 = F_00 x_0; F_00 read before the
+    // stores, which would turn its scalar load into a waited vector load)
+    double Tc[N];
+    Tc[0] = T[a.off.FR] * cdot[0];
     if (active)
 #pragma unroll
       for (int i = 0; i < N; ++i) a.out[i * K + k] = cdot[i];
-    double Tc[N];
-    Tc[0] = T[a.off.FR] * cdot[0];
 #pragma unroll
     for (int i = 1; i < N; ++i) Tc[i] = 0.0;
     sink_taylor<P>(Tc, active, a.sink, a.m.bs, elem_blocks(K, a.m.bs));
@@ -1201,6 +1202,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
   const bool active = k < K;
   const int kk = active ? k : 0;
   const double* __restrict__ T = a.tab;
+  [[maybe_unused]] const double f00 = T[a.off.FR];  // F_00: the Taylor mean is F_00 x_0
 
   double xd[N];
 #pragma unroll
@@ -1261,7 +1263,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
     alpha = fmin(alpha, vertex_alpha(val, t1[0], lo, hi));
   }
   const bool changed = alpha != 1.0;
-  if (a.alphaOut && active) a.alphaOut[k] = alpha;
+  // (every store comes after the last table read: a store before them would
+  // turn the tables' scalar loads into vector loads, each waited for alone)
 
   if constexpr (!DOT) {
     if (changed && active) {
@@ -1271,6 +1274,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
 #pragma unroll
       for (int i = 0; i < N; ++i) a.out[i * K + k] = y[i];
     }
+    if (a.alphaOut && active) a.alphaOut[k] = alpha;
   } else {
     double y[N];
 #pragma unroll
@@ -1306,10 +1310,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
     }
     // centroid (Taylor mean) min / max partials: dg_to_taylor's T_0 = F_00 x_0
     double Tc[N];
-    Tc[0] = T[a.off.FR] * nx[0];
+    Tc[0] = f00 * nx[0];
 #pragma unroll
     for (int i = 1; i < N; ++i) Tc[i] = 0.0;
     sink_taylor<P>(Tc, active, a.sink, a.m.bs, elem_blocks(K, a.m.bs));
+    if (a.alphaOut && active) a.alphaOut[k] = alpha;
   }
 }
 
