@@ -32,6 +32,14 @@
 
 namespace festung {
 
+// The reference tables are read-only while any kernel runs.  Read through
+// the constant address space, their uniform loads stay scalar loads whatever
+// the kernel stores (or hides from the compiler) before them; through a plain
+// pointer a store earlier on the path turns them into vector loads.
+typedef __attribute__((address_space(4))) const double cdbl;
+__device__ __forceinline__ cdbl* const_tab(const double* p) { return (cdbl*)p; }
+
+
 constexpr int BLOCK = 256;
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
@@ -123,9 +131,9 @@ struct QIdx {
 // core/phiTaylorPhy.m: T_0 = mean, T_(a,b) = d_xi^a d_eta^b u at the centroid,
 // xi = dX (x - x_c)).  F maps DG coefficients to [mean; reference-coordinate
 // derivatives at the centroid].
-template <int P>
-__device__ __forceinline__ void dg_to_taylor(const double (&x)[Dims<P>::N], const Geo<P>& g,
-                                             const double* __restrict__ F, double (&T)[Dims<P>::N]) {
+template <int P, class TabPtr>
+__device__ __forceinline__ void dg_to_taylor(const double (&x)[Dims<P>::N], const Geo<P>& g, TabPtr F,
+                                             double (&T)[Dims<P>::N]) {
   constexpr int N = Dims<P>::N;
   double y[N];
 #pragma unroll
@@ -145,9 +153,9 @@ __device__ __forceinline__ void dg_to_taylor(const double (&x)[Dims<P>::N], cons
   if constexpr (P >= 4) apply_sym<4>(C11, C21, C12, C22, y + 10, T + 10);
 }
 
-template <int P>
-__device__ __forceinline__ void taylor_to_dg(const double (&T)[Dims<P>::N], const Geo<P>& g,
-                                             const double* __restrict__ Finv, double (&x)[Dims<P>::N]) {
+template <int P, class TabPtr>
+__device__ __forceinline__ void taylor_to_dg(const double (&T)[Dims<P>::N], const Geo<P>& g, TabPtr Finv,
+                                             double (&x)[Dims<P>::N]) {
   constexpr int N = Dims<P>::N;
   double z[N];
   const double A11 = g.dX * g.B11, A21 = g.dX * g.B21, A12 = g.dX * g.B12, A22 = g.dX * g.B22;
@@ -168,9 +176,9 @@ __device__ __forceinline__ void taylor_to_dg(const double (&T)[Dims<P>::N], cons
 
 // The Taylor coefficients of degree <= 1 (mean and gradient) of dg_to_taylor,
 // with the same operations in the same order.
-template <int P>
-__device__ __forceinline__ void dg_to_taylor_lin(const double (&x)[Dims<P>::N], const Geo<P>& g,
-                                                 const double* __restrict__ F, double (&T)[3]) {
+template <int P, class TabPtr>
+__device__ __forceinline__ void dg_to_taylor_lin(const double (&x)[Dims<P>::N], const Geo<P>& g, TabPtr F,
+                                                 double (&T)[3]) {
   constexpr int N = Dims<P>::N;
   double y[3];
 #pragma unroll
@@ -191,9 +199,9 @@ __device__ __forceinline__ void dg_to_taylor_lin(const double (&x)[Dims<P>::N], 
 // (the 'linear' limiter's output where it limits): the products with the
 // zero coefficients are left out, which leaves every sum unchanged (adding
 // c * 0 to a sum does not change it).
-template <int P>
-__device__ __forceinline__ void taylor_lin_to_dg(const double (&T)[3], const Geo<P>& g,
-                                                 const double* __restrict__ Finv, double (&x)[Dims<P>::N]) {
+template <int P, class TabPtr>
+__device__ __forceinline__ void taylor_lin_to_dg(const double (&T)[3], const Geo<P>& g, TabPtr Finv,
+                                                 double (&x)[Dims<P>::N]) {
   constexpr int N = Dims<P>::N;
   double z[3];
   const double A11 = g.dX * g.B11, A21 = g.dX * g.B21, A12 = g.dX * g.B12, A22 = g.dX * g.B22;
@@ -406,8 +414,8 @@ __device__ __forceinline__ void rk_combine_pre(const RKCoef& rk, double tau, con
 // (225 products per element whose table streamed through the scalar cache);
 // at p <= 3 the reference's 15-digit rule constants leave hatM off the
 // identity by up to 1.7e-14, and the product stays.
-template <int P>
-__device__ __forceinline__ void mass_solve(const double* __restrict__ minvR, const double (&rhs)[Dims<P>::N],
+template <int P, class TabPtr>
+__device__ __forceinline__ void mass_solve(TabPtr minvR, const double (&rhs)[Dims<P>::N],
                                            double idet, double (&out)[Dims<P>::N]) {
   constexpr int N = Dims<P>::N;
   if constexpr (P == 4) {
@@ -758,7 +766,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
     sink_taylor<P>(Tc, active, a.sink, a.m.bs, elem_blocks(K, a.m.bs));
   } else if constexpr (MODE == OUT_DOT_TAYLOR) {
     double Tc[N];
-    dg_to_taylor<P>(cdot, g, T + a.off.FR, Tc);
+    dg_to_taylor<P>(cdot, g, const_tab(T + a.off.FR), Tc);
     if (active) {
 #pragma unroll
       for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
@@ -779,7 +787,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
         for (int i = 0; i < N; ++i) a.out[i * K + k] = nx[i];
     } else {
       double Tc[N];
-      dg_to_taylor<P>(nx, g, T + a.off.FR, Tc);
+      dg_to_taylor<P>(nx, g, const_tab(T + a.off.FR), Tc);
       if (active) {
 #pragma unroll
         for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
@@ -1145,7 +1153,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
     if (changed) lump_correct<P>(g, x, lim);
   }
   double y[N];
-  taylor_to_dg<P>(lim, g, T + a.off.FinvR, y);
+  taylor_to_dg<P>(lim, g, const_tab(T + a.off.FinvR), y);
 
   if (a.alphaOut && active) a.alphaOut[k] = alpha;
   if constexpr (MODE == LIM_SOLUTION) {
@@ -1171,7 +1179,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
         for (int i = 0; i < N; ++i) a.out[i * K + k] = nx[i];
     } else {
       double Tc[N];
-      dg_to_taylor<P>(nx, g, T + a.off.FR, Tc);
+      dg_to_taylor<P>(nx, g, const_tab(T + a.off.FR), Tc);
       if (active) {
 #pragma unroll
         for (int i = 0; i < N; ++i) a.out[i * K + k] = Tc[i];
@@ -1243,7 +1251,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
 
   // core/applySlopeLimiterTaylorLinear.m:108-122 on the mean and gradient
   double t1[3];
-  dg_to_taylor_lin<P>(xd, g, T + a.off.FR, t1);
+  dg_to_taylor_lin<P>(xd, g, const_tab(T + a.off.FR), t1);
   const double ex = (g.B11 + g.B12) / 3.0, ey = (g.B21 + g.B22) / 3.0;
   const double px[3] = {-ex, g.B11 - ex, g.B12 - ex};
   const double py[3] = {-ey, g.B21 - ey, g.B22 - ey};
@@ -1270,7 +1278,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
     if (changed && active) {
       const double lim[3] = {t1[0], alpha * t1[1], alpha * t1[2]};
       double y[N];
-      taylor_lin_to_dg<P>(lim, g, T + a.off.FinvR, y);
+      taylor_lin_to_dg<P>(lim, g, const_tab(T + a.off.FinvR), y);
 #pragma unroll
       for (int i = 0; i < N; ++i) a.out[i * K + k] = y[i];
     }
@@ -1283,14 +1291,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
       // limit_kernel's path: all Taylor coefficients, the limited ones, the
       // mass lumping, back to DG
       double x[N], lim[N];
-      dg_to_taylor<P>(xd, g, T + a.off.FR, x);
+      dg_to_taylor<P>(xd, g, const_tab(T + a.off.FR), x);
       lim[0] = x[0];
       lim[1] = alpha * x[1];
       lim[2] = alpha * x[2];
 #pragma unroll
       for (int i = 3; i < N; ++i) lim[i] = 0.0 * x[i];
       if constexpr (P >= 2) lump_correct<P>(g, x, lim);
-      taylor_to_dg<P>(lim, g, T + a.off.FinvR, y);
+      taylor_to_dg<P>(lim, g, const_tab(T + a.off.FinvR), y);
     }
     double nx[N];
     const double tk = a.tauK ? a.tauK[kk] : a.tau;
@@ -1651,7 +1659,7 @@ __device__ __forceinline__ void adj_body(const AdjArgs& a, int k, double* sTheta
   constexpr bool SENS = SENS_U || SENS_V;
   constexpr int N = Dims<P>::N, RE = Dims<P>::RE, Re = Dims<P>::Re;
   const int K = a.m.K;
-  const double* __restrict__ T = a.tab;
+  cdbl* T = const_tab(a.tab);  // (scalar loads after the element's stores too)
   double mu[N], u1[N], u2[N], acc[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
@@ -1953,8 +1961,8 @@ __global__ __launch_bounds__(BLOCK) void adj_split_kernel(AdjArgs a) {
 // One adjoint stage's sensitivity contributions, formed from zero with the
 // operations of adj_body's SENS_U / SENS_V parts (so that adding them to the
 // running sums in the same order gives the same bits).
-template <int P>
-__device__ __forceinline__ void sens_elem_stage(const double* __restrict__ T, const TabOff& off, const Geo<P>& g,
+template <int P, class TabPtr>
+__device__ __forceinline__ void sens_elem_stage(TabPtr T, const TabOff& off, const Geo<P>& g,
                                                 const double (&mu)[Dims<P>::N], const double (&c)[Dims<P>::N],
                                                 double al, double (&gu1)[Dims<P>::N], double (&gu2)[Dims<P>::N]) {
   constexpr int N = Dims<P>::N, RG = Dims<P>::RG;
@@ -2009,7 +2017,10 @@ __global__ __launch_bounds__(BLOCK) void sens_seq_kernel(SensArgs a) {
   if (k >= K) return;
   const int len = a.count / a.nchunk;  // stages per chunk (one time step)
   const int e0 = a.first, e1 = a.first + a.count;
-  const double* __restrict__ T = a.tab;
+  // the tables through the constant address space: with the opaque offsets
+  // below (an asm statement, which the compiler takes as a possible memory
+  // write), plain loads became vector loads waited for in small batches
+  cdbl* T = const_tab(a.tab);
   Geo<P> g;
   g.load(a.m.B, k, K);
   const double tk = a.tauK ? a.tauK[k] : a.tau;
@@ -2094,7 +2105,7 @@ __global__ __launch_bounds__(BLOCK) void sens_seq_kernel(SensArgs a) {
       const double al = (1.0 - a.omegas[e]) * tk;
       int tz = 0;
       asm volatile("" : "+s"(tz));
-      const double* __restrict__ Te = T + tz;
+      cdbl* Te = T + tz;
       const double* thb = sTheta + tz;
       double mu[N], c[N];
 #pragma unroll
