@@ -36,21 +36,24 @@ METRIC = 'DoF-stage updates/s at p=2 + % HBM roofline; batched & adjoint vs plai
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6):
+def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6, lim_frac=1.0):
     """Algorithmic HBM bytes of all launches of one kernel kind in one time
     step (each datum read or written once; neighbour and vertex gathers count
-    once per element / vertex)."""
+    once per element / vertex).  lim_frac: the fraction of elements the
+    solution limiter changes (at p >= 2 it limits the state in place and
+    stores only those)."""
     N = (p + 1) * (p + 2) // 2
     Re = p + 1
     # velocity data the stage kernel reads per element: at p = 1 .. 3 its two
     # weighted factors at the RE points of quadRule2D(2p), at p = 4 (matrix
     # cores) the DG coefficients u1, u2
     NU = 2 * {0: 0, 1: 3, 2: 6, 3: 12}[p] if p <= 3 else 2 * N
-    # per stage: (reads c0, reads the RK4 accumulator, writes it)
-    rk = {1: [(0, 0, 0)], 2: [(0, 0, 0), (1, 0, 0)], 3: [(0, 0, 0), (1, 0, 0), (1, 0, 0)],
-          4: [(1, 0, 1), (1, 1, 1), (1, 1, 1), (0, 1, 0)]}[ordRK]
+    # per stage: (reads c0, reads the RK4 accumulator, writes it, reads the
+    # stage state c in the RK update: weight a1 != 0, not in classic RK4)
+    rk = {1: [(0, 0, 0, 1)], 2: [(0, 0, 0, 1), (1, 0, 0, 1)], 3: [(0, 0, 0, 1), (1, 0, 0, 1), (1, 0, 0, 1)],
+          4: [(1, 0, 1, 0), (1, 1, 1, 0), (1, 1, 1, 0), (0, 1, 0, 0)]}[ordRK]
     tot = 0.0
-    for (c0, ai, ao) in rk:
+    for (c0, ai, ao, a1) in rk:
         comb = N * (c0 + ai + ao)    # RK-combination operands besides c and the output
         if kind == 'stage':
             # edge flux factors |E| w_r u.nu: 3 Re; B: 4; nbr, flags: 16 B
@@ -61,10 +64,11 @@ def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6):
         elif kind == 'vertex' and lim:
             nv = 2 if p >= 2 else 1
             tot += nv * (4 * maxDeg * numV + 8 * K + 16 * numV)   # padded lists, centroids, bounds
-        elif kind == 'limit_dot' and lim and p >= 2:   # cdotT, B, c, RK operands -> Taylor
-            tot += K * (8 * (N + 4 + N + comb + N) + 12) + 16 * numV
+        elif kind == 'limit_dot' and lim and p >= 2:   # cdot, B, c (a1 != 0), RK operands -> new state
+            tot += K * (8 * (N + 4 + N * a1 + comb + N) + 12) + 16 * numV
         elif kind == 'limit' and lim:
-            tot += K * (8 * (N + 4 + N) + 12) + 16 * numV
+            # state and B read; p >= 2: only the limited elements stored
+            tot += K * (8 * (N + 4 + N * (lim_frac if p >= 2 else 1.0)) + 12) + 16 * numV
     return tot
 
 
@@ -319,11 +323,14 @@ def run_step(hmax, p, ordRK, lim, testcase, steps, warmup, kernel_steps, copy_mb
     prob.step(kernel_steps, tau)
     kt = prob.profile_read()
     prob.profile(False)
+    # fraction of elements the solution limiter changed in the last stage
+    # (it stores only those at p >= 2)
+    lim_frac = float(np.mean(prob.limiter_info()[0] < 1.0)) if lim else 0.0
     kernels = {}
     for kind, (ms, cnt) in kt.items():
         if cnt == 0:
             continue
-        per_step_bytes = kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg)
+        per_step_bytes = kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg, lim_frac)
         launches_per_step = cnt / kernel_steps
         avg_ms = ms / cnt
         bytes_per_launch = per_step_bytes / launches_per_step if launches_per_step else 0.0
@@ -335,7 +342,8 @@ def run_step(hmax, p, ordRK, lim, testcase, steps, warmup, kernel_steps, copy_mb
     for v in kernels.values():
         v['share'] = v['avg_us'] * v['launches_per_step'] / tot_ms
     dom = max(kernels, key=lambda k: kernels[k]['share'])
-    step_bytes = sum(kernel_bytes(k, p, K, numV, lim, ordRK, maxDeg) for k in ('stage', 'vertex', 'limit_dot', 'limit'))
+    step_bytes = sum(kernel_bytes(k, p, K, numV, lim, ordRK, maxDeg, lim_frac)
+                     for k in ('stage', 'vertex', 'limit_dot', 'limit'))
     secs = wall
     roof = dict(bound='hbm', kernel=dom, achieved=kernels[dom]['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
                 frac=kernels[dom]['GBs'] / HBM_PEAK_GBS, traffic=None,
@@ -371,7 +379,8 @@ def run_step(hmax, p, ordRK, lim, testcase, steps, warmup, kernel_steps, copy_mb
         roof['frac_of_copy'] = kernels[dom]['GBs'] / copy_GBs
     return dict(K=K, N=N, stages=nstage, tau=tau, value=K * N * nstage * steps / secs,
                 ms_per_step=1e3 * secs / steps, event_ms_per_step=ev_ms / steps, elem_block=best_bs,
-                autotune_ms_per_step=tune_ms, setup_s=t_setup, roofline=roof, kernels=kernels)
+                autotune_ms_per_step=tune_ms, setup_s=t_setup, roofline=roof, kernels=kernels,
+                limited_fraction=lim_frac)
 
 
 def traffic_from_profile(kernel, p, K):
@@ -474,7 +483,7 @@ def main():
         # the same p = 2 step at about 1M triangles (hmax 1/724: K = 1 048 352),
         # outside the 256 MB Infinity Cache
         r1 = run_step(1.0 / 724, p, ordRK, lim, a.testcase, 40, 5, a.kernel_steps, 0)
-        extra['step_K1048352'] = {k: r1[k] for k in ('K', 'ms_per_step', 'value', 'elem_block')}
+        extra['step_K1048352'] = {k: r1[k] for k in ('K', 'ms_per_step', 'value', 'elem_block', 'limited_fraction')}
         extra['step_K1048352'].update(
             step_GBs=r1['roofline']['step_GBs'], step_frac=r1['roofline']['step_frac'],
             stage_GBs=r1['kernels']['stage']['GBs'], stage_frac=r1['kernels']['stage']['GBs'] / HBM_PEAK_GBS,
@@ -519,7 +528,7 @@ def main():
                    'K': K, 'p': p, 'N': N, 'ordRK': ordRK, 'limiter': lim, 'tau': r['tau'],
                    'stages_per_step': nstage, 'event_ms_per_step': r['event_ms_per_step'],
                    'elem_block': r['elem_block'], 'autotune_ms_per_step': r['autotune_ms_per_step'],
-                   'setup_s': r['setup_s']},
+                   'setup_s': r['setup_s'], 'limited_fraction': r['limited_fraction']},
         'roofline': r['roofline'],
         'kernels': r['kernels'],
         'cpu_baseline': cpu,
