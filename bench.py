@@ -475,8 +475,9 @@ def main():
     r = run_step(a.hmax, p, ordRK, lim, a.testcase, a.steps, a.warmup, a.kernel_steps, a.copy_mb)
     K, N, nstage = r['K'], r['N'], r['stages']
     log = lambda m: print('[bench] %s' % m, file=sys.stderr, flush=True)
-    log('headline: %.4f ms/step, stage kernel %.1f us (%.3f of HBM peak)'
-        % (r['ms_per_step'], r['kernels'][r['roofline']['kernel']]['avg_us'], r['roofline']['frac']))
+    log('headline: %.4f ms/step, %s kernel %.1f us (%.3f of HBM peak); limited fraction %.4f'
+        % (r['ms_per_step'], r['roofline']['kernel'], r['kernels'][r['roofline']['kernel']]['avg_us'],
+           r['roofline']['frac'], r['limited_fraction']))
 
     extra = {}
     if not a.no_extras and a.config is None and a.hmax == 2.0 ** -9 and p == 2:
