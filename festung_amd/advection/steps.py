@@ -129,6 +129,9 @@ def preprocessProblem(pd):
     pd.hasNeumann = bool(np.any(pd.g.markE0TbdrN))
     pd.velocitySet = False
     pd.deviceEval = _device_points(pd) if pd.get('deviceEval', True) else None
+    if pd.isVisGrid:
+        # advection/preprocessProblem.m:54-59 (an image next to the solution files)
+        OUT.visualizeGrid(pd.g, pd.outputBasename + '_grid.png')
     return pd
 
 

@@ -4,6 +4,7 @@ and Tecplot (.plt) text formats, and hotstart dumps (.mat).
 * ``projectDataDisc2DataLagr``  core/projectDataDisc2DataLagr.m:52-72
 * ``visualizeDataLagr``         core/visualizeDataLagr.m:76-234 (triangles)
 * ``outputHotstart`` / ``readHotstart``  core/outputHotstart.m, core/readHotstart.m
+* ``visualizeGrid``             core/visualizeGrid.m:52-78 (an image file instead of a figure)
 
 The file layouts (headers, node order of quadratic cells, number formats)
 are the reference's, so its post-processing tools read these files as they
@@ -194,6 +195,61 @@ def visualizeDataLagr(g, dataLagr, varName, fileName, tLvl, fileTypes='vtk', vec
         else:
             raise ValueError('Unknown file type: %s' % t)
     return out
+
+
+def grid_labels(g):
+    """The labels core/visualizeGrid.m:52-78 draws, as (x1, x2, text, kind)
+    rows: local edge numbers 1..3 inside each triangle (vertex weights 1/12,
+    11/24, 11/24 with 1/12 at the opposite vertex), global vertex numbers at
+    the vertices, local vertex numbers (weights 5/6, 1/12, 1/12), global edge
+    numbers at the edge midpoints, triangle numbers at the centroids, and the
+    boundary ids of boundary edges, moved outward by |E| / 8 along the normal.
+    Numbers are 1-based, as the reference prints them."""
+    cx = np.asarray(g.coordV, dtype=float)
+    V0T = np.asarray(g.V0T)
+    X, Y = cx[V0T, 0], cx[V0T, 1]            # numT x 3
+    out = []
+
+    def add(x1, x2, vals, kind):
+        out.extend((float(a), float(b), str(int(c)), kind) for a, b, c in zip(x1, x2, vals))
+    wE = np.array([[1 / 12, 11 / 24, 11 / 24], [11 / 24, 1 / 12, 11 / 24], [11 / 24, 11 / 24, 1 / 12]])
+    for kE in range(3):
+        add(X @ wE[:, kE], Y @ wE[:, kE], np.full(g.numT, kE + 1), 'local edge')
+    add(cx[:, 0], cx[:, 1], np.arange(1, g.numV + 1), 'vertex')
+    wV = np.array([[5 / 6, 1 / 12, 1 / 12], [1 / 12, 5 / 6, 1 / 12], [1 / 12, 1 / 12, 5 / 6]])
+    for kV in range(3):
+        add(X @ wV[:, kV], Y @ wV[:, kV], np.full(g.numT, kV + 1), 'local vertex')
+    bE, bT = np.asarray(g.baryE), np.asarray(g.baryT)
+    add(bE[:, 0], bE[:, 1], np.arange(1, g.numE + 1), 'edge')
+    add(bT[:, 0], bT[:, 1], np.arange(1, g.numT + 1), 'triangle')
+    idE = np.asarray(g.idE)
+    m = idE != 0
+    nu, aE = np.asarray(g.nuE), np.asarray(g.areaE)
+    add(bE[m, 0] + nu[m, 0] * aE[m] / 8, bE[m, 1] + nu[m, 1] * aE[m] / 8, idE[m], 'boundary id')
+    return out
+
+
+def visualizeGrid(g, fileName='grid.png'):
+    """core/visualizeGrid.m:52-78: the triangles (black, white background,
+    equal aspect, no axes) and the labels of ``grid_labels`` in blue, written
+    to ``fileName`` (any format matplotlib writes: .png, .svg, .pdf) instead
+    of opened as a figure.  Returns the file name."""
+    import matplotlib
+    matplotlib.use('Agg')
+    import matplotlib.pyplot as plt
+    d = os.path.dirname(fileName)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    cx = np.asarray(g.coordV, dtype=float)
+    fig, ax = plt.subplots(figsize=(8, 8), facecolor='white')
+    ax.triplot(cx[:, 0], cx[:, 1], np.asarray(g.V0T), color='black', linewidth=0.6)
+    for x1, x2, t, _ in grid_labels(g):
+        ax.text(x1, x2, t, color='blue', fontsize=6, ha='center', va='center')
+    ax.set_aspect('equal')
+    ax.axis('off')
+    fig.savefig(fileName, dpi=150)
+    plt.close(fig)
+    return fileName
 
 
 def outputHotstart(filename, **variables):

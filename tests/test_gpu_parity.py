@@ -374,10 +374,10 @@ def test_main_writes_visualisation(gpu, tmp_path):
     from oracle import limiter as OL
     base = str(tmp_path / 'vis' / 'adv')
     kw = dict(testcase='solid_body', p=2, hmax=1 / 8, numSteps=4)
-    festung_amd.main('advection', dict(kw, isVisSol=True, outputFrequency=2, outputBasename=base,
+    festung_amd.main('advection', dict(kw, isVisSol=True, isVisGrid=True, outputFrequency=2, outputBasename=base,
                                        outputTypes=['vtk', 'tec']))
     names = sorted(p.name for p in (tmp_path / 'vis').iterdir())
-    assert names == ['adv.%d.%s' % (n, e) for n in (0, 2, 4) for e in ('plt', 'vtu')]
+    assert names == ['adv.%d.%s' % (n, e) for n in (0, 2, 4) for e in ('plt', 'vtu')] + ['adv_grid.png']
     o = OA.main('advection', dict(kw))
     want = OL.projectDataDisc2DataLagr(o.cDisc)[:, [0, 1, 2, 5, 3, 4]].ravel()
     import xml.etree.ElementTree as ET

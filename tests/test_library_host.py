@@ -352,3 +352,40 @@ def test_tuning_table_file_roundtrip(tmp_path):
         core.tune_load(str(tmp_path / 'missing.txt'))
     core.tune_clear()
     assert core.tune_count() == 0
+
+
+def test_visualize_grid(tmp_path):
+    """core/visualizeGrid.m: every label at its place, checked by geometry --
+    a local edge number n lies inside its triangle next to the midpoint of the
+    edge opposite local vertex n, a local vertex number next to that vertex,
+    global edge numbers on the edge midpoints (V0E), triangle numbers at the
+    centroids, boundary ids outside the domain -- and the image is written."""
+    from festung_amd import output as O
+    g = Grid.domainSquare(1 / 3)
+    lab = O.grid_labels(g)
+    kinds = {}
+    for x1, x2, t, k in lab:
+        kinds.setdefault(k, []).append((x1, x2, int(t)))
+    nb = int(np.sum(g.idE != 0))
+    assert {k: len(v) for k, v in kinds.items()} == {'local edge': 3 * g.numT, 'vertex': g.numV,
+                                                     'local vertex': 3 * g.numT, 'edge': g.numE,
+                                                     'triangle': g.numT, 'boundary id': nb}
+    V = g.coordV[g.V0T]                                  # numT x 3 x 2
+    for n in range(3):
+        le = np.array([(a, b) for a, b, t in kinds['local edge'][n * g.numT:(n + 1) * g.numT]])
+        assert all(t == n + 1 for _, _, t in kinds['local edge'][n * g.numT:(n + 1) * g.numT])
+        mids = (V[:, (n + 1) % 3] + V[:, (n + 2) % 3]) / 2
+        others = [(V[:, (m + 1) % 3] + V[:, (m + 2) % 3]) / 2 for m in range(3) if m != n]
+        d0 = np.linalg.norm(le - mids, axis=1)
+        assert all(np.all(d0 < np.linalg.norm(le - o, axis=1)) for o in others)
+        lv = np.array([(a, b) for a, b, _ in kinds['local vertex'][n * g.numT:(n + 1) * g.numT]])
+        dv = np.linalg.norm(lv - V[:, n], axis=1)
+        assert all(np.all(dv < np.linalg.norm(lv - V[:, m], axis=1)) for m in range(3) if m != n)
+    ev = np.array([(a, b) for a, b, _ in kinds['edge']])
+    assert np.allclose(ev, g.coordV[g.V0E].mean(axis=1))
+    assert [t for _, _, t in kinds['edge']] == list(range(1, g.numE + 1))
+    assert np.allclose(np.array([(a, b) for a, b, _ in kinds['triangle']]), V.mean(axis=1))
+    bid = np.array([(a, b) for a, b, _ in kinds['boundary id']])
+    assert np.all(np.any((bid < 0) | (bid > 1), axis=1))
+    fn = O.visualizeGrid(g, str(tmp_path / 'vis' / 'grid.png'))
+    assert open(fn, 'rb').read(8) == b'\x89PNG\r\n\x1a\n'
