@@ -478,11 +478,24 @@ __device__ __forceinline__ void fill_lds(double* __restrict__ dst, const double*
 // where the one-element-per-lane code picks it up.  Every lane of the wave
 // must call this (the MFMA is a wave operation).
 template <int P>
-__device__ __forceinline__ void elem_term_mfma(const StageArgs& a, const double* __restrict__ sM,
-                                               double* __restrict__ X, int e0) {
+__device__ __forceinline__ void elem_term_mfma(const StageArgs& a, double* __restrict__ X, int e0) {
   using M = MfmaElem<P>;
   constexpr int N = Dims<P>::N, S = M::S;
   const int K = a.m.K, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+  // this lane's table operands (22 values) in registers, read once per wave
+  // from the L2-resident table: no workgroup copy to LDS and no barrier
+  // before the first MFMA
+  const double* __restrict__ tm = a.tab + a.off.mfmaE;
+  double tA[M::QB][S], tD1[M::NT], tD2[M::NT];
+#pragma unroll
+  for (int qb = 0; qb < M::QB; ++qb)
+#pragma unroll
+    for (int s = 0; s < S; ++s) tA[qb][s] = tm[(qb * S + s) * 64 + l];
+#pragma unroll
+  for (int t = 0; t < M::NT; ++t) {
+    tD1[t] = tm[M::offD1 + t * 64 + l];
+    tD2[t] = tm[M::offD2 + t * 64 + l];
+  }
   // operands of group g in buffer g & 1; group g + 1's loads are issued
   // before group g's MFMAs
   double cB[2][S], u1B[2][S], u2B[2][S], Bg[2][4];
@@ -511,7 +524,7 @@ __device__ __forceinline__ void elem_term_mfma(const StageArgs& a, const double*
       dbl4 vc = {0.0, 0.0, 0.0, 0.0}, v1 = vc, v2 = vc;
 #pragma unroll
       for (int s = 0; s < S; ++s) {
-        const double ta = sM[(qb * S + s) * 64 + l];
+        const double ta = tA[qb][s];
         vc = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, cB[b][s], vc, 0, 0, 0);
         v1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, u1B[b][s], v1, 0, 0, 0);
         v2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ta, u2B[b][s], v2, 0, 0, 0);
@@ -521,8 +534,8 @@ __device__ __forceinline__ void elem_term_mfma(const StageArgs& a, const double*
         if (t >= M::steps(qb)) break;
         const double f1 = vc[t] * (B22 * v1[t] - B12 * v2[t]);
         const double f2 = vc[t] * (B11 * v2[t] - B21 * v1[t]);
-        r1 = __builtin_amdgcn_mfma_f64_16x16x4f64(sM[M::offD1 + (qb * 4 + t) * 64 + l], f1, r1, 0, 0, 0);
-        r2 = __builtin_amdgcn_mfma_f64_16x16x4f64(sM[M::offD2 + (qb * 4 + t) * 64 + l], f2, r2, 0, 0, 0);
+        r1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tD1[qb * 4 + t], f1, r1, 0, 0, 0);
+        r2 = __builtin_amdgcn_mfma_f64_16x16x4f64(tD2[qb * 4 + t], f2, r2, 0, 0, 0);
       }
     }
     const dbl4 r = r1 + r2;
@@ -537,14 +550,15 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   constexpr bool MF = MfmaElem<P>::value;
   __shared__ double sTheta[Re * N * 9];
   __shared__ double sQ[MF ? 1 : ElemTabInLDS<P>::size];
-  __shared__ double sM[MF ? MfmaElem<P>::size : 1];
   extern __shared__ double sXchg[];  // MF: MfmaElem<P>::xchg per wave (dynamic)
-  // At p <= 3 only the edge term reads LDS (the neighbour-trace table): its
-  // loads are issued first, the element's own loads behind them, and the
-  // table is written to LDS after those are issued, with a barrier that
-  // waits for LDS only -- so the element's loads stay in flight across it
-  // instead of starting after the workgroup's table fill.
-  constexpr bool kLateFill = !MF && !ElemTabInLDS<P>::value;
+  // Only the edge term reads LDS (the neighbour-trace table; at p = 4 the
+  // element term runs on the matrix cores with its table operands in
+  // registers): the table's loads are issued first, the element's own loads
+  // (and the matrix-core element term) behind them, and the table is written
+  // to LDS after those, with a barrier that waits for LDS only -- so the
+  // element's loads stay in flight across it instead of starting after the
+  // workgroup's table fill.
+  constexpr bool kLateFill = MF || !ElemTabInLDS<P>::value;
   constexpr int NTH = Re * N * 9, TPT = (NTH + 63) / 64;
   [[maybe_unused]] double thv[kLateFill ? TPT : 1];
   if constexpr (kLateFill) {
@@ -555,10 +569,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
     }
   } else {
     fill_lds(sTheta, a.tab + a.off.theta, NTH, a.m.bs);
-    if constexpr (MF)
-      fill_lds(sM, a.tab + a.off.mfmaE, MfmaElem<P>::size, a.m.bs);
-    else if constexpr (ElemTabInLDS<P>::value)
-      fill_lds(sQ, a.tab + a.off.elemQ, ElemTabInLDS<P>::size, a.m.bs);
+    if constexpr (ElemTabInLDS<P>::value) fill_lds(sQ, a.tab + a.off.elemQ, ElemTabInLDS<P>::size, a.m.bs);
     __syncthreads();
   }
   const double* __restrict__ EQ = ElemTabInLDS<P>::value ? sQ : a.tab + a.off.elemQ;
@@ -577,7 +588,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   // last -- no longer pays: 32.7 against 34.4 us at K = 131 072.)
   double c[N], acc[N];
   [[maybe_unused]] double* X = sXchg + (threadIdx.x >> 6) * MfmaElem<P>::xchg;
-  if constexpr (MF) elem_term_mfma<P>(a, sM, X, k - (threadIdx.x & 63));
+  if constexpr (MF) elem_term_mfma<P>(a, X, k - (threadIdx.x & 63));
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     c[i] = a.c[i * K + kk];
