@@ -1041,9 +1041,23 @@ int festung_limit(festung_problem* h) {
     h->set_device();
     if (!(h->flags & FESTUNG_FLAG_LIMITER)) fail(FESTUNG_ERR_STATE, "problem created without the limiter");
     h->finish_stage_sequence();
-    ToTaylorArgs ta{h->K, h->B.d(), h->X.d(), h->tab.d(), h->off, h->T2.d(), h->sink(), h->bs};
-    HIPCHK(launch_to_taylor(h->p, ta, h->stream));
-    h->enqueue_vertex(h->T2.d());
+    // 'linear' (p >= 1): on the DG coefficients in place, as in the step
+    // (limit_dg_kernel): an element the limiter keeps is left as it is, and a
+    // limited one is rebuilt from its Taylor mean and gradient.  The change
+    // of basis and back that the Taylor path applies to every element scales
+    // its terms of degree d with up to cond(B)^d, so on badly shaped elements
+    // at p = 4 it returned the coefficients with errors of 1e-10 (a sliver
+    // with cond(B) = 188), where the reference's mass-matrix solves keep them.
+    const bool dg = h->limType == LT_LINEAR && h->p >= 1;
+    if (dg) {
+      HIPCHK(launch_centroid(h->K, h->bs, h->X.d(), h->tabs.F[0], h->T2.d(), MinMaxSink{h->partials.d(), 1},
+                             h->stream));
+      h->enqueue_vertex(h->T2.d(), 1);
+    } else {
+      ToTaylorArgs ta{h->K, h->B.d(), h->X.d(), h->tab.d(), h->off, h->T2.d(), h->sink(), h->bs};
+      HIPCHK(launch_to_taylor(h->p, ta, h->stream));
+      h->enqueue_vertex(h->T2.d());
+    }
     LimitArgs l{};
     l.m = h->mesh();
     l.taylor = h->T2.d();
@@ -1056,7 +1070,8 @@ int festung_limit(festung_problem* h) {
     l.out = h->X.d();
     l.sink = h->sink();
     l.alphaOut = h->alpha.d();
-    HIPCHK(launch_limit(h->p, LIM_SOLUTION, h->limType, l, h->stream));
+    if (dg) l.taylor = h->X.d();
+    HIPCHK(launch_limit(h->p, dg ? LIM_SOLUTION_DG : LIM_SOLUTION, h->limType, l, h->stream));
     sync(h);
     h->cur = 0;
   });

@@ -407,10 +407,18 @@ def _delaunay(n, seed):
     return pts, tri
 
 
-@pytest.mark.parametrize('p,ordRK,lim', [(1, 2, True), (2, 3, True), (3, 3, False), (2, 3, False)])
+@pytest.mark.parametrize('p,ordRK,lim', [(1, 2, True), (2, 3, True), (3, 3, False), (2, 3, False), (3, 3, True),
+                                         (4, 4, True), (4, 3, False)])
 def test_unstructured_mesh_steps(gpu, p, ordRK, lim):
     """Random Delaunay meshes (irregular vertex degrees, every orientation of
-    the neighbour's local edge): four steps against the oracle."""
+    the neighbour's local edge): four steps against the oracle.  The p = 4
+    mesh holds slivers (cond(B) up to 1447); on limited ones the dc/dt
+    limiter's closed-form change to the Taylor basis and back rounds like
+    cond(B)^4 eps, where the reference's mass-matrix solves stay at eps, so
+    that case is held to 1e-10 per step (measured: 5.9e-12 after one step,
+    all of it on elements with cond(B) > 100 and their neighbours); without
+    the limiter, and for p <= 3, 1e-12."""
+    tol = 1e-10 if (p == 4 and lim) else TOL
     from festung_amd.core import Grid
     from oracle import grid as OG
     pts, tri = _delaunay(400, 7 + p)
@@ -440,7 +448,7 @@ def test_unstructured_mesh_steps(gpu, p, ordRK, lim):
     for n in range(1, 5):
         c = oracle_step(o, c, n)
         got = product_step(d, n)
-        assert rel_l2(got, c) < TOL * n, n
+        assert rel_l2(got, c) < tol * n, n
 
 
 def test_mid_size_step_parity(gpu):
