@@ -940,6 +940,228 @@ __device__ __forceinline__ void lump_correct(const Geo<P>& g, const double (&x)[
   for (int i = 1; i < N; ++i) lim[i] += (s[i] - mlo[i] * S0) * tfab(i) / den[i];
 }
 
+// ------------------------------------------------ double-double (fix-up) --
+// The dc/dt limiter's branch for limited elements takes every Taylor
+// coefficient through the closed-form change of basis and back, which
+// rounds like cond(B)^p eps, the rounding of the tables F and Finv
+// included.  On badly shaped elements (the host lists those with cond(B)
+// above a threshold; none on structured meshes) the same operations run in
+// double-double arithmetic (about 106 bits: Dekker's and Knuth's error-free
+// sum and product, Bailey's double-double; the products through FMA), with
+// F and Finv as double-double tables, in limit_dot_fixup_kernel, which hands
+// the limited dc/dt to the limiter.
+struct dd {
+  double hi, lo;
+};
+__device__ __forceinline__ dd mkdd(double h) { return dd{h, 0.0}; }
+__device__ __forceinline__ dd two_sum(double a, double b) {
+  const double s = a + b, v = s - a;
+  return dd{s, (a - (s - v)) + (b - v)};
+}
+__device__ __forceinline__ dd fast_two_sum(double a, double b) {
+  const double s = a + b;
+  return dd{s, b - (s - a)};
+}
+__device__ __forceinline__ dd operator+(const dd& a, const dd& b) {
+  const dd s = two_sum(a.hi, b.hi), t = two_sum(a.lo, b.lo);
+  double c = s.lo + t.hi;
+  const dd v = fast_two_sum(s.hi, c);
+  c = t.lo + v.lo;
+  return fast_two_sum(v.hi, c);
+}
+__device__ __forceinline__ dd operator-(const dd& a) { return dd{-a.hi, -a.lo}; }
+__device__ __forceinline__ dd operator-(const dd& a, const dd& b) { return a + (-b); }
+__device__ __forceinline__ dd operator*(const dd& a, const dd& b) {
+  const double p = a.hi * b.hi;
+  double e = __builtin_fma(a.hi, b.hi, -p);
+  e = __builtin_fma(a.hi, b.lo, e);
+  e = __builtin_fma(a.lo, b.hi, e);
+  return fast_two_sum(p, e);
+}
+__device__ __forceinline__ dd operator/(const dd& a, const dd& b) {
+  const double q1 = a.hi / b.hi;
+  dd r = a - b * mkdd(q1);
+  const double q2 = r.hi / b.hi;
+  r = r - b * mkdd(q2);
+  const double q3 = r.hi / b.hi;
+  return fast_two_sum(q1, q2) + mkdd(q3);
+}
+__device__ __forceinline__ dd dd_sqrt(const dd& a) {  // one Newton step from the double root
+  const double x = sqrt(a.hi);
+  return mkdd(x) + (a - mkdd(x) * mkdd(x)) / mkdd(2.0 * x);
+}
+
+template <int P>
+struct GeoDD {
+  double B11, B21, B12, B22;
+  dd det, dX;
+  __device__ __forceinline__ void load(const double* __restrict__ Bp, int k, int K) {
+    B11 = Bp[k];
+    B21 = Bp[k + K];
+    B12 = Bp[k + 2 * K];
+    B22 = Bp[k + 3 * K];
+    det = mkdd(B11) * mkdd(B22) - mkdd(B12) * mkdd(B21);
+    dX = mkdd(2.0) / dd_sqrt(det);
+  }
+};
+
+template <int D>
+__device__ __forceinline__ void apply_sym_dd(const dd& A11, const dd& A21, const dd& A12, const dd& A22, const dd* x,
+                                             dd* y) {
+#pragma unroll
+  for (int b = 0; b <= D; ++b) {
+    dd pw[D + 1];
+    pw[0] = mkdd(1.0);
+#pragma unroll
+    for (int t = 0; t < D; ++t) {
+      const dd c0 = t < D - b ? A21 : A22;
+      const dd c1 = t < D - b ? A11 : A12;
+      pw[t + 1] = pw[t] * c1;
+#pragma unroll
+      for (int i = t; i >= 1; --i) pw[i] = pw[i] * c0 + pw[i - 1] * c1;
+      pw[0] = pw[0] * c0;
+    }
+    dd s = mkdd(0.0);
+#pragma unroll
+    for (int bp = 0; bp <= D; ++bp) s = s + pw[D - bp] * x[bp];
+    y[b] = s;
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void dg_to_taylor_dd(const double (&x)[Dims<P>::N], const GeoDD<P>& g,
+                                                cdbl* F, cdbl* Flo,
+                                                dd (&T)[Dims<P>::N]) {
+  constexpr int N = Dims<P>::N;
+  dd y[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    dd s = mkdd(0.0);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if (taylor_nz(false, P, r, j)) s = s + dd{F[r * N + j], Flo[r * N + j]} * mkdd(x[j]);
+    y[r] = s;
+  }
+  const dd sc = mkdd(1.0) / (g.det * g.dX);
+  const dd C11 = mkdd(g.B22) * sc, C12 = -(mkdd(g.B12) * sc), C21 = -(mkdd(g.B21) * sc), C22 = mkdd(g.B11) * sc;
+  T[0] = y[0];
+  if constexpr (P >= 1) apply_sym_dd<1>(C11, C21, C12, C22, y + 1, T + 1);
+  if constexpr (P >= 2) apply_sym_dd<2>(C11, C21, C12, C22, y + 3, T + 3);
+  if constexpr (P >= 3) apply_sym_dd<3>(C11, C21, C12, C22, y + 6, T + 6);
+  if constexpr (P >= 4) apply_sym_dd<4>(C11, C21, C12, C22, y + 10, T + 10);
+}
+
+template <int P>
+__device__ __forceinline__ void taylor_to_dg_dd(const dd (&T)[Dims<P>::N], const GeoDD<P>& g,
+                                                cdbl* Finv, cdbl* Finvlo,
+                                                dd (&x)[Dims<P>::N]) {
+  constexpr int N = Dims<P>::N;
+  dd z[N];
+  const dd A11 = g.dX * mkdd(g.B11), A21 = g.dX * mkdd(g.B21), A12 = g.dX * mkdd(g.B12), A22 = g.dX * mkdd(g.B22);
+  z[0] = T[0];
+  if constexpr (P >= 1) apply_sym_dd<1>(A11, A21, A12, A22, T + 1, z + 1);
+  if constexpr (P >= 2) apply_sym_dd<2>(A11, A21, A12, A22, T + 3, z + 3);
+  if constexpr (P >= 3) apply_sym_dd<3>(A11, A21, A12, A22, T + 6, z + 6);
+  if constexpr (P >= 4) apply_sym_dd<4>(A11, A21, A12, A22, T + 10, z + 10);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    dd s = mkdd(0.0);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if (taylor_nz(true, P, i, j)) s = s + dd{Finv[i * N + j], Finvlo[i * N + j]} * z[j];
+    x[i] = s;
+  }
+}
+
+// tri_forms / h_next / moment_degrees in double-double
+struct TriFormsDD {
+  dd e2[3], e3[4];
+};
+template <int P>
+__device__ __forceinline__ TriFormsDD tri_forms_dd(const GeoDD<P>& g) {
+  const dd three = mkdd(3.0);
+  const dd ex = (mkdd(g.B11) + mkdd(g.B12)) / three, ey = (mkdd(g.B21) + mkdd(g.B22)) / three;
+  const dd X[3] = {-(ex * g.dX), (mkdd(g.B11) - ex) * g.dX, (mkdd(g.B12) - ex) * g.dX};
+  const dd Y[3] = {-(ey * g.dX), (mkdd(g.B21) - ey) * g.dX, (mkdd(g.B22) - ey) * g.dX};
+  TriFormsDD f;
+  f.e2[0] = X[0] * X[1] + X[0] * X[2] + X[1] * X[2];
+  f.e2[1] = X[0] * Y[1] + Y[0] * X[1] + X[0] * Y[2] + Y[0] * X[2] + X[1] * Y[2] + Y[1] * X[2];
+  f.e2[2] = Y[0] * Y[1] + Y[0] * Y[2] + Y[1] * Y[2];
+  const dd q0 = X[0] * X[1], q1 = X[0] * Y[1] + Y[0] * X[1], q2 = Y[0] * Y[1];
+  f.e3[0] = q0 * X[2];
+  f.e3[1] = q0 * Y[2] + q1 * X[2];
+  f.e3[2] = q1 * Y[2] + q2 * X[2];
+  f.e3[3] = q2 * Y[2];
+  return f;
+}
+template <int D>
+__device__ __forceinline__ void h_next_dd(const TriFormsDD& f, const dd* hm2, const dd* hm3, dd* h) {
+#pragma unroll
+  for (int k = 0; k <= D; ++k) {
+    dd s = mkdd(0.0);
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+      if (k - u >= 0 && k - u <= D - 2) s = s - f.e2[u] * hm2[k - u];
+    if constexpr (D >= 3) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (k - u >= 0 && k - u <= D - 3) s = s + f.e3[u] * hm3[k - u];
+    }
+    h[k] = s;
+  }
+}
+template <int D, int DMAX, typename F>
+__device__ __forceinline__ void moment_degrees_from_dd(const TriFormsDD& tf, const dd* hm3, const dd* hm2,
+                                                       const dd* hm1, F& fn) {
+  dd h[D + 1], m[D + 1];
+  h_next_dd<D>(tf, hm2, hm3, h);
+#pragma unroll
+  for (int b = 0; b <= D; ++b) m[b] = h[b] * (mkdd(2.0) / mkdd(double((D + 1) * (D + 2)) * binom(D, b)));
+  fn(Deg<D>{}, m);
+  if constexpr (D < DMAX) moment_degrees_from_dd<D + 1, DMAX>(tf, hm2, hm1, h, fn);
+}
+template <int DMAX, int P, typename F>
+__device__ __forceinline__ void moment_degrees_dd(const GeoDD<P>& g, F&& fn) {
+  const TriFormsDD tf = tri_forms_dd<P>(g);
+  const dd h0[1] = {mkdd(1.0)}, h1[2] = {mkdd(0.0), mkdd(0.0)};
+  fn(Deg<0>{}, h0);
+  if constexpr (DMAX >= 1) fn(Deg<1>{}, h1);
+  if constexpr (DMAX >= 2) moment_degrees_from_dd<2, DMAX>(tf, nullptr, h0, h1, fn);
+}
+// lump_correct in double-double
+template <int P>
+__device__ __forceinline__ void lump_correct_dd(const GeoDD<P>& g, const dd (&x)[Dims<P>::N], dd (&lim)[Dims<P>::N]) {
+  constexpr int N = Dims<P>::N;
+  dd dj[N];
+  dj[0] = mkdd(0.0);
+#pragma unroll
+  for (int j = 1; j < N; ++j) dj[j] = (x[j] - lim[j]) / mkdd(tfab(j));
+  dd mlo[N], s[N], den[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) s[i] = den[i] = mkdd(0.0);
+  moment_degrees_dd<2 * P>(g, [&](auto dc, const dd* m) {
+    constexpr int D = decltype(dc)::value;
+    if constexpr (D <= P) {
+#pragma unroll
+      for (int b = 0; b <= D; ++b) mlo[D * (D + 1) / 2 + b] = m[b];
+    }
+    if constexpr (D >= 2) {
+#pragma unroll
+      for (int i = 1; i < N; ++i) {
+#pragma unroll
+        for (int j = 1; j < N; ++j)
+          if (tdeg(i) + tdeg(j) == D) s[i] = s[i] + m[tb(i) + tb(j)] * dj[j];
+        if (2 * tdeg(i) == D) den[i] = m[2 * tb(i)] - mlo[i] * mlo[i];
+      }
+    }
+  });
+  dd S0 = mkdd(0.0);
+#pragma unroll
+  for (int j = 1; j < N; ++j) S0 = S0 + mlo[j] * dj[j];
+#pragma unroll
+  for (int i = 1; i < N; ++i) lim[i] = lim[i] + (s[i] - mlo[i] * S0) * mkdd(tfab(i)) / den[i];
+}
+
 // Occupancy floor of the limiter at p = 3, 4 (waves per SIMD): the register
 // allocator needs the hint to stay within 168 / 256 registers there.
 template <int P>
@@ -1210,7 +1432,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
 // (core/applySlopeLimiterDisc.m:56-60), which returns it up to rounding.  So
 // the full changes of basis run only in waves that hold a limited element,
 // and the solution limiter stores only limited elements.
-template <int P, int MODE>
+// FIX (LIM_DOT_RK_DG on a mesh with listed badly shaped elements): the
+// listed elements take their limited dc/dt from limit_dot_fixup_kernel.
+template <int P, int MODE, bool FIX = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min_waves<P>()))) void limit_dg_kernel(
     LimitArgs a) {
   static_assert(MODE == LIM_DOT_RK_DG || MODE == LIM_SOLUTION_DG, "DG-input modes");
@@ -1242,6 +1466,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
 #pragma unroll
     for (int n = 0; n < 3; ++n) vdr[n] = a.vertexD[vtx[n]];
   }
+  // badly shaped element whose limited dc/dt comes from the fix-up kernel
+  [[maybe_unused]] const int fi = FIX ? a.fixIdx[kk] : -1;
   constexpr bool RKOP = DOT && P <= 2;
   [[maybe_unused]] double cs[RKOP ? N : 1], c0v[RKOP ? N : 1], accv[RKOP ? N : 1];
   if constexpr (RKOP) {
@@ -1298,7 +1524,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
     double y[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) y[i] = xd[i];
-    if (changed) {
+    if (FIX && fi >= 0) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) y[i] = a.fixY[size_t(i) * a.fixCount + fi];
+    } else if (changed) {
       // limit_kernel's path: all Taylor coefficients, the limited ones, the
       // mass lumping, back to DG
       double x[N], lim[N];
@@ -1335,6 +1564,70 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
     sink_taylor<P>(Tc, active, a.sink, a.m.bs, elem_blocks(K, a.m.bs));
     if (a.alphaOut && active) a.alphaOut[k] = alpha;
   }
+}
+
+// The dc/dt limiter's limited dc/dt (limit_dg_kernel, LIM_DOT_RK_DG) of the
+// listed badly shaped elements: alpha with the limiter's own double
+// operations, then the full change of basis, the limited coefficients, the
+// mass lumping and the way back in double-double (see dd above), rounded
+// once at the end.  One thread per listed element.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void limit_dot_fixup_kernel(LimitArgs a, const int* __restrict__ list, int count,
+                                                               double* __restrict__ yOut) {
+  constexpr int N = Dims<P>::N;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int K = a.m.K;
+  const int k = list[t];
+  cdbl* T = const_tab(a.tab);
+  double xd[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) xd[i] = a.taylor[i * K + k];
+  Geo<P> g;
+  g.load(a.m.B, k, K);
+  double vlo[3], vhi[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const int v = a.m.V0T[n * K + k];
+    vlo[n] = a.vmin[v];
+    vhi[n] = a.vmax[v];
+  }
+  double t1[3];
+  dg_to_taylor_lin<P>(xd, g, T + a.off.FR, t1);
+  const double ex = (g.B11 + g.B12) / 3.0, ey = (g.B21 + g.B22) / 3.0;
+  const double px[3] = {-ex, g.B11 - ex, g.B12 - ex};
+  const double py[3] = {-ey, g.B21 - ey, g.B22 - ey};
+  const double sNeg = fabs(a.gmm[1]) + 1.0;
+  const double sPos = fabs(a.gmm[0]) + 1.0;
+  double alpha = 1.0;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const double val = t1[0] + t1[1] * (px[n] * g.dX) + t1[2] * (py[n] * g.dX);
+    const double lo = (vlo[n] - sNeg) + sNeg;
+    const double hi = (vhi[n] + sPos) - sPos;
+    alpha = fmin(alpha, vertex_alpha(val, t1[0], lo, hi));
+  }
+  double y[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) y[i] = xd[i];
+  if (alpha != 1.0) {
+    GeoDD<P> gd;
+    gd.load(a.m.B, k, K);
+    dd x[N], lim[N], yd[N];
+    dg_to_taylor_dd<P>(xd, gd, T + a.off.FR, T + a.off.FloR, x);
+    const dd al = mkdd(alpha);
+    lim[0] = x[0];
+    lim[1] = al * x[1];
+    lim[2] = al * x[2];
+#pragma unroll
+    for (int i = 3; i < N; ++i) lim[i] = mkdd(0.0);
+    if constexpr (P >= 2) lump_correct_dd<P>(gd, x, lim);
+    taylor_to_dg_dd<P>(lim, gd, T + a.off.FinvR, T + a.off.FinvloR, yd);
+#pragma unroll
+    for (int i = 0; i < N; ++i) y[i] = yd[i].hi + yd[i].lo;
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) yOut[size_t(i) * count + t] = y[i];
 }
 
 // ------------------------------------------------------ velocity weights --
@@ -2459,24 +2752,36 @@ hipError_t launch_limit(int p, int mode, int type, const LimitArgs& a, hipStream
     case LT_STRICT: L_LIM(MODE_, LT_STRICT); break;     \
     default: return hipErrorInvalidValue;              \
   }
-#define L_LIM_DG(MODE_) \
-  FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((limit_dg_kernel<PP, MODE_>), gr, dim3(BS), 0, s, a))
+#define L_LIM_DG(MODE_, FIX_) \
+  FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((limit_dg_kernel<PP, MODE_, FIX_>), gr, dim3(BS), 0, s, a))
   switch (mode) {
     case LIM_SOLUTION: L_LIM_T(LIM_SOLUTION); break;
     case LIM_DOT_RK_TAYLOR: L_LIM_T(LIM_DOT_RK_TAYLOR); break;
     case LIM_DOT_RK_DG:
-      if (type != LT_LINEAR || a.sink.F != 1) return hipErrorInvalidValue;
-      L_LIM_DG(LIM_DOT_RK_DG);
+      if (type != LT_LINEAR || a.sink.F != 1 || (a.fixIdx && (!a.fixY || a.fixCount <= 0)))
+        return hipErrorInvalidValue;
+      if (a.fixIdx) {
+        L_LIM_DG(LIM_DOT_RK_DG, true);
+      } else {
+        L_LIM_DG(LIM_DOT_RK_DG, false);
+      }
       break;
     case LIM_SOLUTION_DG:
       if (type != LT_LINEAR || a.out != a.taylor) return hipErrorInvalidValue;
-      L_LIM_DG(LIM_SOLUTION_DG);
+      L_LIM_DG(LIM_SOLUTION_DG, false);
       break;
     default: return hipErrorInvalidValue;
   }
 #undef L_LIM_DG
 #undef L_LIM
 #undef L_LIM_T
+  return hipGetLastError();
+}
+
+hipError_t launch_limit_dot_fixup(int p, const LimitArgs& a, const int* list, int count, double* y, hipStream_t s) {
+  if (p < 2 || count <= 0) return hipErrorInvalidValue;
+  FESTUNG_DISPATCH_P1(p, FESTUNG_LAUNCH((limit_dot_fixup_kernel<PP>), grid_for(count), dim3(BLOCK), 0, s, a, list,
+                                        count, y));
   return hipGetLastError();
 }
 

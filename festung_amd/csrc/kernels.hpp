@@ -77,6 +77,7 @@ struct TabOff {
   int elemQ;                // per point q of quadRule2D(max(2p,1)): w_q, phi_j(q), d1phi_j(q), d2phi_j(q)
   int edgeQ;                // per edge n, point r: w_r, phi_j(gamma_n(s_r))
   int hatMinvR, FR, FinvR;  // N x N row-major (FR, FinvR: the Taylor change of basis and its inverse)
+  int FloR, FinvloR;        // FR's and FinvR's remainders: hi + lo double-double tables
   int elemG;                // elemQ's layout for the element term's rule (RefTables::ruleG)
   // operands of the MFMA element term (MfmaElem) on that rule, per lane l
   // (r = l % 16, k = l / 16): A[qb][s] = phi_(4s+k)(q = 16qb + r) (zero past
@@ -202,6 +203,12 @@ struct LimitArgs {
   double* out;           // DG (LIM_SOLUTION, the DG-input modes) or Taylor (LIM_DOT_RK_TAYLOR)
   MinMaxSink sink;
   double* alphaOut;      // K or null
+  // LIM_DOT_RK_DG: the limited dc/dt of badly shaped elements, formed in
+  // double-double by limit_dot_fixup_kernel: fixIdx (K, -1 = none) indexes
+  // fixY (N x fixCount); null when the mesh has no such elements
+  const int* fixIdx;
+  const double* fixY;
+  int fixCount;
 };
 
 struct ToTaylorArgs {
@@ -307,6 +314,9 @@ hipError_t launch_vertex_limiter(const Mesh& m, const double* cen, const double*
 hipError_t launch_fill(size_t n, double v, double* x, hipStream_t s);
 hipError_t launch_vertex(const VertexArgs& a, hipStream_t s);
 hipError_t launch_limit(int p, int mode, int type, const LimitArgs& a, hipStream_t s);
+// the dc/dt limiter's result for the `count` listed elements (device ids) in
+// double-double into y (N x count), for LIM_DOT_RK_DG's fixIdx / fixY
+hipError_t launch_limit_dot_fixup(int p, const LimitArgs& a, const int* list, int count, double* y, hipStream_t s);
 hipError_t launch_to_taylor(int p, const ToTaylorArgs& a, hipStream_t s);
 // centroid values c_1 phi1c -> cen (K) and their min / max partials
 hipError_t launch_centroid(int K, int bs, const double* c, double phi1c, double* cen, const MinMaxSink& sink,

@@ -173,6 +173,11 @@ struct festung_problem {
   DBuf uq;   // the stage kernel's velocity factors at the element points (p = 1 .. 3), formed with u1, u2
   int limType = LT_LINEAR;  // limiter type (core/applySlopeLimiterTaylor.m:69-85)
   int nF = 1;               // Taylor coefficients with vertex-patch bounds (limiter_fields)
+  // badly shaped elements (cond(B)^p > 4096), whose limited dc/dt the
+  // double-double fix-up kernel forms ('linear' limiter, p >= 2): device ids,
+  // per-element index into fixY (-1: none), fixY N x fixCount
+  DBuf fixList, fixIdx, fixY;
+  int fixCount = 0;
   bool hasRhs = false, hasVertexD = false, hasVelocity = false;
   int velocityParts = 0;  // fields set by festung_set_velocity_disc: 1 u1Disc, 2 u2Disc, 4 vNormal
   int cur = 0;           // buffer holding the current state (0 = X)
@@ -412,6 +417,12 @@ struct festung_problem {
       timed(FESTUNG_KERNEL_STAGE, [&] { HIPCHK(launch_stage(p, OUT_DOT_DG_CEN, a, stream)); });
       enqueue_vertex(T1.d(), -1, f00);
       l.taylor = T1.d();
+      if (fixCount > 0) {
+        HIPCHK(launch_limit_dot_fixup(p, l, fixList.i(), fixCount, fixY.d(), stream));
+        l.fixIdx = fixIdx.i();
+        l.fixY = fixY.d();
+        l.fixCount = fixCount;
+      }
       l.c = cin;
       l.c0 = X.d();
       l.acc = acc.d();
@@ -535,6 +546,7 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
       // loop-ordered copies (see TabOff)
       const int RE = t.RE, Re = t.Re;
       std::vector<double> eq, ed, mi(size_t(N) * N), fr(size_t(N) * N), fir(size_t(N) * N);
+      std::vector<double> flo(size_t(N) * N), filo(size_t(N) * N);
       for (int q = 0; q < RE; ++q) {
         eq.push_back(t.ruleE.w[q]);
         for (int j = 0; j < N; ++j) eq.push_back(t.phiE[q + j * RE]);
@@ -551,6 +563,8 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
           mi[size_t(i) * N + j] = t.hatMinv[i + size_t(j) * N];
           fr[size_t(i) * N + j] = t.F[i + size_t(j) * N];
           fir[size_t(i) * N + j] = t.Finv[i + size_t(j) * N];
+          flo[size_t(i) * N + j] = t.Flo[i + size_t(j) * N];
+          filo[size_t(i) * N + j] = t.Finvlo[i + size_t(j) * N];
           // the kernels skip the products outside the compile-time pattern
           if ((fr[size_t(i) * N + j] != 0 && !taylor_nz(false, p, i, j)) ||
               (fir[size_t(i) * N + j] != 0 && !taylor_nz(true, p, i, j)))
@@ -573,6 +587,8 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
       o.hatMinvR = put(mi);
       o.FR = put(fr);
       o.FinvR = put(fir);
+      o.FloR = put(flo);
+      o.FinvloR = put(filo);
       if (p == 4) {
         // the kernels' mass solve takes hatM as the identity at p = 4
         for (int i = 0; i < N; ++i)
@@ -675,6 +691,31 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
       }
     h->upload(h->nbr, nb.data(), nb.size());
     h->upload(h->flg, fl.data(), fl.size());
+
+    // badly shaped elements: cond(B)^p above 4096, where the closed-form
+    // change to the Taylor basis and back can lose more than about 1e-12
+    // (none on the structured meshes: cond(B) <= 2.62 there)
+    if (p >= 2) {
+      std::vector<int32_t> list, idx(K, -1);
+      for (int32_t d = 0; d < K; ++d) {
+        const int32_t k = h->eperm[d];
+        const double a = g.B[k], c = g.B[k + size_t(K)], b = g.B[k + 2 * size_t(K)], e = g.B[k + 3 * size_t(K)];
+        const double S = a * a + b * b + c * c + e * e, det = std::fabs(a * e - b * c);
+        const double r = std::sqrt(std::max(S * S - 4.0 * det * det, 0.0));
+        const double smax = std::sqrt(0.5 * (S + r)), smin = det / smax;  // singular values of B
+        const double cond = smin > 0.0 ? smax / smin : INFINITY;
+        if (std::pow(cond, p) > 4096.0) {
+          idx[d] = int32_t(list.size());
+          list.push_back(d);
+        }
+      }
+      h->fixCount = int(list.size());
+      if (h->fixCount > 0) {
+        h->upload(h->fixList, list.data(), list.size());
+        h->upload(h->fixIdx, idx.data(), idx.size());
+        h->fixY.alloc(size_t(N) * h->fixCount * sizeof(double));
+      }
+    }
 
     // state and work buffers
     const size_t nk = size_t(N) * K * sizeof(double);

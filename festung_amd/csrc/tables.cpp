@@ -274,27 +274,38 @@ Rule2D quad_rule_2d(int qOrd) {
 }
 
 // One definition of the basis in Horner form, instantiated for double
-// (evaluation) and for Poly (exact monomial coefficients).
-struct Poly {
-  double c[5][5] = {};
-  Poly() = default;
-  explicit Poly(double v) { c[0][0] = v; }
+// (evaluation) and for PolyT (monomial coefficients: in double, and in
+// __float128 for the Taylor change of basis that the double-double limiter
+// path reads as hi + lo pairs).
+template <class S>
+struct PolyT {
+  using scalar = S;
+  S c[5][5] = {};
+  PolyT() = default;
+  explicit PolyT(S v) { c[0][0] = v; }
 };
-static Poly operator+(const Poly& a, const Poly& b) {
-  Poly r;
+using Poly = PolyT<double>;
+template <class S>
+using PS = typename PolyT<S>::scalar;  // a scalar operand: S is deduced from the polynomial
+template <class S>
+static PolyT<S> operator+(const PolyT<S>& a, const PolyT<S>& b) {
+  PolyT<S> r;
   for (int i = 0; i < 5; ++i)
     for (int j = 0; j < 5; ++j) r.c[i][j] = a.c[i][j] + b.c[i][j];
   return r;
 }
-static Poly operator-(const Poly& a) {
-  Poly r;
+template <class S>
+static PolyT<S> operator-(const PolyT<S>& a) {
+  PolyT<S> r;
   for (int i = 0; i < 5; ++i)
     for (int j = 0; j < 5; ++j) r.c[i][j] = -a.c[i][j];
   return r;
 }
-static Poly operator-(const Poly& a, const Poly& b) { return a + (-b); }
-static Poly operator*(const Poly& a, const Poly& b) {
-  Poly r;
+template <class S>
+static PolyT<S> operator-(const PolyT<S>& a, const PolyT<S>& b) { return a + (-b); }
+template <class S>
+static PolyT<S> operator*(const PolyT<S>& a, const PolyT<S>& b) {
+  PolyT<S> r;
   for (int i = 0; i < 5; ++i)
     for (int j = 0; j < 5; ++j)
       if (a.c[i][j] != 0)
@@ -302,16 +313,29 @@ static Poly operator*(const Poly& a, const Poly& b) {
           for (int l = 0; j + l < 5; ++l) r.c[i + k][j + l] += a.c[i][j] * b.c[k][l];
   return r;
 }
-static Poly operator+(double a, const Poly& b) { return Poly(a) + b; }
-static Poly operator+(const Poly& a, double b) { return a + Poly(b); }
-static Poly operator-(double a, const Poly& b) { return Poly(a) - b; }
-static Poly operator-(const Poly& a, double b) { return a - Poly(b); }
-static Poly operator*(double a, const Poly& b) { return Poly(a) * b; }
+template <class S>
+static PolyT<S> operator+(PS<S> a, const PolyT<S>& b) { return PolyT<S>(a) + b; }
+template <class S>
+static PolyT<S> operator+(const PolyT<S>& a, PS<S> b) { return a + PolyT<S>(b); }
+template <class S>
+static PolyT<S> operator-(PS<S> a, const PolyT<S>& b) { return PolyT<S>(a) - b; }
+template <class S>
+static PolyT<S> operator-(const PolyT<S>& a, PS<S> b) { return a - PolyT<S>(b); }
+template <class S>
+static PolyT<S> operator*(PS<S> a, const PolyT<S>& b) { return PolyT<S>(a) * b; }
 
-template <class T>
+static double sqrt_s(double x) { return std::sqrt(x); }
+static __float128 sqrt_s(__float128 x) {
+  // Newton's iteration r <- (r + x / r) / 2 from the double root: 53 -> 106 -> 212 bits
+  __float128 r = std::sqrt(double(x));
+  r = (r + x / r) / 2;
+  return (r + x / r) / 2;
+}
+
+template <class S, class T>
 static T phi_t(int i, const T& X1, const T& X2) {
-  const double S2 = std::sqrt(2.0), S3 = std::sqrt(3.0), S5 = std::sqrt(5.0), S6 = std::sqrt(6.0);
-  const double S10 = std::sqrt(10.0), S14 = std::sqrt(14.0), S30 = std::sqrt(30.0), S70 = std::sqrt(70.0);
+  const S S2 = sqrt_s(S(2)), S3 = sqrt_s(S(3)), S5 = sqrt_s(S(5)), S6 = sqrt_s(S(6));
+  const S S10 = sqrt_s(S(10)), S14 = sqrt_s(S(14)), S30 = sqrt_s(S(30)), S70 = sqrt_s(S(70));
   switch (i) {
     case 1: return S2 * (0.0 * X1 + 1.0);
     case 2: return 2 - 6 * X1;
@@ -338,15 +362,6 @@ static T phi_t(int i, const T& X1, const T& X2) {
 double phi(int i, double x1, double x2) {
   if (i == 1) return std::sqrt(2.0);  // sqrt(2)*ones(size(X1))
   return phi_t<double>(i, x1, x2);
-}
-
-void phi_monomials(int i, double coef[25]) {
-  Poly x1, x2;
-  x1.c[1][0] = 1;
-  x2.c[0][1] = 1;
-  Poly r = phi_t<Poly>(i, x1, x2);
-  for (int a = 0; a < 5; ++a)
-    for (int b = 0; b < 5; ++b) coef[a * 5 + b] = r.c[a][b];
 }
 
 double grad_phi(int i, int m, double X1, double X2) {
@@ -418,27 +433,31 @@ void theta_map(int nn, int np, double X1, double X2, double& y1, double& y2) {
   fail(FESTUNG_ERR_ARG, "theta: (%d, %d)", nn, np);
 }
 
-std::vector<double> invert(const std::vector<double>& A0, int n) {
-  std::vector<double> A(A0), X(size_t(n) * n, 0.0);
+template <class S>
+static S abs_s(S x) { return x < 0 ? -x : x; }
+
+template <class S>
+static std::vector<S> invert_t(const std::vector<S>& A0, int n) {
+  std::vector<S> A(A0), X(size_t(n) * n, S(0));
   for (int i = 0; i < n; ++i) X[i + i * n] = 1;
   for (int c = 0; c < n; ++c) {
     int piv = c;
     for (int r = c + 1; r < n; ++r)
-      if (std::fabs(A[r + c * n]) > std::fabs(A[piv + c * n])) piv = r;
+      if (abs_s(A[r + c * n]) > abs_s(A[piv + c * n])) piv = r;
     if (A[piv + c * n] == 0) fail(FESTUNG_ERR_INTERNAL, "singular matrix");
     if (piv != c)
       for (int k = 0; k < n; ++k) {
         std::swap(A[c + k * n], A[piv + k * n]);
         std::swap(X[c + k * n], X[piv + k * n]);
       }
-    double d = A[c + c * n];
+    S d = A[c + c * n];
     for (int k = 0; k < n; ++k) {
       A[c + k * n] /= d;
       X[c + k * n] /= d;
     }
     for (int r = 0; r < n; ++r)
       if (r != c) {
-        double f = A[r + c * n];
+        S f = A[r + c * n];
         if (f != 0)
           for (int k = 0; k < n; ++k) {
             A[r + k * n] -= f * A[c + k * n];
@@ -448,6 +467,8 @@ std::vector<double> invert(const std::vector<double>& A0, int n) {
   }
   return X;
 }
+
+std::vector<double> invert(const std::vector<double>& A0, int n) { return invert_t(A0, n); }
 
 static double fact(int n) {
   double r = 1;
@@ -552,27 +573,50 @@ RefTables build_ref_tables(int p) {
         }
   // Taylor change of basis: row 0 = mean (sqrt(2) e_1 by orthonormality),
   // row idx(a, b) = d1^a d2^b phi_j at the centroid (1/3, 1/3), Taylor order
-  // (1,0),(0,1),(2,0),(1,1),(0,2),...
-  t.F.assign(size_t(N) * N, 0.0);
-  t.F[0] = std::sqrt(2.0);
+  // (1,0),(0,1),(2,0),(1,1),(0,2),...  Computed in __float128 and kept as
+  // the nearest double (F, Finv) plus the remainder (Flo, Finvlo).
+  using Q = __float128;
+  std::vector<Q> Fq(size_t(N) * N, Q(0));
+  Fq[0] = sqrt_s(Q(2));
   for (int j = 0; j < N; ++j) {
-    double coef[25];
-    phi_monomials(j + 1, coef);
+    PolyT<Q> x1, x2;
+    x1.c[1][0] = 1;
+    x2.c[0][1] = 1;
+    const PolyT<Q> ph = phi_t<Q>(j + 1, x1, x2);
     for (int d = 1; d <= p; ++d)
       for (int b = 0; b <= d; ++b) {
         int a = d - b, row = d * (d + 1) / 2 + b;
-        double s = 0;
+        Q s = 0;
         for (int m = a; m < 5; ++m)
           for (int n = b; m + n < 5; ++n) {
-            double c = coef[m * 5 + n];
+            const Q c = ph.c[m][n];
             if (c == 0) continue;
-            s += c * (fact(m) / fact(m - a)) * (fact(n) / fact(n - b)) * std::pow(1.0 / 3, m - a) *
-                 std::pow(1.0 / 3, n - b);
+            Q third = 1;
+            for (int k = 0; k < m - a + n - b; ++k) third /= 3;
+            s += c * Q(fact(m) / fact(m - a)) * Q(fact(n) / fact(n - b)) * third;
           }
-        t.F[row + j * N] = s;
+        Fq[row + j * N] = s;
       }
   }
-  t.Finv = invert(t.F, N);
+  std::vector<Q> Fiq = invert_t(Fq, N);
+  // entries that vanish exactly keep a residue of about 1e-35 relative to
+  // the largest (the 1/3 powers and the elimination round in __float128)
+  for (std::vector<Q>* m : {&Fq, &Fiq}) {
+    Q big = 0;
+    for (Q v : *m) big = std::max(big, abs_s(v));
+    for (Q& v : *m)
+      if (abs_s(v) <= big * Q(1e-24)) v = 0;
+  }
+  t.F.resize(size_t(N) * N);
+  t.Flo.resize(size_t(N) * N);
+  t.Finv.resize(size_t(N) * N);
+  t.Finvlo.resize(size_t(N) * N);
+  for (size_t i = 0; i < size_t(N) * N; ++i) {
+    t.F[i] = double(Fq[i]);
+    t.Flo[i] = double(Fq[i] - Q(t.F[i]));
+    t.Finv[i] = double(Fiq[i]);
+    t.Finvlo[i] = double(Fiq[i] - Q(t.Finv[i]));
+  }
   return t;
 }
 

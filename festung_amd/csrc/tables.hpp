@@ -25,7 +25,6 @@ void gamma_map(int n, double s, double& x1, double& x2);
 void theta_map(int nn, int np, double x1, double x2, double& y1, double& y2);
 
 // Monomial coefficients of phi_i: coef[a*5 + b] multiplies x1^a x2^b.
-void phi_monomials(int i, double coef[25]);
 
 // Reference blocks and basis tables for polynomial degree p (all
 // column-major, Matlab index order).
@@ -54,8 +53,9 @@ struct RefTables {
   std::vector<double> hatRoff;     // N x N x 3 x 3 x Re, core/integrateRefEdgePhiIntPhiExtPerQuad.m
   // Taylor change of basis (limiter): F (N x N) maps DG coefficients to
   // [mean; reference-coordinate derivatives at the centroid], Finv its
-  // inverse.
-  std::vector<double> F, Finv;
+  // inverse, each the nearest double to the exact value; Flo, Finvlo the
+  // remainders (hi + lo is a double-double table).
+  std::vector<double> F, Finv, Flo, Finvlo;
 };
 
 RefTables build_ref_tables(int p);
