@@ -3,6 +3,8 @@ seeded inputs: projection, the DG right-hand side, single RK steps with and
 without the slope limiter, multi-step runs, the analytical 'transient' case
 with time-dependent source and Dirichlet data, limiter outputs, and
 run-to-run determinism.  Tolerance: 1e-12 relative L2 per step (FP64)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -454,6 +456,47 @@ def test_unstructured_mesh_steps(gpu, p, ordRK, lim):
         c = oracle_step(o, c, n)
         got = product_step(d, n)
         assert rel_l2(got, c) < tol * n, n
+
+
+def test_sliver_limited_dcdt_near_exact(gpu):
+    """Element 319 of the p = 4 Delaunay mesh above (cond(B) 105) is limited
+    with alpha = 0 in the first stage, where the dc/dt limiter's change of
+    basis and mass lumping are ill-conditioned (cond(M_T) 3.9e19).  Its
+    limited dc/dt, read back from the first RK4 stage (c_1 = c_0 + tau/2 y),
+    against the same limiter evaluated in 50-digit arithmetic
+    (tests/golden/make_sliver_exact.py): the double-double fix-up lands at
+    1.8e-9 (relative), the oracle at 2.5e-8; with F and Finv as plain doubles
+    the fix-up sat at 6.2e-9.  Held to 3e-9 and to a quarter of the oracle's
+    error."""
+    import json
+    from festung_amd.core import Grid
+    from oracle import grid as OG
+    fx = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'sliver_p4_exact.json')))
+    p, k = fx['p'], fx['element']
+    pts, tri = _delaunay(fx['points'], fx['seed'])
+    og = OG.generateGridData(pts, tri)
+    idE = (og.E0E[:, 1] < 0).astype(np.int32)
+    g = Grid.generateGridData(pts, tri)
+    g.set_boundary(idE)
+    tc = TC.rotating_humps()
+    kw = dict(p=p, hmax=0.05, ordRK=4, isSlopeLim=True, numSteps=4, tEnd=4 * 0.1 * 0.02 / (2 * p + 1))
+    d = S.Struct(testcase='custom', useGraph=False, **kw)
+    d.update(tc)
+    d = S.configureProblem(d)
+    d.generateGridData = lambda h: g
+    d = S.initializeProblem(S.preprocessProblem(d))
+    assert d.tau == fx['tau']
+    c0 = d.problem.get_state()
+    d.t, d.omega = S.rungeKuttaExplicit(d.ordRK, d.tau, 0.0)
+    d.problem.begin_step()
+    S.preprocessSubStep(d, 1, 1)
+    S.solveSubStep(d, 1, 1)
+    y = (d.problem.get_state()[k] - c0[k]) / (0.5 * d.tau)
+    ex = np.asarray(fx['limited_exact'])
+    err = np.linalg.norm(y - ex) / np.linalg.norm(ex)
+    print('element %d: product %.2e, oracle %.2e from the 50-digit value' % (k, err, fx['oracle_rel_error']))
+    assert err < 3e-9
+    assert err < fx['oracle_rel_error'] / 4
 
 
 def test_mid_size_step_parity(gpu):

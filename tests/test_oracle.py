@@ -4,6 +4,7 @@ operand) fails it, independently of the oracle's own formulas."""
 import json
 import math
 import os
+import sys
 
 import numpy as np
 import pytest
@@ -687,6 +688,29 @@ def test_limiter_types_vs_loop_evaluation(type_, p):
     assert np.max(np.abs(got - want)) < 1e-11
     assert np.array_equal(got[:, 0], t[:, 0])                # cell means unchanged
     assert np.all(np.abs(got) <= np.abs(t) + 1e-15)          # every coefficient scaled by a factor in [0, 1]
+
+
+# --------------------------------------------------- sliver, 50 digits --
+def test_sliver_exact_fixture():
+    """tests/golden/sliver_p4_exact.json (element 319 of the p = 4 Delaunay
+    mesh, cond(B) 105): the oracle still forms the fixture's first-stage
+    dc/dt and alpha = 0; the 50-digit limited dc/dt keeps the element mean
+    (degree >= 1 Taylor functions are mean-free and the lumping changes only
+    their coefficients, so y_0 = x_0 exactly); and the oracle's own limited
+    dc/dt lies 1e-9 .. 1e-7 (relative) from the 50-digit value: the
+    reference algorithm's rounding on this element, which bounds how close
+    any double implementation can agree with it there."""
+    import json
+    sys.path.insert(0, GOLDEN)
+    import make_sliver_exact as MS
+    fx = json.load(open(os.path.join(GOLDEN, 'sliver_p4_exact.json')))
+    pts, tri, x, alpha, y_or, tau = MS.oracle_stage1()
+    assert alpha == fx['alpha'] == 0.0 and tau == fx['tau']
+    assert np.linalg.norm(x - fx['dcdt']) <= 1e-13 * np.linalg.norm(x)
+    ex = np.asarray(fx['limited_exact'])
+    assert abs(ex[0] - fx['dcdt'][0]) <= 1e-15 * np.linalg.norm(ex)
+    err = np.linalg.norm(y_or - ex) / np.linalg.norm(ex)
+    assert 1e-9 < err < 1e-7, err
 
 
 # -------------------------------------------------------------------- EOC --
