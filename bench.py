@@ -67,8 +67,12 @@ def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6, lim_frac=1.0):
         elif kind == 'limit_dot' and lim and p >= 2:   # cdot, B, c (a1 != 0), RK operands -> new state
             tot += K * (8 * (N + 4 + N * a1 + comb + N) + 12) + 16 * numV
         elif kind == 'limit' and lim:
-            # state and B read; p >= 2: only the limited elements stored
-            tot += K * (8 * (N + 4 + N * (lim_frac if p >= 2 else 1.0)) + 12) + 16 * numV
+            # p >= 2 ('linear' on DG coefficients): the state read, no geometry,
+            # only the limited elements stored; p = 1: state and B read, all stored
+            if p >= 2:
+                tot += K * (8 * (N + N * lim_frac) + 12) + 16 * numV
+            else:
+                tot += K * (8 * (N + 4 + N) + 12) + 16 * numV
     return tot
 
 

@@ -217,6 +217,44 @@ __device__ __forceinline__ void taylor_lin_to_dg(const double (&T)[3], const Geo
   }
 }
 
+// The 'linear' limiter without geometry: the first-order Taylor polynomial
+// at the centroid, mean + grad f(x_c) . (x - x_c), is invariant under the
+// affine map of the element, so (i) its deviation at vertex n is
+// ghat . (vhat_n - chat), ghat the reference gradient at the reference
+// centroid chat = (1/3, 1/3), and (ii) its DG coefficients are Finv's first
+// three columns applied to (mean, ghat).  dg_to_taylor_lin / taylor_lin_to_dg
+// reach the same values through B^-1 / dX and dX B, which cancel.
+// ref_lin: (mean, ghat) = rows 0 .. 2 of F applied to the DG coefficients.
+template <int P, class TabPtr>
+__device__ __forceinline__ void ref_lin(const double (&x)[Dims<P>::N], TabPtr F, double (&y)[3]) {
+  constexpr int N = Dims<P>::N;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      if (taylor_nz(false, P, r, j)) s += F[r * N + j] * x[j];
+    y[r] = s;
+  }
+}
+// ghat . (vhat_n - chat) at the vertices (0, 0), (1, 0), (0, 1)
+__device__ __forceinline__ double ref_lin_dev(const double (&y)[3], int n) {
+  constexpr double third = 1.0 / 3.0;
+  return n == 0 ? -(y[1] + y[2]) * third : n == 1 ? (2.0 * y[1] - y[2]) * third : (2.0 * y[2] - y[1]) * third;
+}
+template <int P, class TabPtr>
+__device__ __forceinline__ void ref_lin_to_dg(const double (&z)[3], TabPtr Finv, double (&x)[Dims<P>::N]) {
+  constexpr int N = Dims<P>::N;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (taylor_nz(true, P, i, j)) s += Finv[i * N + j] * z[j];
+    x[i] = s;
+  }
+}
+
 // Element moments mu_(a,b) = mean over the element of xi^a eta^b, xi = dX
 // (x - x_c), degree by degree.  For the linear form z = s xi + t eta the mean
 // of z^D over a triangle is 2 D! / (D + 2)! h_D(z_0, z_1, z_2), with z_m its
@@ -1450,8 +1488,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
   double xd[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) xd[i] = a.taylor[i * K + kk];
-  Geo<P> g;
-  g.load(a.m.B, kk, K);
+  // the geometry only for the dc/dt limiter's full path (mass lumping)
+  [[maybe_unused]] Geo<P> g;
+  if constexpr (DOT) g.load(a.m.B, kk, K);
   // every load that does not depend on the limiting up front (as in limit_kernel)
   int vtx[3];
 #pragma unroll
@@ -1488,16 +1527,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
 
   // core/applySlopeLimiterTaylorLinear.m:108-122 on the mean and gradient
   double t1[3];
-  dg_to_taylor_lin<P>(xd, g, const_tab(T + a.off.FR), t1);
-  const double ex = (g.B11 + g.B12) / 3.0, ey = (g.B21 + g.B22) / 3.0;
-  const double px[3] = {-ex, g.B11 - ex, g.B12 - ex};
-  const double py[3] = {-ey, g.B21 - ey, g.B22 - ey};
+  ref_lin<P>(xd, const_tab(T + a.off.FR), t1);
   const double sNeg = fabs(a.gmm[1]) + 1.0;
   const double sPos = fabs(a.gmm[0]) + 1.0;
   double alpha = 1.0;
 #pragma unroll
   for (int n = 0; n < 3; ++n) {
-    const double val = t1[0] + t1[1] * (px[n] * g.dX) + t1[2] * (py[n] * g.dX);
+    const double val = t1[0] + ref_lin_dev(t1, n);
     double lo = (vlo[n] - sNeg) + sNeg;
     double hi = (vhi[n] + sPos) - sPos;
     if (a.vertexD) {
@@ -1515,7 +1551,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
     if (changed && active) {
       const double lim[3] = {t1[0], alpha * t1[1], alpha * t1[2]};
       double y[N];
-      taylor_lin_to_dg<P>(lim, g, const_tab(T + a.off.FinvR), y);
+      ref_lin_to_dg<P>(lim, const_tab(T + a.off.FinvR), y);
 #pragma unroll
       for (int i = 0; i < N; ++i) a.out[i * K + k] = y[i];
     }
@@ -1583,8 +1619,6 @@ __global__ __launch_bounds__(BLOCK) void limit_dot_fixup_kernel(LimitArgs a, con
   double xd[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) xd[i] = a.taylor[i * K + k];
-  Geo<P> g;
-  g.load(a.m.B, k, K);
   double vlo[3], vhi[3];
 #pragma unroll
   for (int n = 0; n < 3; ++n) {
@@ -1593,16 +1627,13 @@ __global__ __launch_bounds__(BLOCK) void limit_dot_fixup_kernel(LimitArgs a, con
     vhi[n] = a.vmax[v];
   }
   double t1[3];
-  dg_to_taylor_lin<P>(xd, g, T + a.off.FR, t1);
-  const double ex = (g.B11 + g.B12) / 3.0, ey = (g.B21 + g.B22) / 3.0;
-  const double px[3] = {-ex, g.B11 - ex, g.B12 - ex};
-  const double py[3] = {-ey, g.B21 - ey, g.B22 - ey};
+  ref_lin<P>(xd, T + a.off.FR, t1);
   const double sNeg = fabs(a.gmm[1]) + 1.0;
   const double sPos = fabs(a.gmm[0]) + 1.0;
   double alpha = 1.0;
 #pragma unroll
   for (int n = 0; n < 3; ++n) {
-    const double val = t1[0] + t1[1] * (px[n] * g.dX) + t1[2] * (py[n] * g.dX);
+    const double val = t1[0] + ref_lin_dev(t1, n);
     const double lo = (vlo[n] - sNeg) + sNeg;
     const double hi = (vhi[n] + sPos) - sPos;
     alpha = fmin(alpha, vertex_alpha(val, t1[0], lo, hi));
