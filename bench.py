@@ -56,16 +56,21 @@ def kernel_bytes(kind, p, K, numV, lim, ordRK, maxDeg=6, lim_frac=1.0):
     for (c0, ai, ao, a1) in rk:
         comb = N * (c0 + ai + ao)    # RK-combination operands besides c and the output
         if kind == 'stage':
-            # edge flux factors |E| w_r u.nu: 3 Re; B: 4; nbr, flags: 16 B
-            if lim and p >= 2:       # cdot -> Taylor: c, velocity, flux factors, B, nbr, flags -> cdotT
-                tot += K * (8 * (N + NU + 3 * Re + 4 + N) + 4 * 4)
-            else:                    # RK -> Taylor / DG
-                tot += K * (8 * (N + comb + NU + 3 * Re + 4 + N) + 16)
+            # edge flux factors |E| w_r u.nu: 3 Re; geometry: 1 / det at p <= 3 with a DG
+            # output (the rest of it is in the velocity weights and flux factors), B (4)
+            # at p = 4 and for Taylor output; nbr, flags: 16 B
+            if lim and p >= 2:       # cdot in DG + centroid partials: c, velocity, flux factors, geometry, nbr, flags
+                geo = 1 if p <= 3 else 4
+                tot += K * (8 * (N + NU + 3 * Re + geo + N) + 4 * 4)
+            else:                    # RK -> Taylor (p = 1 with the limiter) / DG
+                geo = 1 if (p <= 3 and not lim) else 4
+                tot += K * (8 * (N + comb + NU + 3 * Re + geo + N) + 16)
         elif kind == 'vertex' and lim:
             nv = 2 if p >= 2 else 1
             tot += nv * (4 * maxDeg * numV + 8 * K + 16 * numV)   # padded lists, centroids, bounds
-        elif kind == 'limit_dot' and lim and p >= 2:   # cdot, B, c (a1 != 0), RK operands -> new state
-            tot += K * (8 * (N + 4 + N * a1 + comb + N) + 12) + 16 * numV
+        elif kind == 'limit_dot' and lim and p >= 2:   # cdot, c (a1 != 0), RK operands -> new state
+            # (B, 32 B, is read only for the elements it limits: not counted, a lower bound)
+            tot += K * (8 * (N + N * a1 + comb + N) + 12) + 16 * numV
         elif kind == 'limit' and lim:
             # p >= 2 ('linear' on DG coefficients): the state read, no geometry,
             # only the limited elements stored; p = 1: state and B read, all stored

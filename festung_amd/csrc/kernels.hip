@@ -632,8 +632,17 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
     c[i] = a.c[i * K + kk];
     acc[i] = 0.0;
   }
-  Geo<P> g;
-  g.load(a.m.B, kk, K);
+  // the DG outputs at p <= 3 need only 1 / det (the element term's geometry
+  // is in the velocity weights, the edge term's in the flux factors)
+  constexpr bool kRdetOnly = !MF && (MODE == OUT_DOT_DG || MODE == OUT_DOT_DG_CEN || MODE == OUT_RK_DG);
+  [[maybe_unused]] Geo<P> g;
+  double rdet;
+  if constexpr (kRdetOnly) {
+    rdet = a.m.rdet[kk];
+  } else {
+    g.load(a.m.B, kk, K);
+    rdet = 1.0 / g.det;
+  }
   // RK operands (c_0, the accumulator) and the element's time step with the
   // element's other loads: read where they are combined, each load came
   // after the previous coefficient's store (the output may alias them) and
@@ -794,7 +803,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
   }
   // block mass solve: cdot = (detJ hatM)^-1 rhs
   double cdot[N];
-  mass_solve<P>(T + a.off.hatMinvR, acc, 1.0 / g.det, cdot);
+  mass_solve<P>(T + a.off.hatMinvR, acc, rdet, cdot);
 
   if constexpr (MODE == OUT_DOT_DG) {
     if (active)
@@ -1488,9 +1497,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
   double xd[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) xd[i] = a.taylor[i * K + kk];
-  // the geometry only for the dc/dt limiter's full path (mass lumping)
+  // the geometry only for the dc/dt limiter's full path (mass lumping), loaded there
   [[maybe_unused]] Geo<P> g;
-  if constexpr (DOT) g.load(a.m.B, kk, K);
   // every load that does not depend on the limiting up front (as in limit_kernel)
   int vtx[3];
 #pragma unroll
@@ -1567,6 +1575,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(limit_min
       // limit_kernel's path: all Taylor coefficients, the limited ones, the
       // mass lumping, back to DG
       double x[N], lim[N];
+      g.load(a.m.B, kk, K);
       dg_to_taylor<P>(xd, g, const_tab(T + a.off.FR), x);
       lim[0] = x[0];
       lim[1] = alpha * x[1];
@@ -2902,6 +2911,19 @@ __global__ __launch_bounds__(BLOCK) void permute_kernel(int K, int cols, const d
 
 hipError_t launch_permute(int K, int cols, const double* src, const int* idx, double* dst, hipStream_t s) {
   FESTUNG_LAUNCH(permute_kernel, grid_for(K), dim3(BLOCK), 0, s, K, cols, src, idx, dst);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(BLOCK) void rdet_kernel(int K, const double* __restrict__ B, double* __restrict__ rdet) {
+  const int k = gtid();
+  if (k >= K) return;
+  Geo<1> g;
+  g.load(B, k, K);
+  rdet[k] = 1.0 / g.det;
+}
+
+hipError_t launch_rdet(int K, const double* B, double* rdet, hipStream_t s) {
+  FESTUNG_LAUNCH(rdet_kernel, grid_for(K), dim3(BLOCK), 0, s, K, B, rdet);
   return hipGetLastError();
 }
 

@@ -124,6 +124,7 @@ struct Mesh {
   const int* flags;      // K
   const int* V0T;        // 3 x K (device vertex ids)
   int bs;                // elements per workgroup of the element kernels (64, 128 or 256)
+  const double* rdet;    // K: 1 / (B11 B22 - B12 B21), as the kernels form it (launch_rdet)
 };
 
 // Operators of festung_apply_operator (values of FESTUNG_OP_* in the C-ABI)
@@ -336,6 +337,8 @@ hipError_t launch_source(int p, const Mesh& m, const double* fq, const double* c
 hipError_t launch_l2(int p, bool ordF, int K, const double* B, const double* c, const double* cq, const double* tab,
                      const TabOff& off, double* partial, hipStream_t s);
 hipError_t launch_permute(int K, int cols, const double* src, const int* idx, double* dst, hipStream_t s);
+// rdet[k] = 1 / det(B_k) with Geo::load's operations (Mesh::rdet)
+hipError_t launch_rdet(int K, const double* B, double* rdet, hipStream_t s);
 int block_size();
 // profiling: events recorded at the start / end of the calling thread's next
 // kernels (null: plain launches)

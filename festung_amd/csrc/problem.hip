@@ -164,7 +164,7 @@ struct festung_problem {
   DBuf dEperm, dEinv, dVperm;
   DBuf devQ1, devQ2, devE1, devE2;  // device-order copies of point values given by device pointer
   std::vector<StagePlan> plan;
-  DBuf tab, B, area, nbr, flg, V0T, v2tPad, nuE0T;
+  DBuf tab, B, rdet, area, nbr, flg, V0T, v2tPad, nuE0T;
   int maxDeg = 0;
   DBuf tauK;
   bool hasTauK = false;
@@ -238,7 +238,7 @@ struct festung_problem {
 
   double* buf(int id) const { return id == 0 ? X.d() : id == 1 ? Y.d() : Z.d(); }
   Mesh mesh() const {
-    return Mesh{K, numV, B.d(), area.d(), nbr.i(), flg.i(), V0T.i(), bs};
+    return Mesh{K, numV, B.d(), area.d(), nbr.i(), flg.i(), V0T.i(), bs, rdet.d()};
   }
   MinMaxSink sink() const {
     return MinMaxSink{partials.d(), nF};
@@ -653,6 +653,8 @@ int festung_problem_create(const festung_grid* gh, int p, int ordRK, int flags, 
 
     // mesh, in device order
     h->upload_elem(h->B, g.B.data(), 4);
+    h->rdet.alloc(size_t(h->K) * sizeof(double));
+    HIPCHK(launch_rdet(h->K, h->B.d(), h->rdet.d(), h->stream));
     h->upload_elem(h->area, g.areaE0T.data(), 3);
     h->upload_elem(h->nuE0T, g.nuE0T.data(), 6);
     {
